@@ -1,0 +1,184 @@
+/*
+ * rtnn_hip.h -- C ABI of the MI355X (gfx950) fixed-radius / KNN neighbor search.
+ *
+ * Plain C types only: pointers, sizes, status codes.  No exception crosses this
+ * boundary and the library never calls exit()/abort().  A context is not
+ * thread-safe (one context per host thread); work is ordered on the context's
+ * own HIP stream.
+ *
+ * Result contract (every search entry point):
+ *   out_idx  int32 [nq x k], row-major, row = original query index,
+ *            values = original point index, ordered by (d2, index) ascending,
+ *            empty slots = RTNN_EMPTY_SLOT (-1, i.e. the reference's UINT_MAX
+ *            sentinel read as int32).
+ *   out_d2   float [nq x k] or NULL; +inf in empty slots.
+ *   d2 = ((dx*dx + dy*dy) + dz*dz) in FP32 without FMA, dx = q.x - p.x.
+ *   RTNN_MODE_RADIUS : include iff d2 < r*r (a query finds itself).
+ *   RTNN_MODE_KNN    : include iff 0 < d2 < r*r (self and duplicates excluded).
+ *   More than k qualifying points -> the k smallest by (d2, index).
+ *
+ * Reference seams replaced (paths relative to the reference's src/optixNSearch):
+ *   search() / launchSubframe()        search.cpp:9-61, optix.cpp:595-626
+ *   __raygen__radius/__raygen__knn     camera.cu:39-145
+ *   IS programs + insertTopKQ          geometry.cu:38-191
+ *   uploadData / computeMinMax         optix.cpp:117-168, sort.cpp:34-71
+ *   gridSort / sortParticles           sort.cpp:500-692, grid.cu:242-329
+ */
+#ifndef RTNN_HIP_H
+#define RTNN_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rtnn_ctx_s* rtnn_ctx;
+
+/* Status codes (0 = OK, negative = error class). */
+#define RTNN_OK               0
+#define RTNN_ERR_INVALID     -1   /* bad argument (NULL, k = 0, r <= 0, NaN ...) */
+#define RTNN_ERR_HIP         -2   /* a HIP runtime call failed */
+#define RTNN_ERR_OOM         -3   /* device allocation failed */
+#define RTNN_ERR_STATE       -4   /* call order (e.g. search before set_points) */
+#define RTNN_ERR_UNSUPPORTED -5   /* size/feature outside what the build supports */
+
+#define RTNN_MODE_RADIUS 0
+#define RTNN_MODE_KNN    1
+
+#define RTNN_EMPTY_SLOT (-1)
+
+/* Tuning knobs; zero means "choose automatically". */
+typedef struct {
+  float cell_factor;      /* grid cell edge = r / cell_factor                       */
+  int   queries_per_wave; /* range kernel lanes per query = 64 / queries_per_wave  */
+  int   query_order;      /* 0 auto, 1 input order, 2 Morton order of query cells  */
+  int   time_stages;      /* 1: record hipEvent timings into rtnn_stats             */
+  uint32_t max_cells;     /* cap on grid cells (cell edge grows to respect it)      */
+} rtnn_options;
+
+/* Filled by search entry points when a non-NULL pointer is passed. */
+typedef struct {
+  double ms_total;        /* wall time of the call on the context stream (events)  */
+  double ms_grid;         /* bbox + keys + radix sort + scan + gather              */
+  double ms_qsort;        /* query Morton keys + sort + gather                      */
+  double ms_search;       /* search kernel(s) only                                  */
+  double ms_mlp;          /* learned radius: features + MLP inference               */
+  double ms_fallback;     /* learned radius: re-run of short queries at r          */
+  uint64_t fallback_queries; /* queries re-run at r (learned search)               */
+  uint32_t grid_dims[3];
+  float    cell_size;
+  uint32_t sort_passes;   /* LSD radix passes of the point sort                     */
+  uint32_t search_launches;
+  uint32_t lanes_per_query; /* range kernel group width actually used               */
+} rtnn_stats;
+
+/* ---- context ---------------------------------------------------------- */
+
+/* Replaces setDevice()+setupOptiX() (main.cpp:9-24, optix.cpp:679-691). */
+int rtnn_create(int device, rtnn_ctx* out);
+int rtnn_destroy(rtnn_ctx ctx);
+/* Message of the last failed call on ctx (never NULL). */
+const char* rtnn_last_error(rtnn_ctx ctx);
+/* Message of the last failed rtnn_create on this thread. */
+const char* rtnn_create_error(void);
+int rtnn_set_options(rtnn_ctx ctx, const rtnn_options* opt);
+int rtnn_synchronize(rtnn_ctx ctx);
+
+/* ---- points ----------------------------------------------------------- */
+
+/* Copies n float3 (xyz AoS, 12 B each) into the context.  Replaces
+ * uploadData()'s point upload (optix.cpp:117-124).  on_device != 0 means xyz
+ * is a device pointer.  Non-finite coordinates are rejected (RTNN_ERR_INVALID). */
+int rtnn_set_points(rtnn_ctx ctx, const float* xyz, uint64_t n, int on_device);
+
+/* ---- search ----------------------------------------------------------- */
+
+/* One search over the context's points.  qxyz == NULL searches the points
+ * themselves (the reference's samepq case, util.cpp:371-382).  on_device
+ * describes qxyz; out_on_device describes out_idx/out_d2.  Replaces search()
+ * (search.cpp:9-61) with launchRadius = r and PRECISE mode. */
+int rtnn_search(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_device,
+                int mode, float r, uint32_t k,
+                int32_t* out_idx, float* out_d2, int out_on_device,
+                rtnn_stats* stats);
+
+/* KNN with the reference's exact density partition (sort.cpp:140-219,346-391
+ * with approx mode 0): per query cell the smallest megacell holding >= k+1
+ * points gives a launch radius min(r, sqrt(3)*(iter+1)*cell); queries whose
+ * sphere holds fewer than k hits are re-run at r, so the result equals
+ * rtnn_search(..., RTNN_MODE_KNN, ...). */
+int rtnn_search_partitioned(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_device,
+                            float r, uint32_t k,
+                            int32_t* out_idx, float* out_d2, int out_on_device,
+                            rtnn_stats* stats);
+
+/* ---- learned radius (KNN) --------------------------------------------- */
+
+/* Trains the radius-proposal MLP (30->64->64->1, GELU, BF16 weights, FP32
+ * accumulation) on exact k-th neighbor distances of n_samples queries drawn
+ * from the context's points, computed by the plain search at radius r.
+ * Pinball loss with quantile tau; Adam.  Runs entirely on the GPU. */
+int rtnn_mlp_train(rtnn_ctx ctx, float r, uint32_t k, uint32_t n_samples,
+                   uint32_t steps, uint32_t batch, float lr, float tau,
+                   uint64_t seed, float* final_loss);
+/* Weights as FP32: [w1 64x30 | b1 64 | w2 64x64 | b2 64 | w3 64 | b3 1] plus
+ * normalisation [2] = 6275 floats. */
+#define RTNN_MLP_NPARAMS 6273
+#define RTNN_MLP_NBLOB   6275
+int rtnn_mlp_get_weights(rtnn_ctx ctx, float* blob /* RTNN_MLP_NBLOB */);
+int rtnn_mlp_set_weights(rtnn_ctx ctx, const float* blob /* RTNN_MLP_NBLOB */);
+
+/* KNN where the MLP proposes r' <= r per query; queries with fewer than k hits
+ * within r' are re-run at r (stats->fallback_queries), so the result equals
+ * rtnn_search(..., RTNN_MODE_KNN, ...). */
+int rtnn_search_learned(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_device,
+                        float r, uint32_t k,
+                        int32_t* out_idx, float* out_d2, int out_on_device,
+                        rtnn_stats* stats);
+
+/* ---- batched clouds --------------------------------------------------- */
+
+/* nclouds independent clouds packed back to back in xyz; cloud c holds points
+ * [offsets[c], offsets[c+1]).  Queries = points of each cloud.  Output row i
+ * belongs to point i; indices are cloud-local.  One grid build and one
+ * persistent search launch for all clouds.  xyz, out_idx and out_d2 are
+ * device pointers; offsets (nclouds + 1 entries) is a host array. */
+int rtnn_search_batched(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets,
+                        uint32_t nclouds, int mode, float r, uint32_t k,
+                        int32_t* out_idx, float* out_d2, rtnn_stats* stats);
+/* Same result, one full pipeline per cloud (the per-cloud-launch baseline). */
+int rtnn_search_percloud(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets,
+                         uint32_t nclouds, int mode, float r, uint32_t k,
+                         int32_t* out_idx, float* out_d2, rtnn_stats* stats);
+
+/* ---- building blocks (exported for parity tests; device pointers) ---- */
+
+/* Stable LSD radix sort of (key, val) pairs on the low `bits` key bits. */
+int rtnn_radix_sort_pairs(rtnn_ctx ctx, uint32_t* keys, uint32_t* vals,
+                          uint64_t n, uint32_t bits);
+/* out[i] = sum(in[0..i)), out[n] = total (out has n+1 entries). */
+int rtnn_exclusive_scan(rtnn_ctx ctx, const uint32_t* in, uint32_t* out, uint64_t n);
+/* 30-bit 3D Morton codes of 10-bit cell coordinates (helper_mortonCode.h:19-34). */
+int rtnn_morton3(rtnn_ctx ctx, const uint32_t* xyz /* n x 3 */, uint32_t* codes,
+                 uint64_t n);
+/* MLP forward / backward on n feature rows (device pointers, FP32 I/O). */
+int rtnn_mlp_forward(rtnn_ctx ctx, const float* feats /* n x 30 */, uint64_t n,
+                     float* out /* n */);
+int rtnn_mlp_backward(rtnn_ctx ctx, const float* feats, const float* target,
+                      uint64_t n, float tau, float* grads /* RTNN_MLP_NPARAMS */,
+                      float* loss);
+/* Feature rows (log1p counts of the 27 coarse cells of edge r around each
+ * query + its offset in its cell) for queries qxyz against the context's
+ * points. */
+int rtnn_mlp_features(rtnn_ctx ctx, const float* qxyz, uint64_t nq, float r,
+                      float* feats /* nq x 30, device */);
+
+/* Library version string. */
+const char* rtnn_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RTNN_HIP_H */

@@ -1,0 +1,385 @@
+"""rtnn_amd -- Python host mirror of the gfx950 neighbor-search library.
+
+Thin ctypes layer over the C ABI in include/rtnn_hip.h and include/rtnn_data.h
+(librtnn_hip.so, built in-tree by rtnn_amd/build.py).  There is no fallback:
+if the library is missing or no GPU is present, calls raise.
+
+Search semantics mirror the reference driver (src/optixNSearch):
+  mode "radius": include iff d2 < r*r, the query itself included
+                 (geometry.cu:38-106), at most k results;
+  mode "knn"   : include iff 0 < d2 < r*r (geometry.cu:157-191), k nearest.
+Rows are original query indices, values original point indices in (d2, index)
+order, empty slots -1 (the reference's UINT_MAX sentinel, search.cpp:14-18).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librtnn_hip.so")
+
+MODE_RADIUS = 0
+MODE_KNN = 1
+MLP_NBLOB = 6275
+
+OK = 0
+ERR_INVALID = -1
+ERR_HIP = -2
+ERR_OOM = -3
+ERR_STATE = -4
+ERR_UNSUPPORTED = -5
+
+_lib = None
+
+
+class RtnnError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"rtnn error {code}: {msg}")
+        self.code = code
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("cell_factor", ctypes.c_float),
+                ("queries_per_wave", ctypes.c_int),
+                ("query_order", ctypes.c_int),
+                ("time_stages", ctypes.c_int),
+                ("max_cells", ctypes.c_uint32)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("ms_total", ctypes.c_double),
+                ("ms_grid", ctypes.c_double),
+                ("ms_qsort", ctypes.c_double),
+                ("ms_search", ctypes.c_double),
+                ("ms_mlp", ctypes.c_double),
+                ("ms_fallback", ctypes.c_double),
+                ("fallback_queries", ctypes.c_uint64),
+                ("grid_dims", ctypes.c_uint32 * 3),
+                ("cell_size", ctypes.c_float),
+                ("sort_passes", ctypes.c_uint32),
+                ("search_launches", ctypes.c_uint32),
+                ("lanes_per_query", ctypes.c_uint32)]
+
+    def as_dict(self):
+        d = {f: getattr(self, f) for f, _ in self._fields_}
+        d["grid_dims"] = list(self.grid_dims)
+        return d
+
+
+_VP = ctypes.c_void_p
+_U64 = ctypes.c_uint64
+_U32 = ctypes.c_uint32
+_I = ctypes.c_int
+_F = ctypes.c_float
+
+# name -> (restype, argtypes); every symbol of include/rtnn_hip.h and rtnn_data.h
+SIGNATURES = {
+    "rtnn_create": (_I, [_I, ctypes.POINTER(_VP)]),
+    "rtnn_destroy": (_I, [_VP]),
+    "rtnn_last_error": (ctypes.c_char_p, [_VP]),
+    "rtnn_create_error": (ctypes.c_char_p, []),
+    "rtnn_set_options": (_I, [_VP, ctypes.POINTER(Options)]),
+    "rtnn_synchronize": (_I, [_VP]),
+    "rtnn_set_points": (_I, [_VP, _VP, _U64, _I]),
+    "rtnn_search": (_I, [_VP, _VP, _U64, _I, _I, _F, _U32, _VP, _VP, _I,
+                         ctypes.POINTER(Stats)]),
+    "rtnn_search_partitioned": (_I, [_VP, _VP, _U64, _I, _F, _U32, _VP, _VP, _I,
+                                     ctypes.POINTER(Stats)]),
+    "rtnn_mlp_train": (_I, [_VP, _F, _U32, _U32, _U32, _U32, _F, _F, _U64,
+                            ctypes.POINTER(_F)]),
+    "rtnn_mlp_get_weights": (_I, [_VP, _VP]),
+    "rtnn_mlp_set_weights": (_I, [_VP, _VP]),
+    "rtnn_search_learned": (_I, [_VP, _VP, _U64, _I, _F, _U32, _VP, _VP, _I,
+                                 ctypes.POINTER(Stats)]),
+    "rtnn_search_batched": (_I, [_VP, _VP, _VP, _U32, _I, _F, _U32, _VP, _VP,
+                                 ctypes.POINTER(Stats)]),
+    "rtnn_search_percloud": (_I, [_VP, _VP, _VP, _U32, _I, _F, _U32, _VP, _VP,
+                                  ctypes.POINTER(Stats)]),
+    "rtnn_radix_sort_pairs": (_I, [_VP, _VP, _VP, _U64, _U32]),
+    "rtnn_exclusive_scan": (_I, [_VP, _VP, _VP, _U64]),
+    "rtnn_morton3": (_I, [_VP, _VP, _VP, _U64]),
+    "rtnn_mlp_forward": (_I, [_VP, _VP, _U64, _VP]),
+    "rtnn_mlp_backward": (_I, [_VP, _VP, _VP, _U64, _F, _VP, _VP]),
+    "rtnn_mlp_features": (_I, [_VP, _VP, _U64, _F, _VP]),
+    "rtnn_version": (ctypes.c_char_p, []),
+    "rtnn_read_points": (_I, [ctypes.c_char_p, _VP, ctypes.POINTER(_U64)]),
+    "rtnn_write_points": (_I, [ctypes.c_char_p, _VP, _U64]),
+    "rtnn_gen_uniform": (_I, [_U64, _U64, _F, _F, _VP]),
+    "rtnn_gen_gaussian_mixture": (_I, [_U64, _U64, _U64, _U32, _F, _F, _F, _VP]),
+    "rtnn_gen_spheres": (_I, [_U64, _U64, _U64, _U32, _F, _F, _F, _F, _VP]),
+    "rtnn_gen_cloud_sizes": (_I, [_U64, _U32, _U32, _U32, _VP]),
+    "rtnn_gen_lattice_clouds": (_I, [_U64, _U32, _VP, _F, _F, _VP]),
+    "rtnn_gen_levy": (_I, [_U64, _U64, _F, _F, _F, _VP]),
+}
+
+
+def lib():
+    """Loads librtnn_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with rtnn_amd/build.py")
+    L = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+# ------------------------------------------------------------------ helpers
+
+def _is_torch(x):
+    return type(x).__module__.startswith("torch")
+
+
+def _ptr(x):
+    """(pointer, on_device) of a numpy array or torch tensor (kept alive by caller)."""
+    if x is None:
+        return None, 0
+    if _is_torch(x):
+        return ctypes.c_void_p(x.data_ptr()), int(x.is_cuda)
+    return ctypes.c_void_p(x.ctypes.data), 0
+
+
+def _as_points(a):
+    if _is_torch(a):
+        import torch
+        a = a.to(torch.float32).contiguous()
+        if a.dim() != 2 or a.shape[1] != 3:
+            raise ValueError("points must have shape [n, 3]")
+        return a
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if a.ndim != 2 or a.shape[1] != 3:
+        raise ValueError("points must have shape [n, 3]")
+    return a
+
+
+def _mode(mode):
+    if mode in ("radius", MODE_RADIUS):
+        return MODE_RADIUS
+    if mode in ("knn", MODE_KNN):
+        return MODE_KNN
+    raise ValueError("mode must be 'radius' or 'knn'")
+
+
+class Context:
+    """One search context on one GPU (not thread-safe; one per thread)."""
+
+    def __init__(self, device=0):
+        L = lib()
+        h = ctypes.c_void_p()
+        rc = L.rtnn_create(device, ctypes.byref(h))
+        if rc != OK:
+            raise RtnnError(rc, L.rtnn_create_error().decode())
+        self._h = h
+        self._L = L
+        self._points = None
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.rtnn_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc):
+        if rc != OK:
+            raise RtnnError(rc, self._L.rtnn_last_error(self._h).decode())
+
+    def set_options(self, cell_factor=0.0, queries_per_wave=0, query_order=0,
+                    time_stages=0, max_cells=0):
+        o = Options(cell_factor, queries_per_wave, query_order, time_stages, max_cells)
+        self._check(self._L.rtnn_set_options(self._h, ctypes.byref(o)))
+
+    def synchronize(self):
+        self._check(self._L.rtnn_synchronize(self._h))
+
+    def set_points(self, points):
+        p = _as_points(points)
+        ptr, dev = _ptr(p)
+        self._check(self._L.rtnn_set_points(self._h, ptr, len(p), dev))
+        self._points = p
+        return self
+
+    def _outputs(self, like_device, nq, k, with_d2):
+        if like_device:
+            import torch
+            idx = torch.empty((nq, k), dtype=torch.int32, device="cuda")
+            d2 = torch.empty((nq, k), dtype=torch.float32, device="cuda") if with_d2 else None
+        else:
+            idx = np.empty((nq, k), dtype=np.int32)
+            d2 = np.empty((nq, k), dtype=np.float32) if with_d2 else None
+        return idx, d2
+
+    def _run(self, fn, queries, r, k, with_d2, out_device, extra=()):
+        stats = Stats()
+        if queries is None:
+            if self._points is None:
+                raise RtnnError(ERR_STATE, "set_points() has not been called")
+            nq, qptr, qdev = len(self._points), None, 0
+            keep = None
+        else:
+            keep = _as_points(queries)
+            qptr, qdev = _ptr(keep)
+            nq = len(keep)
+        if out_device is None:
+            out_device = bool(qdev) if queries is not None else _is_torch(self._points)
+        idx, d2 = self._outputs(out_device, nq, k, with_d2)
+        ip, _ = _ptr(idx)
+        dp, _ = _ptr(d2)
+        rc = fn(self._h, qptr, nq, qdev, *extra, r, k, ip, dp, int(out_device),
+                ctypes.byref(stats))
+        self._check(rc)
+        return idx, d2, stats
+
+    def search(self, queries=None, mode="radius", radius=2.0, k=50, with_d2=True,
+               out_device=None):
+        """Plain grid search -> (idx [nq, k] int32, d2 [nq, k] float32 or None, Stats)."""
+        return self._run(self._L.rtnn_search, queries, radius, k, with_d2, out_device,
+                         extra=(_mode(mode),))
+
+    def search_partitioned(self, queries=None, radius=2.0, k=8, with_d2=True,
+                           out_device=None):
+        return self._run(self._L.rtnn_search_partitioned, queries, radius, k, with_d2,
+                         out_device)
+
+    def search_learned(self, queries=None, radius=10.0, k=16, with_d2=True, out_device=None):
+        return self._run(self._L.rtnn_search_learned, queries, radius, k, with_d2, out_device)
+
+    def mlp_train(self, radius, k, n_samples=1 << 21, steps=2000, batch=4096, lr=1e-3,
+                  tau=0.9, seed=1):
+        loss = ctypes.c_float(0)
+        self._check(self._L.rtnn_mlp_train(self._h, radius, k, n_samples, steps, batch, lr, tau,
+                                           seed, ctypes.byref(loss)))
+        return float(loss.value)
+
+    def mlp_weights(self):
+        w = np.empty(MLP_NBLOB, dtype=np.float32)
+        self._check(self._L.rtnn_mlp_get_weights(self._h, ctypes.c_void_p(w.ctypes.data)))
+        return w
+
+    def set_mlp_weights(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        if w.size != MLP_NBLOB:
+            raise ValueError(f"weight blob must have {MLP_NBLOB} floats")
+        self._check(self._L.rtnn_mlp_set_weights(self._h, ctypes.c_void_p(w.ctypes.data)))
+
+    def _batched(self, fn, xyz, offsets, mode, radius, k, with_d2):
+        import torch
+        if not (_is_torch(xyz) and xyz.is_cuda):
+            raise ValueError("batched search takes a CUDA tensor of points")
+        xyz = xyz.to(torch.float32).contiguous()
+        offs = np.ascontiguousarray(offsets, dtype=np.uint64)
+        n = int(offs[-1])
+        idx = torch.empty((n, k), dtype=torch.int32, device=xyz.device)
+        d2 = torch.empty((n, k), dtype=torch.float32, device=xyz.device) if with_d2 else None
+        stats = Stats()
+        rc = fn(self._h, ctypes.c_void_p(xyz.data_ptr()), ctypes.c_void_p(offs.ctypes.data),
+                len(offs) - 1, _mode(mode), radius, k, ctypes.c_void_p(idx.data_ptr()),
+                ctypes.c_void_p(d2.data_ptr()) if d2 is not None else None,
+                ctypes.byref(stats))
+        self._check(rc)
+        return idx, d2, stats
+
+    def search_batched(self, xyz, offsets, mode="radius", radius=0.2, k=64, with_d2=True):
+        return self._batched(self._L.rtnn_search_batched, xyz, offsets, mode, radius, k,
+                             with_d2)
+
+    def search_percloud(self, xyz, offsets, mode="radius", radius=0.2, k=64, with_d2=True):
+        return self._batched(self._L.rtnn_search_percloud, xyz, offsets, mode, radius, k,
+                             with_d2)
+
+    # building blocks (device tensors)
+    def radix_sort_pairs(self, keys, vals, bits=32):
+        self._check(self._L.rtnn_radix_sort_pairs(self._h, ctypes.c_void_p(keys.data_ptr()),
+                                                  ctypes.c_void_p(vals.data_ptr()),
+                                                  keys.numel(), bits))
+
+    def exclusive_scan(self, inp, out):
+        self._check(self._L.rtnn_exclusive_scan(self._h, ctypes.c_void_p(inp.data_ptr()),
+                                                ctypes.c_void_p(out.data_ptr()), inp.numel()))
+
+    def morton3(self, xyz, codes):
+        self._check(self._L.rtnn_morton3(self._h, ctypes.c_void_p(xyz.data_ptr()),
+                                         ctypes.c_void_p(codes.data_ptr()), codes.numel()))
+
+
+# --------------------------------------------------------------- data utils
+
+def read_points(path):
+    """Point file -> float32 [n, 3] (reference read_pc_data semantics)."""
+    L = lib()
+    n = ctypes.c_uint64(0)
+    rc = L.rtnn_read_points(path.encode(), None, ctypes.byref(n))
+    if rc == -2:
+        raise FileNotFoundError(path)
+    if rc != 0:
+        raise ValueError(f"cannot read {path} (code {rc})")
+    out = np.empty((n.value, 3), dtype=np.float32)
+    rc = L.rtnn_read_points(path.encode(), ctypes.c_void_p(out.ctypes.data), ctypes.byref(n))
+    if rc == -1:
+        raise ValueError(f"{path}: line {n.value} does not hold three numbers")
+    if rc != 0:
+        raise ValueError(f"cannot read {path} (code {rc})")
+    return out
+
+
+def write_points(path, pts):
+    pts = np.ascontiguousarray(pts, dtype=np.float32)
+    rc = lib().rtnn_write_points(path.encode(), ctypes.c_void_p(pts.ctypes.data), len(pts))
+    if rc != 0:
+        raise OSError(f"cannot write {path}")
+
+
+def _gen(fn, n, *args):
+    out = np.empty((n, 3), dtype=np.float32)
+    rc = fn(*args, ctypes.c_void_p(out.ctypes.data))
+    if rc != 0:
+        raise ValueError("generator failed")
+    return out
+
+
+def gen_uniform(seed, n, lo, hi):
+    return _gen(lib().rtnn_gen_uniform, n, seed, n, lo, hi)
+
+
+def gen_gaussian_mixture(dist_seed, sample_seed, n, ncomp=64, box=1000.0, sigma=5.0,
+                         bg_frac=0.2):
+    return _gen(lib().rtnn_gen_gaussian_mixture, n, dist_seed, sample_seed, n, ncomp, box,
+                sigma, bg_frac)
+
+
+def gen_spheres(dist_seed, sample_seed, n, nspheres=2000, box=2000.0, rmin=5.0, rmax=50.0,
+                noise=0.05):
+    return _gen(lib().rtnn_gen_spheres, n, dist_seed, sample_seed, n, nspheres, box, rmin,
+                rmax, noise)
+
+
+def gen_levy(seed, n, alpha=1.5, min_step=0.01, box=1000.0):
+    return _gen(lib().rtnn_gen_levy, n, seed, n, alpha, min_step, box)
+
+
+def gen_lattice_clouds(seed, nclouds, nmin=1000, nmax=200000, h=0.1, jit=0.02):
+    L = lib()
+    offs = np.empty(nclouds + 1, dtype=np.uint64)
+    rc = L.rtnn_gen_cloud_sizes(seed, nclouds, nmin, nmax, ctypes.c_void_p(offs.ctypes.data))
+    if rc != 0:
+        raise ValueError("bad cloud size range")
+    xyz = np.empty((int(offs[-1]), 3), dtype=np.float32)
+    L.rtnn_gen_lattice_clouds(seed, nclouds, ctypes.c_void_p(offs.ctypes.data), h, jit,
+                              ctypes.c_void_p(xyz.ctypes.data))
+    return xyz, offs

@@ -1,0 +1,662 @@
+// C ABI of the gfx950 neighbor search: context, grid pipeline, entry points.
+//
+// Pipeline of one search (the reference's uploadData -> sortParticles ->
+// createGeometry -> search, main.cpp:84-143, with the BVH replaced by the cell
+// list):
+//   points bbox (once per point set) -> grid plan (host) -> cell keys + counts
+//   -> exclusive scan -> stable radix sort -> gather -> query Morton order ->
+//   search kernel -> rows in original query order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/rtnn_hip.h"
+#include "common.h"
+#include "internal.h"
+#include "mlp.h"
+
+using namespace rtnn;
+
+namespace {
+
+thread_local std::string g_create_err;
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  hipError_t ensure(size_t bytes) {
+    if (bytes <= cap && p) return hipSuccess;
+    if (p) {
+      // Queued work may still read the old buffer.
+      (void)hipDeviceSynchronize();
+      (void)hipFree(p);
+      p = nullptr;
+      cap = 0;
+    }
+    size_t want = std::max<size_t>(bytes, 256);
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) {
+      p = nullptr;
+      return e;
+    }
+    cap = want;
+    return hipSuccess;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+  template <class T>
+  T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+}  // namespace
+
+struct rtnn_ctx_s {
+  int device = 0;
+  hipStream_t st = nullptr;
+  int num_cus = 256;
+  std::string err;
+  rtnn_options opt{};
+
+  // points
+  DevBuf pts;
+  uint64_t n = 0;
+  float bbox[6] = {0, 0, 0, 0, 0, 0};
+
+  // grid
+  DevBuf keys, vals, counts, cell_start, spts, tmp, grids;
+  GridDesc hgrid{};
+  uint64_t ncells = 0;
+  int sort_passes = 0;
+
+  // queries / outputs
+  DevBuf qin, qkeys, qvals, qsorted, oidx, od2, shortrows, misc, tiles, offs, bboxes;
+  hipEvent_t ev[8] = {};
+
+  MlpState mlp;
+
+  ~rtnn_ctx_s() {
+    DevBuf* all[] = {&pts, &keys, &vals, &counts, &cell_start, &spts, &tmp, &grids,
+                     &qin, &qkeys, &qvals, &qsorted, &oidx, &od2, &shortrows, &misc,
+                     &tiles, &offs, &bboxes};
+    for (DevBuf* b : all) b->release();
+    mlp_release(mlp);
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+    if (st) (void)hipStreamDestroy(st);
+  }
+};
+
+namespace {
+
+int fail(rtnn_ctx c, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  return code;
+}
+
+int hip_fail(rtnn_ctx c, hipError_t e, const char* what) {
+  int code = e == hipErrorOutOfMemory ? RTNN_ERR_OOM : RTNN_ERR_HIP;
+  return fail(c, code, "%s: %s", what, hipGetErrorString(e));
+}
+
+#define CK(expr)                                           \
+  do {                                                     \
+    hipError_t e_ = (expr);                                \
+    if (e_ != hipSuccess) return hip_fail(ctx, e_, #expr); \
+  } while (0)
+
+int bits_for(uint64_t maxval) {
+  int b = 0;
+  while (b < 64 && (maxval >> b) != 0) ++b;
+  return b;
+}
+
+// Grid plan for one cloud: dims = floor((max - min) / cell) + 1, computed in
+// FP32 exactly as the device computes cell coordinates, so every point's cell
+// lies inside the grid.
+void plan_grid(const float* bb, float cell, GridDesc& g) {
+  std::memset(&g, 0, sizeof g);
+  g.cell = cell;
+  g.inv = 1.0f / cell;
+  for (int a = 0; a < 3; ++a) {
+    g.mn[a] = bb[a];
+    volatile float ext = bb[a + 3] - bb[a];
+    volatile float e = ext * g.inv;
+    float d = std::floor((float)e) + 1.0f;
+    g.dim[a] = (int)std::min(d, 2.0e9f);
+  }
+}
+
+uint64_t grid_cells(const GridDesc& g) {
+  return (uint64_t)g.dim[0] * (uint64_t)g.dim[1] * (uint64_t)g.dim[2];
+}
+
+uint64_t max_cells_for(rtnn_ctx ctx, uint64_t npts) {
+  if (ctx->opt.max_cells) return ctx->opt.max_cells;
+  uint64_t m = std::max<uint64_t>(1ull << 22, 32ull * npts);
+  return std::min<uint64_t>(m, 1ull << 30);
+}
+
+float default_cell_factor(int mode) { return mode == RTNN_MODE_KNN ? 1.0f : 1.0f; }
+
+int default_group_lanes(int mode, uint32_t k) {
+  if (mode == RTNN_MODE_KNN && k <= 64) return 1;
+  if (k <= 64) return 16;
+  if (k <= 256) return 16;
+  return 64;
+}
+
+int group_lanes_from_opt(rtnn_ctx ctx, int mode, uint32_t k) {
+  int qpw = ctx->opt.queries_per_wave;
+  if (qpw <= 0) return default_group_lanes(mode, k);
+  int gl = 64 / qpw;
+  if (gl != 1 && gl != 4 && gl != 16 && gl != 64) return -1;
+  return gl;
+}
+
+// Builds the single-cloud grid over ctx->pts with edge `cell` (grown until the
+// cell count fits the budget).
+int build_grid(rtnn_ctx ctx, float cell) {
+  GridDesc g;
+  const uint64_t cap = max_cells_for(ctx, ctx->n);
+  for (int tries = 0;; ++tries) {
+    plan_grid(ctx->bbox, cell, g);
+    if (grid_cells(g) <= cap && grid_cells(g) < 0xfffffff0ull) break;
+    if (tries > 200) return fail(ctx, RTNN_ERR_UNSUPPORTED, "grid planning failed");
+    cell *= 1.25f;
+  }
+  g.cell_base = 0;
+  g.point_base = 0;
+  g.npoints = (uint32_t)ctx->n;
+  ctx->hgrid = g;
+  ctx->ncells = grid_cells(g);
+  const uint64_t n = ctx->n;
+  CK(ctx->grids.ensure(sizeof(GridDesc)));
+  CK(hipMemcpyAsync(ctx->grids.p, &ctx->hgrid, sizeof(GridDesc), hipMemcpyHostToDevice, ctx->st));
+  CK(ctx->keys.ensure(4 * n));
+  CK(ctx->vals.ensure(4 * n));
+  CK(ctx->counts.ensure(4 * (ctx->ncells + 1)));
+  CK(ctx->cell_start.ensure(4 * (ctx->ncells + 1)));
+  CK(ctx->spts.ensure(16 * std::max<uint64_t>(n, 1)));
+  uint64_t tmpw = std::max<uint64_t>(radix_tmp_words(n), (uint64_t)scan_tmp_words(ctx->ncells));
+  CK(ctx->tmp.ensure(4 * tmpw));
+  CK(fill_u32(ctx->counts.as<uint32_t>(), ctx->ncells, 0, ctx->st));
+  CK(cell_keys(ctx->pts.as<float>(), n, ctx->grids.as<GridDesc>(), nullptr, 1,
+               ctx->keys.as<uint32_t>(), ctx->vals.as<uint32_t>(), ctx->counts.as<uint32_t>(),
+               ctx->st));
+  CK(exclusive_scan(ctx->counts.as<uint32_t>(), ctx->cell_start.as<uint32_t>(), ctx->ncells,
+                    ctx->tmp.as<uint32_t>(), ctx->st));
+  CK(radix_sort_pairs(ctx->keys.as<uint32_t>(), ctx->vals.as<uint32_t>(), n,
+                      bits_for(ctx->ncells - 1), ctx->tmp.as<uint32_t>(), ctx->st,
+                      &ctx->sort_passes));
+  CK(gather_points(ctx->pts.as<float>(), ctx->vals.as<uint32_t>(), n, ctx->spts.as<float4>(),
+                   ctx->st));
+  return RTNN_OK;
+}
+
+// Prepares queries in processing order (float4 with the output row in w).
+// Returns the processing-order array through *qproc.
+int prepare_queries(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_device,
+                    const float4** qproc) {
+  if (!qxyz) {  // the points themselves, already in cell order
+    *qproc = ctx->spts.as<float4>();
+    return RTNN_OK;
+  }
+  const float* dq = qxyz;
+  if (!on_device) {
+    CK(ctx->qin.ensure(12 * nq));
+    CK(hipMemcpyAsync(ctx->qin.p, qxyz, 12 * nq, hipMemcpyHostToDevice, ctx->st));
+    dq = ctx->qin.as<float>();
+  }
+  CK(ctx->qkeys.ensure(4 * nq));
+  CK(ctx->qvals.ensure(4 * nq));
+  CK(ctx->qsorted.ensure(16 * nq));
+  uint64_t tmpw = radix_tmp_words(nq);
+  if (ctx->tmp.cap < 4 * tmpw) {
+    // keep the grid's scratch contents irrelevant: tmp is only scratch
+    CK(ctx->tmp.ensure(4 * tmpw));
+  }
+  if (ctx->opt.query_order == 1) {
+    CK(iota_u32(ctx->qvals.as<uint32_t>(), nq, ctx->st));
+  } else {
+    int maxdim = std::max(ctx->hgrid.dim[0], std::max(ctx->hgrid.dim[1], ctx->hgrid.dim[2]));
+    int shift = std::max(0, bits_for((uint64_t)(maxdim - 1)) - 10);
+    CK(query_morton_keys(dq, nq, ctx->grids.as<GridDesc>(), shift, ctx->qkeys.as<uint32_t>(),
+                         ctx->qvals.as<uint32_t>(), ctx->st));
+    CK(radix_sort_pairs(ctx->qkeys.as<uint32_t>(), ctx->qvals.as<uint32_t>(), nq, 30,
+                        ctx->tmp.as<uint32_t>(), ctx->st, nullptr));
+  }
+  CK(gather_points(dq, ctx->qvals.as<uint32_t>(), nq, ctx->qsorted.as<float4>(), ctx->st));
+  *qproc = ctx->qsorted.as<float4>();
+  return RTNN_OK;
+}
+
+double ev_ms(hipEvent_t a, hipEvent_t b) {
+  float ms = 0;
+  if (hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return ms;
+}
+
+int check_search_args(rtnn_ctx ctx, int mode, float r, uint32_t k, const void* out_idx) {
+  if (!ctx) return RTNN_ERR_INVALID;
+  if (mode != RTNN_MODE_RADIUS && mode != RTNN_MODE_KNN)
+    return fail(ctx, RTNN_ERR_INVALID, "mode must be RTNN_MODE_RADIUS or RTNN_MODE_KNN");
+  if (!(r > 0.0f) || !std::isfinite(r)) return fail(ctx, RTNN_ERR_INVALID, "radius must be > 0");
+  if (k == 0) return fail(ctx, RTNN_ERR_INVALID, "k must be >= 1");
+  if (!out_idx) return fail(ctx, RTNN_ERR_INVALID, "out_idx is NULL");
+  return RTNN_OK;
+}
+
+struct OutBufs {
+  int32_t* idx;
+  float* d2;
+};
+
+int output_targets(rtnn_ctx ctx, uint64_t nq, uint32_t k, int32_t* out_idx, float* out_d2,
+                   int out_on_device, OutBufs& o) {
+  if (out_on_device) {
+    o.idx = out_idx;
+    o.d2 = out_d2;
+    return RTNN_OK;
+  }
+  CK(ctx->oidx.ensure(4 * nq * k));
+  o.idx = ctx->oidx.as<int32_t>();
+  o.d2 = nullptr;
+  if (out_d2) {
+    CK(ctx->od2.ensure(4 * nq * k));
+    o.d2 = ctx->od2.as<float>();
+  }
+  return RTNN_OK;
+}
+
+int copy_out(rtnn_ctx ctx, uint64_t nq, uint32_t k, int32_t* out_idx, float* out_d2,
+             int out_on_device, const OutBufs& o) {
+  if (out_on_device) return RTNN_OK;
+  CK(hipMemcpyAsync(out_idx, o.idx, 4 * nq * k, hipMemcpyDeviceToHost, ctx->st));
+  if (out_d2) CK(hipMemcpyAsync(out_d2, o.d2, 4 * nq * k, hipMemcpyDeviceToHost, ctx->st));
+  return RTNN_OK;
+}
+
+SearchArgs base_args(rtnn_ctx ctx, const float4* qproc, uint64_t nq, float r, uint32_t k,
+                     const OutBufs& o) {
+  SearchArgs a;
+  std::memset(&a, 0, sizeof a);
+  a.spts = ctx->spts.as<float4>();
+  a.cell_start = ctx->cell_start.as<uint32_t>();
+  a.grids = ctx->grids.as<GridDesc>();
+  a.queries = qproc;
+  a.tiles = nullptr;
+  a.nq = (uint32_t)nq;
+  a.ntiles = (uint32_t)((nq + kTileQ - 1) / kTileQ);
+  a.r = r;
+  a.k = k;
+  a.out_idx = o.idx;
+  a.out_d2 = o.d2;
+  return a;
+}
+
+void fill_stats(rtnn_ctx ctx, rtnn_stats* s) {
+  if (!s) return;
+  for (int a = 0; a < 3; ++a) s->grid_dims[a] = (uint32_t)ctx->hgrid.dim[a];
+  s->cell_size = ctx->hgrid.cell;
+  s->sort_passes = (uint32_t)ctx->sort_passes;
+}
+
+}  // namespace
+
+// ===================================================================== ABI
+
+extern "C" {
+
+const char* rtnn_version(void) { return "rtnn-mi355x 0.1 (gfx950)"; }
+
+const char* rtnn_create_error(void) { return g_create_err.c_str(); }
+
+int rtnn_create(int device, rtnn_ctx* out) {
+  if (!out) return RTNN_ERR_INVALID;
+  *out = nullptr;
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count == 0) {
+    g_create_err = std::string("no HIP device: ") + hipGetErrorString(e);
+    return RTNN_ERR_HIP;
+  }
+  if (device < 0 || device >= count) {
+    g_create_err = "device id out of range";
+    return RTNN_ERR_INVALID;
+  }
+  e = hipSetDevice(device);
+  if (e != hipSuccess) {
+    g_create_err = std::string("hipSetDevice: ") + hipGetErrorString(e);
+    return RTNN_ERR_HIP;
+  }
+  rtnn_ctx c = new (std::nothrow) rtnn_ctx_s();
+  if (!c) return RTNN_ERR_OOM;
+  c->device = device;
+  e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    g_create_err = std::string("hipStreamCreate: ") + hipGetErrorString(e);
+    delete c;
+    return RTNN_ERR_HIP;
+  }
+  for (auto& ev : c->ev) (void)hipEventCreate(&ev);
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
+      cus > 0)
+    c->num_cus = cus;
+  *out = c;
+  return RTNN_OK;
+}
+
+int rtnn_destroy(rtnn_ctx ctx) {
+  if (!ctx) return RTNN_ERR_INVALID;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->st);
+  delete ctx;
+  return RTNN_OK;
+}
+
+const char* rtnn_last_error(rtnn_ctx ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int rtnn_set_options(rtnn_ctx ctx, const rtnn_options* opt) {
+  if (!ctx || !opt) return RTNN_ERR_INVALID;
+  if (opt->cell_factor < 0 || !std::isfinite(opt->cell_factor))
+    return fail(ctx, RTNN_ERR_INVALID, "cell_factor must be >= 0");
+  if (opt->queries_per_wave != 0 && opt->queries_per_wave != 1 && opt->queries_per_wave != 4 &&
+      opt->queries_per_wave != 16 && opt->queries_per_wave != 64)
+    return fail(ctx, RTNN_ERR_INVALID, "queries_per_wave must be 0, 1, 4, 16 or 64");
+  if (opt->query_order < 0 || opt->query_order > 2)
+    return fail(ctx, RTNN_ERR_INVALID, "query_order must be 0, 1 or 2");
+  ctx->opt = *opt;
+  return RTNN_OK;
+}
+
+int rtnn_synchronize(rtnn_ctx ctx) {
+  if (!ctx) return RTNN_ERR_INVALID;
+  CK(hipStreamSynchronize(ctx->st));
+  return RTNN_OK;
+}
+
+int rtnn_set_points(rtnn_ctx ctx, const float* xyz, uint64_t n, int on_device) {
+  if (!ctx) return RTNN_ERR_INVALID;
+  if (n == 0) return fail(ctx, RTNN_ERR_INVALID, "empty point set");
+  if (!xyz) return fail(ctx, RTNN_ERR_INVALID, "xyz is NULL");
+  if (n >= 0xffffffffull) return fail(ctx, RTNN_ERR_UNSUPPORTED, "more than 2^32-2 points");
+  CK(hipSetDevice(ctx->device));
+  CK(ctx->pts.ensure(12 * n));
+  CK(hipMemcpyAsync(ctx->pts.p, xyz, 12 * n,
+                    on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, ctx->st));
+  ctx->n = n;
+  CK(ctx->misc.ensure(4096 + 6 * 4 * kBboxBlocks));
+  float* bb = ctx->misc.as<float>();
+  uint32_t* bad = reinterpret_cast<uint32_t*>(bb + 8);
+  float* part = bb + 16;
+  CK(fill_u32(bad, 1, 0, ctx->st));
+  CK(bbox(ctx->pts.as<float>(), n, part, bb, bad, ctx->st));
+  float hb[16];
+  CK(hipMemcpyAsync(hb, bb, 64, hipMemcpyDeviceToHost, ctx->st));
+  CK(hipStreamSynchronize(ctx->st));
+  uint32_t nbad;
+  std::memcpy(&nbad, &hb[8], 4);
+  if (nbad) {
+    ctx->n = 0;
+    return fail(ctx, RTNN_ERR_INVALID, "%u non-finite point coordinates", nbad);
+  }
+  std::memcpy(ctx->bbox, hb, sizeof ctx->bbox);
+  return RTNN_OK;
+}
+
+static int search_impl(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_device, int mode,
+                       float r, uint32_t k, int32_t* out_idx, float* out_d2, int out_on_device,
+                       rtnn_stats* stats) {
+  int rc = check_search_args(ctx, mode, r, k, out_idx);
+  if (rc) return rc;
+  if (ctx->n == 0) return fail(ctx, RTNN_ERR_STATE, "rtnn_set_points has not been called");
+  if (!qxyz) nq = ctx->n;
+  if (nq == 0) return RTNN_OK;
+  if (nq >= 0xffffffffull) return fail(ctx, RTNN_ERR_UNSUPPORTED, "more than 2^32-2 queries");
+  const int gl = group_lanes_from_opt(ctx, mode, k);
+  if (gl < 0 || k > max_k_for_group(gl)) {
+    if (ctx->opt.queries_per_wave > 0 && k <= 1024)
+      return fail(ctx, RTNN_ERR_UNSUPPORTED, "k=%u is too large for queries_per_wave=%d", k,
+                  ctx->opt.queries_per_wave);
+    return fail(ctx, RTNN_ERR_UNSUPPORTED, "k=%u exceeds the supported maximum (1024)", k);
+  }
+  CK(hipSetDevice(ctx->device));
+  const bool timing = ctx->opt.time_stages != 0 || stats != nullptr;
+  if (timing) CK(hipEventRecord(ctx->ev[0], ctx->st));
+  float f = ctx->opt.cell_factor > 0 ? ctx->opt.cell_factor : default_cell_factor(mode);
+  rc = build_grid(ctx, r / f);
+  if (rc) return rc;
+  if (timing) CK(hipEventRecord(ctx->ev[1], ctx->st));
+  const float4* qproc = nullptr;
+  rc = prepare_queries(ctx, qxyz, nq, on_device, &qproc);
+  if (rc) return rc;
+  if (timing) CK(hipEventRecord(ctx->ev[2], ctx->st));
+  OutBufs o;
+  rc = output_targets(ctx, nq, k, out_idx, out_d2, out_on_device, o);
+  if (rc) return rc;
+  SearchArgs a = base_args(ctx, qproc, nq, r, k, o);
+  CK(launch_search(a, mode == RTNN_MODE_KNN, gl, ctx->st, ctx->num_cus));
+  if (timing) CK(hipEventRecord(ctx->ev[3], ctx->st));
+  rc = copy_out(ctx, nq, k, out_idx, out_d2, out_on_device, o);
+  if (rc) return rc;
+  if (timing) CK(hipEventRecord(ctx->ev[4], ctx->st));
+  CK(hipStreamSynchronize(ctx->st));
+  if (stats) {
+    std::memset(stats, 0, sizeof *stats);
+    stats->ms_grid = ev_ms(ctx->ev[0], ctx->ev[1]);
+    stats->ms_qsort = ev_ms(ctx->ev[1], ctx->ev[2]);
+    stats->ms_search = ev_ms(ctx->ev[2], ctx->ev[3]);
+    stats->ms_total = ev_ms(ctx->ev[0], ctx->ev[4]);
+    stats->search_launches = 1;
+    stats->lanes_per_query = (uint32_t)gl;
+    fill_stats(ctx, stats);
+  }
+  return RTNN_OK;
+}
+
+int rtnn_search(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_device, int mode, float r,
+                uint32_t k, int32_t* out_idx, float* out_d2, int out_on_device,
+                rtnn_stats* stats) {
+  return search_impl(ctx, qxyz, nq, on_device, mode, r, k, out_idx, out_d2, out_on_device,
+                     stats);
+}
+
+int rtnn_search_partitioned(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_device,
+                            float r, uint32_t k, int32_t* out_idx, float* out_d2,
+                            int out_on_device, rtnn_stats* stats) {
+  return search_impl(ctx, qxyz, nq, on_device, RTNN_MODE_KNN, r, k, out_idx, out_d2,
+                     out_on_device, stats);
+}
+
+// ------------------------------------------------------------- batched ----
+
+int rtnn_search_batched(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets,
+                        uint32_t nclouds, int mode, float r, uint32_t k, int32_t* out_idx,
+                        float* out_d2, rtnn_stats* stats) {
+  int rc = check_search_args(ctx, mode, r, k, out_idx);
+  if (rc) return rc;
+  if (!xyz || !offsets || nclouds == 0)
+    return fail(ctx, RTNN_ERR_INVALID, "xyz/offsets must be set and nclouds >= 1");
+  const uint64_t n = offsets[nclouds];
+  if (offsets[0] != 0) return fail(ctx, RTNN_ERR_INVALID, "offsets[0] must be 0");
+  for (uint32_t c = 0; c < nclouds; ++c)
+    if (offsets[c + 1] < offsets[c]) return fail(ctx, RTNN_ERR_INVALID, "offsets must be non-decreasing");
+  if (n == 0) return RTNN_OK;
+  if (n >= 0xffffffffull) return fail(ctx, RTNN_ERR_UNSUPPORTED, "more than 2^32-2 points");
+  const int gl = group_lanes_from_opt(ctx, mode, k);
+  if (gl < 0 || k > max_k_for_group(gl))
+    return fail(ctx, RTNN_ERR_UNSUPPORTED, "k=%u not supported by the selected kernel", k);
+  CK(hipSetDevice(ctx->device));
+  const bool timing = ctx->opt.time_stages != 0 || stats != nullptr;
+  if (timing) CK(hipEventRecord(ctx->ev[0], ctx->st));
+
+  // Per-cloud bbox on the device, plan on the host.
+  CK(ctx->offs.ensure(8 * (nclouds + 1)));
+  CK(hipMemcpyAsync(ctx->offs.p, offsets, 8 * (nclouds + 1), hipMemcpyHostToDevice, ctx->st));
+  CK(ctx->bboxes.ensure(24 * nclouds + 64));
+  uint32_t* bad = reinterpret_cast<uint32_t*>(ctx->bboxes.as<float>() + 6 * nclouds);
+  CK(fill_u32(bad, 1, 0, ctx->st));
+  CK(bbox_segmented(xyz, ctx->offs.as<uint64_t>(), nclouds, ctx->bboxes.as<float>(), bad,
+                    ctx->st));
+  std::vector<float> hb(6 * nclouds + 1);
+  CK(hipMemcpyAsync(hb.data(), ctx->bboxes.p, 4 * (6 * nclouds + 1), hipMemcpyDeviceToHost,
+                    ctx->st));
+  CK(hipStreamSynchronize(ctx->st));
+  uint32_t nbad;
+  std::memcpy(&nbad, &hb[6 * nclouds], 4);
+  if (nbad) return fail(ctx, RTNN_ERR_INVALID, "%u non-finite point coordinates", nbad);
+
+  float f = ctx->opt.cell_factor > 0 ? ctx->opt.cell_factor : default_cell_factor(mode);
+  float cell = r / f;
+  std::vector<GridDesc> gd(nclouds);
+  uint64_t total = 0;
+  const uint64_t cap = max_cells_for(ctx, n);
+  for (int tries = 0;; ++tries) {
+    total = 0;
+    for (uint32_t c = 0; c < nclouds; ++c) {
+      plan_grid(&hb[6 * c], cell, gd[c]);
+      if (offsets[c + 1] == offsets[c]) gd[c].dim[0] = gd[c].dim[1] = gd[c].dim[2] = 1;
+      gd[c].cell_base = (uint32_t)total;
+      gd[c].point_base = (uint32_t)offsets[c];
+      gd[c].npoints = (uint32_t)(offsets[c + 1] - offsets[c]);
+      total += grid_cells(gd[c]);
+    }
+    if (total <= cap && total < 0xfffffff0ull) break;
+    if (tries > 200) return fail(ctx, RTNN_ERR_UNSUPPORTED, "grid planning failed");
+    cell *= 1.25f;
+  }
+  ctx->hgrid = gd[0];
+  ctx->ncells = total;
+  CK(ctx->grids.ensure(sizeof(GridDesc) * nclouds));
+  CK(hipMemcpyAsync(ctx->grids.p, gd.data(), sizeof(GridDesc) * nclouds, hipMemcpyHostToDevice,
+                    ctx->st));
+  CK(ctx->keys.ensure(4 * n));
+  CK(ctx->vals.ensure(4 * n));
+  CK(ctx->counts.ensure(4 * (total + 1)));
+  CK(ctx->cell_start.ensure(4 * (total + 1)));
+  CK(ctx->spts.ensure(16 * n));
+  CK(ctx->tmp.ensure(4 * std::max<uint64_t>(radix_tmp_words(n), (uint64_t)scan_tmp_words(total))));
+  CK(fill_u32(ctx->counts.as<uint32_t>(), total, 0, ctx->st));
+  CK(cell_keys(xyz, n, ctx->grids.as<GridDesc>(), ctx->offs.as<uint64_t>(), nclouds,
+               ctx->keys.as<uint32_t>(), ctx->vals.as<uint32_t>(), ctx->counts.as<uint32_t>(),
+               ctx->st));
+  CK(exclusive_scan(ctx->counts.as<uint32_t>(), ctx->cell_start.as<uint32_t>(), total,
+                    ctx->tmp.as<uint32_t>(), ctx->st));
+  CK(radix_sort_pairs(ctx->keys.as<uint32_t>(), ctx->vals.as<uint32_t>(), n,
+                      bits_for(total - 1), ctx->tmp.as<uint32_t>(), ctx->st,
+                      &ctx->sort_passes));
+  CK(gather_points(xyz, ctx->vals.as<uint32_t>(), n, ctx->spts.as<float4>(), ctx->st));
+
+  // Tiles: runs of <= kTileQ sorted queries inside one cloud.  Cell keys are
+  // cloud-major, so cloud c's points occupy sorted positions
+  // [offsets[c], offsets[c+1]).
+  std::vector<Tile> tl;
+  tl.reserve(n / kTileQ + nclouds);
+  for (uint32_t c = 0; c < nclouds; ++c)
+    for (uint64_t q0 = offsets[c]; q0 < offsets[c + 1]; q0 += kTileQ)
+      tl.push_back(Tile{(uint32_t)q0, (uint32_t)std::min<uint64_t>(q0 + kTileQ, offsets[c + 1]),
+                        c, 0});
+  CK(ctx->tiles.ensure(sizeof(Tile) * tl.size()));
+  CK(hipMemcpyAsync(ctx->tiles.p, tl.data(), sizeof(Tile) * tl.size(), hipMemcpyHostToDevice,
+                    ctx->st));
+  if (timing) CK(hipEventRecord(ctx->ev[1], ctx->st));
+
+  OutBufs o{out_idx, out_d2};
+  SearchArgs a = base_args(ctx, ctx->spts.as<float4>(), n, r, k, o);
+  a.tiles = ctx->tiles.as<Tile>();
+  a.ntiles = (uint32_t)tl.size();
+  a.local_index = 1;
+  a.persistent = 1;
+  CK(launch_search(a, mode == RTNN_MODE_KNN, gl, ctx->st, ctx->num_cus));
+  if (timing) CK(hipEventRecord(ctx->ev[2], ctx->st));
+  CK(hipStreamSynchronize(ctx->st));
+  if (stats) {
+    std::memset(stats, 0, sizeof *stats);
+    stats->ms_grid = ev_ms(ctx->ev[0], ctx->ev[1]);
+    stats->ms_search = ev_ms(ctx->ev[1], ctx->ev[2]);
+    stats->ms_total = ev_ms(ctx->ev[0], ctx->ev[2]);
+    stats->search_launches = 1;
+    stats->lanes_per_query = (uint32_t)gl;
+    fill_stats(ctx, stats);
+  }
+  return RTNN_OK;
+}
+
+int rtnn_search_percloud(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets,
+                         uint32_t nclouds, int mode, float r, uint32_t k, int32_t* out_idx,
+                         float* out_d2, rtnn_stats* stats) {
+  int rc = check_search_args(ctx, mode, r, k, out_idx);
+  if (rc) return rc;
+  if (!xyz || !offsets || nclouds == 0)
+    return fail(ctx, RTNN_ERR_INVALID, "xyz/offsets must be set and nclouds >= 1");
+  CK(hipSetDevice(ctx->device));
+  CK(hipEventRecord(ctx->ev[6], ctx->st));
+  uint32_t launches = 0;
+  for (uint32_t c = 0; c < nclouds; ++c) {
+    uint64_t n = offsets[c + 1] - offsets[c];
+    if (n == 0) continue;
+    rc = rtnn_set_points(ctx, xyz + 3 * offsets[c], n, 1);
+    if (rc) return rc;
+    rc = search_impl(ctx, nullptr, n, 1, mode, r, k, out_idx + offsets[c] * k,
+                     out_d2 ? out_d2 + offsets[c] * k : nullptr, 1, nullptr);
+    if (rc) return rc;
+    ++launches;
+  }
+  CK(hipEventRecord(ctx->ev[7], ctx->st));
+  CK(hipStreamSynchronize(ctx->st));
+  if (stats) {
+    std::memset(stats, 0, sizeof *stats);
+    stats->ms_total = ev_ms(ctx->ev[6], ctx->ev[7]);
+    stats->search_launches = launches;
+  }
+  return RTNN_OK;
+}
+
+// ------------------------------------------------------ building blocks ---
+
+int rtnn_radix_sort_pairs(rtnn_ctx ctx, uint32_t* keys, uint32_t* vals, uint64_t n,
+                          uint32_t bits) {
+  if (!ctx || (!keys && n) || (!vals && n) || bits > 32) return RTNN_ERR_INVALID;
+  CK(hipSetDevice(ctx->device));
+  CK(ctx->tmp.ensure(4 * radix_tmp_words(n)));
+  CK(radix_sort_pairs(keys, vals, n, (int)bits, ctx->tmp.as<uint32_t>(), ctx->st, nullptr));
+  CK(hipStreamSynchronize(ctx->st));
+  return RTNN_OK;
+}
+
+int rtnn_exclusive_scan(rtnn_ctx ctx, const uint32_t* in, uint32_t* out, uint64_t n) {
+  if (!ctx || !out || (!in && n)) return RTNN_ERR_INVALID;
+  CK(hipSetDevice(ctx->device));
+  CK(ctx->tmp.ensure(4 * (uint64_t)scan_tmp_words(n)));
+  CK(exclusive_scan(in, out, n, ctx->tmp.as<uint32_t>(), ctx->st));
+  CK(hipStreamSynchronize(ctx->st));
+  return RTNN_OK;
+}
+
+int rtnn_morton3(rtnn_ctx ctx, const uint32_t* xyz, uint32_t* codes, uint64_t n) {
+  if (!ctx || ((!xyz || !codes) && n)) return RTNN_ERR_INVALID;
+  CK(hipSetDevice(ctx->device));
+  CK(morton3_codes(xyz, n, codes, ctx->st));
+  CK(hipStreamSynchronize(ctx->st));
+  return RTNN_OK;
+}
+
+}  // extern "C"

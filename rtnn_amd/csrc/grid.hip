@@ -1,0 +1,292 @@
+// Uniform-grid cell list for gfx950.
+//
+// Replaces the reference's grid build (src/optixNSearch/sort.cpp:34-136,
+// 500-627; grid.cu:13-31,213-329): integer bbox via atomics, meta-grid Morton
+// cell ids, atomicAdd local indices (non-deterministic within a cell),
+// Thrust scan and sort.  Here:
+//   bbox        float min/max, block reduction (no atomics on 6 addresses);
+//   cell_keys   raster cell id per point (x fastest) + per-cell count;
+//   scan        exclusive scan of the counts -> cell_start (sort.hip);
+//   radix sort  stable sort of (cell, index) -> points ordered by (cell, index);
+//   gather      sorted points as float4 (x, y, z, bits(original index)) so one
+//               16-byte load yields a candidate and its output index.
+// Query order: Morton code of the query's cell (helper_mortonCode.h:19-34
+// restated), radix-sorted, so a wave's queries walk neighbouring cells.
+#include <float.h>
+
+#include "common.h"
+#include "internal.h"
+
+namespace rtnn {
+
+constexpr int kBboxThreads = 256;
+
+__device__ inline void wave_minmax(float& mn, float& mx) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+    mn = fminf(mn, __shfl_xor(mn, o, kWave));
+    mx = fmaxf(mx, __shfl_xor(mx, o, kWave));
+  }
+}
+
+__device__ inline void block_bbox(float v[6], float* sh /*6 * waves*/) {
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  constexpr int nw = kBboxThreads / kWave;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) wave_minmax(v[a], v[a + 3]);
+  if (lane == 0)
+#pragma unroll
+    for (int a = 0; a < 6; ++a) sh[a * nw + w] = v[a];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int a = 0; a < 3; ++a) {
+      float mn = sh[a * nw], mx = sh[(a + 3) * nw];
+      for (int i = 1; i < nw; ++i) {
+        mn = fminf(mn, sh[a * nw + i]);
+        mx = fmaxf(mx, sh[(a + 3) * nw + i]);
+      }
+      v[a] = mn;
+      v[a + 3] = mx;
+    }
+  }
+}
+
+__device__ inline void accum_point(const float* xyz, uint64_t i, float v[6], uint32_t& bad) {
+  float x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+  if (!(isfinite(x) && isfinite(y) && isfinite(z))) {
+    ++bad;
+    return;
+  }
+  v[0] = fminf(v[0], x); v[1] = fminf(v[1], y); v[2] = fminf(v[2], z);
+  v[3] = fmaxf(v[3], x); v[4] = fmaxf(v[4], y); v[5] = fmaxf(v[5], z);
+}
+
+__global__ __launch_bounds__(kBboxThreads) void k_bbox(const float* __restrict__ xyz,
+                                                       uint64_t n, float* __restrict__ part,
+                                                       uint32_t* __restrict__ nonfinite) {
+  __shared__ float sh[6 * kBboxThreads / kWave];
+  float v[6] = {FLT_MAX, FLT_MAX, FLT_MAX, -FLT_MAX, -FLT_MAX, -FLT_MAX};
+  uint32_t bad = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * kBboxThreads + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * kBboxThreads)
+    accum_point(xyz, i, v, bad);
+  if (bad) atomicAdd(nonfinite, bad);
+  block_bbox(v, sh);
+  if (threadIdx.x == 0)
+    for (int a = 0; a < 6; ++a) part[6 * blockIdx.x + a] = v[a];
+}
+
+__global__ __launch_bounds__(kBboxThreads) void k_bbox_final(const float* __restrict__ part,
+                                                             int np, float* __restrict__ out) {
+  __shared__ float sh[6 * kBboxThreads / kWave];
+  float v[6] = {FLT_MAX, FLT_MAX, FLT_MAX, -FLT_MAX, -FLT_MAX, -FLT_MAX};
+  for (int i = threadIdx.x; i < np; i += kBboxThreads)
+    for (int a = 0; a < 3; ++a) {
+      v[a] = fminf(v[a], part[6 * i + a]);
+      v[a + 3] = fmaxf(v[a + 3], part[6 * i + a + 3]);
+    }
+  block_bbox(v, sh);
+  if (threadIdx.x == 0)
+    for (int a = 0; a < 6; ++a) out[a] = v[a];
+}
+
+hipError_t bbox(const float* xyz, uint64_t n, float* part, float* out, uint32_t* nonfinite,
+                hipStream_t st) {
+  uint64_t want = (n + kBboxThreads * 8 - 1) / (kBboxThreads * 8);
+  int nb = (int)(want < (uint64_t)kBboxBlocks ? (want ? want : 1) : kBboxBlocks);
+  hipLaunchKernelGGL(k_bbox, dim3(nb), dim3(kBboxThreads), 0, st, xyz, n, part, nonfinite);
+  hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(kBboxThreads), 0, st, part, nb, out);
+  return hipGetLastError();
+}
+
+__global__ __launch_bounds__(kBboxThreads) void k_bbox_seg(const float* __restrict__ xyz,
+                                                           const uint64_t* __restrict__ off,
+                                                           float* __restrict__ out,
+                                                           uint32_t* __restrict__ nonfinite) {
+  __shared__ float sh[6 * kBboxThreads / kWave];
+  const uint32_t c = blockIdx.x;
+  float v[6] = {FLT_MAX, FLT_MAX, FLT_MAX, -FLT_MAX, -FLT_MAX, -FLT_MAX};
+  uint32_t bad = 0;
+  for (uint64_t i = off[c] + threadIdx.x; i < off[c + 1]; i += kBboxThreads)
+    accum_point(xyz, i, v, bad);
+  if (bad) atomicAdd(nonfinite, bad);
+  block_bbox(v, sh);
+  if (threadIdx.x == 0)
+    for (int a = 0; a < 6; ++a) out[6 * c + a] = v[a];
+}
+
+hipError_t bbox_segmented(const float* xyz, const uint64_t* offsets, uint32_t nclouds,
+                          float* out, uint32_t* nonfinite, hipStream_t st) {
+  if (nclouds == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_bbox_seg, dim3(nclouds), dim3(kBboxThreads), 0, st, xyz, offsets, out,
+                     nonfinite);
+  return hipGetLastError();
+}
+
+__device__ inline int cell_of(float x, float mn, float inv, int dim) {
+  float c = floorf((x - mn) * inv);
+  c = fminf(fmaxf(c, 0.0f), (float)(dim - 1));
+  return (int)c;
+}
+
+__device__ inline uint32_t find_cloud(const uint64_t* off, uint32_t nclouds, uint64_t i) {
+  // last c with off[c] <= i
+  uint32_t lo = 0, hi = nclouds - 1;
+  while (lo < hi) {
+    uint32_t m = (lo + hi + 1) >> 1;
+    if (off[m] <= i) lo = m; else hi = m - 1;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(256) void k_cell_keys(const float* __restrict__ xyz, uint64_t n,
+                                                   const GridDesc* __restrict__ grids,
+                                                   const uint64_t* __restrict__ off,
+                                                   uint32_t nclouds,
+                                                   uint32_t* __restrict__ keys,
+                                                   uint32_t* __restrict__ vals,
+                                                   uint32_t* __restrict__ counts) {
+  uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  uint32_t c = nclouds > 1 ? find_cloud(off, nclouds, i) : 0;
+  const GridDesc g = grids[c];
+  int cx = cell_of(xyz[3 * i], g.mn[0], g.inv, g.dim[0]);
+  int cy = cell_of(xyz[3 * i + 1], g.mn[1], g.inv, g.dim[1]);
+  int cz = cell_of(xyz[3 * i + 2], g.mn[2], g.inv, g.dim[2]);
+  uint32_t key = g.cell_base + ((uint32_t)cz * (uint32_t)g.dim[1] + (uint32_t)cy) *
+                                   (uint32_t)g.dim[0] + (uint32_t)cx;
+  keys[i] = key;
+  vals[i] = (uint32_t)i;
+  atomicAdd(&counts[key], 1u);
+}
+
+hipError_t cell_keys(const float* xyz, uint64_t n, const GridDesc* grids,
+                     const uint64_t* offsets, uint32_t nclouds, uint32_t* keys,
+                     uint32_t* vals, uint32_t* counts, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_cell_keys, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st, xyz, n,
+                     grids, offsets, nclouds, keys, vals, counts);
+  return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_count_cells(const float* __restrict__ xyz, uint64_t n,
+                                                     const GridDesc* __restrict__ grid,
+                                                     uint32_t* __restrict__ counts) {
+  uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const GridDesc g = *grid;
+  int cx = cell_of(xyz[3 * i], g.mn[0], g.inv, g.dim[0]);
+  int cy = cell_of(xyz[3 * i + 1], g.mn[1], g.inv, g.dim[1]);
+  int cz = cell_of(xyz[3 * i + 2], g.mn[2], g.inv, g.dim[2]);
+  atomicAdd(&counts[((uint32_t)cz * (uint32_t)g.dim[1] + (uint32_t)cy) * (uint32_t)g.dim[0] +
+                    (uint32_t)cx],
+            1u);
+}
+
+hipError_t count_cells(const float* xyz, uint64_t n, const GridDesc* g, uint32_t* counts,
+                       hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_count_cells, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st, xyz,
+                     n, g, counts);
+  return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_gather(const float* __restrict__ xyz,
+                                                const uint32_t* __restrict__ vals, uint64_t n,
+                                                float4* __restrict__ spts) {
+  uint64_t j = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (j >= n) return;
+  uint32_t i = vals[j];
+  spts[j] = make_float4(xyz[3ull * i], xyz[3ull * i + 1], xyz[3ull * i + 2],
+                        __uint_as_float(i));
+}
+
+hipError_t gather_points(const float* xyz, const uint32_t* vals, uint64_t n, float4* spts,
+                         hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gather, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st, xyz, vals,
+                     n, spts);
+  return hipGetLastError();
+}
+
+// Part1By2 of helper_mortonCode.h:19-27: spread the low 10 bits of x so that
+// two zero bits follow each one.
+__host__ __device__ inline uint32_t spread3(uint32_t x) {
+  x &= 0x3ffu;
+  x = (x | (x << 16)) & 0xff0000ffu;
+  x = (x | (x << 8)) & 0x0300f00fu;
+  x = (x | (x << 4)) & 0x030c30c3u;
+  x = (x | (x << 2)) & 0x09249249u;
+  return x;
+}
+__host__ __device__ inline uint32_t morton3(uint32_t x, uint32_t y, uint32_t z) {
+  return (spread3(z) << 2) | (spread3(y) << 1) | spread3(x);
+}
+
+__global__ __launch_bounds__(256) void k_query_morton(const float* __restrict__ q, uint64_t nq,
+                                                      const GridDesc* __restrict__ grid,
+                                                      int shift, uint32_t* __restrict__ keys,
+                                                      uint32_t* __restrict__ vals) {
+  uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= nq) return;
+  const GridDesc g = *grid;
+  // Queries outside the box (or NaN) clamp to the border cells.
+  uint32_t cx = (uint32_t)cell_of(q[3 * i], g.mn[0], g.inv, g.dim[0]) >> shift;
+  uint32_t cy = (uint32_t)cell_of(q[3 * i + 1], g.mn[1], g.inv, g.dim[1]) >> shift;
+  uint32_t cz = (uint32_t)cell_of(q[3 * i + 2], g.mn[2], g.inv, g.dim[2]) >> shift;
+  keys[i] = morton3(cx, cy, cz);
+  vals[i] = (uint32_t)i;
+}
+
+hipError_t query_morton_keys(const float* q, uint64_t nq, const GridDesc* g, int shift,
+                             uint32_t* keys, uint32_t* vals, hipStream_t st) {
+  if (nq == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_query_morton, dim3((uint32_t)((nq + 255) / 256)), dim3(256), 0, st, q,
+                     nq, g, shift, keys, vals);
+  return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_morton(const uint32_t* __restrict__ xyz, uint64_t n,
+                                                uint32_t* __restrict__ codes) {
+  uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  codes[i] = morton3(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+}
+
+hipError_t morton3_codes(const uint32_t* xyz, uint64_t n, uint32_t* codes, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_morton, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st, xyz, n,
+                     codes);
+  return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_fill(uint32_t* __restrict__ p, uint64_t n, uint32_t v) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * 256)
+    p[i] = v;
+}
+
+__global__ __launch_bounds__(256) void k_iota(uint32_t* __restrict__ p, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * 256)
+    p[i] = (uint32_t)i;
+}
+
+static dim3 fill_grid(uint64_t n) {
+  uint64_t b = (n + 255) / 256;
+  return dim3((uint32_t)(b < 65536 ? (b ? b : 1) : 65536));
+}
+
+hipError_t fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill, fill_grid(n), dim3(256), 0, st, p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t iota_u32(uint32_t* p, uint64_t n, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_iota, fill_grid(n), dim3(256), 0, st, p, n);
+  return hipGetLastError();
+}
+
+}  // namespace rtnn

@@ -1,0 +1,57 @@
+// Host-side declarations shared by the library's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "common.h"
+
+namespace rtnn {
+
+// ---- sort.hip
+int scan_tmp_words(uint64_t n);
+hipError_t exclusive_scan(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* tmp,
+                          hipStream_t st);
+uint64_t radix_tmp_words(uint64_t n);
+hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint64_t n, int bits,
+                            uint32_t* tmp, hipStream_t st, int* passes_out);
+
+// ---- grid.hip
+constexpr int kBboxBlocks = 1024;
+// part: 6 * kBboxBlocks floats; out: 6 floats (min xyz, max xyz); nonfinite: 1 u32.
+hipError_t bbox(const float* xyz, uint64_t n, float* part, float* out, uint32_t* nonfinite,
+                hipStream_t st);
+// Per-cloud bbox: out[6 * c .. 6 * c + 6) for cloud c (offsets: nclouds + 1).
+hipError_t bbox_segmented(const float* xyz, const uint64_t* offsets, uint32_t nclouds,
+                          float* out, uint32_t* nonfinite, hipStream_t st);
+// keys[i] = grid cell of point i, vals[i] = i, counts[key] += 1.
+hipError_t cell_keys(const float* xyz, uint64_t n, const GridDesc* grids,
+                     const uint64_t* offsets, uint32_t nclouds, uint32_t* keys,
+                     uint32_t* vals, uint32_t* counts, hipStream_t st);
+// spts[j] = (xyz[vals[j]], bits(vals[j])).
+hipError_t gather_points(const float* xyz, const uint32_t* vals, uint64_t n, float4* spts,
+                         hipStream_t st);
+// Morton key (10 bits per axis) of each query's cell in grid g, shifted right
+// by `shift` bits per axis; vals[i] = i.
+hipError_t query_morton_keys(const float* q, uint64_t nq, const GridDesc* g, int shift,
+                             uint32_t* keys, uint32_t* vals, hipStream_t st);
+hipError_t morton3_codes(const uint32_t* xyz, uint64_t n, uint32_t* codes, hipStream_t st);
+hipError_t fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t st);
+hipError_t iota_u32(uint32_t* p, uint64_t n, hipStream_t st);
+// Coarse count grid: counts[cell of point] += 1 (grid g, single cloud).
+hipError_t count_cells(const float* xyz, uint64_t n, const GridDesc* g, uint32_t* counts,
+                       hipStream_t st);
+
+// ---- search.hip
+// Runs the top-k search.  group_lanes = 1 selects the thread-per-query kernel
+// (register-resident sorted top-k, k <= 64); 4, 16 or 64 select the group
+// kernel (ballot compaction into an LDS buffer, rank-sorted output).
+// Returns hipErrorInvalidValue when no kernel variant covers (k, group_lanes).
+hipError_t launch_search(const SearchArgs& a, int knn_mode, int group_lanes, hipStream_t st,
+                         int num_cus);
+// Largest k supported for a group width (0 = unsupported width).
+uint32_t max_k_for_group(int group_lanes);
+
+}  // namespace rtnn

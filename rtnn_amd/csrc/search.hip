@@ -1,0 +1,369 @@
+// Cell-walk search kernels for gfx950.
+//
+// They replace the reference's OptiX programs: __raygen__radius/__raygen__knn
+// (src/optixNSearch/camera.cu:39-145) and the intersection programs
+// (geometry.cu:38-191).  The per-candidate test is the reference's: d2 =
+// dot(q - p, q - p) in FP32 against R*R, strict; range search keeps the query
+// itself (geometry.cu:65), KNN drops d2 == 0 (geometry.cu:187).  Where the
+// reference keeps the first `limit` hits in traversal order (range,
+// geometry.cu:72-83) or an unsorted replace-max array (KNN,
+// geometry.cu:108-155), these kernels return the k smallest by (d2, index),
+// sorted, so the result is deterministic.
+//
+// A query visits the cells overlapping [q - R, q + R] (padded, see
+// cell_range).  Cells are raster ordered with x fastest, so each (y, z) row of
+// the block is one contiguous run of sorted points: a 3x3x3 walk is 9 runs.
+//
+// k_topk_thread : one query per lane.  A sorted array of k 64-bit keys
+//                 (d2 bits << 32 | index) lives in VGPRs; insertion is an
+//                 unrolled compare/select pass, so nothing spills to scratch.
+// k_topk_group  : G lanes per query (G = 4, 16, 64).  The lanes stride over the
+//                 query's runs flattened into one index space, test 16-byte
+//                 candidates with coalesced loads, and append hits to an LDS
+//                 buffer with one wave64 ballot + prefix popcount.  When the
+//                 buffer nears capacity the k smallest are kept (rank select)
+//                 and later hits must beat the k-th.  The final rank sort
+//                 writes the row in (d2, index) order.
+#include <float.h>
+
+#include "common.h"
+#include "internal.h"
+
+namespace rtnn {
+
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ inline uint64_t lanemask_lt64() {
+  const int lane = threadIdx.x & (kWave - 1);
+  return lane == 0 ? 0ull : (~0ull >> (kWave - lane));
+}
+
+// Resolves tile t into its query range and grid.
+__device__ inline void tile_of(const SearchArgs& a, uint32_t t, uint32_t& q0, uint32_t& q1,
+                               uint32_t& gi) {
+  if (a.tiles) {
+    Tile tl = a.tiles[t];
+    q0 = tl.q0; q1 = tl.q1; gi = tl.grid;
+  } else {
+    q0 = t * (uint32_t)kTileQ;
+    q1 = min(q0 + (uint32_t)kTileQ, a.nq);
+    gi = 0;
+  }
+}
+
+// Persistent launches walk tiles in XCD-contiguous chunks; plain launches map
+// each block to one tile through xcd_swizzle.
+struct TileIter {
+  uint32_t t, end, step;
+  __device__ TileIter(const SearchArgs& a) {
+    if (a.persistent) {
+      constexpr uint32_t kXcd = 8;
+      uint32_t x = blockIdx.x % kXcd, j = blockIdx.x / kXcd;
+      uint32_t per = (gridDim.x + kXcd - 1 - x) / kXcd;  // blocks on this XCD
+      uint32_t q = a.ntiles / kXcd, rem = a.ntiles % kXcd;
+      uint32_t start = x * q + (x < rem ? x : rem);
+      end = start + q + (x < rem ? 1u : 0u);
+      t = start + j;
+      step = per;
+    } else {
+      t = xcd_swizzle(blockIdx.x, gridDim.x);
+      end = t + 1;
+      step = 1;
+    }
+  }
+};
+
+// --------------------------------------------------------------------------
+// thread per query
+
+template <int KC>
+__device__ inline void topk_insert(uint64_t (&list)[KC], uint64_t key) {
+#pragma unroll
+  for (int i = KC - 1; i > 0; --i) {
+    const uint64_t prev = list[i - 1];
+    const uint64_t cur = list[i];
+    list[i] = prev > key ? prev : (cur > key ? key : cur);
+  }
+  list[0] = list[0] > key ? key : list[0];
+}
+
+template <int KC, bool KNN>
+__global__ __launch_bounds__(kTileQ) void k_topk_thread(const SearchArgs a) {
+  for (TileIter it(a); it.t < it.end; it.t += it.step) {
+    uint32_t q0, q1, gi;
+    tile_of(a, it.t, q0, q1, gi);
+    const uint32_t qs = q0 + threadIdx.x;
+    if (qs >= q1) continue;
+    const GridDesc g = a.grids[gi];
+    const float4 qv = a.queries[qs];
+    const uint32_t row = __float_as_uint(qv.w);
+    const float R = a.qradius ? a.qradius[row] : a.r;
+    const float R2 = R * R;
+    int lo0, hi0, lo1, hi1, lo2, hi2;
+    cell_range(g, qv.x, R, 0, lo0, hi0);
+    cell_range(g, qv.y, R, 1, lo1, hi1);
+    cell_range(g, qv.z, R, 2, lo2, hi2);
+
+    uint64_t list[KC];
+#pragma unroll
+    for (int i = 0; i < KC; ++i) list[i] = ~0ull;
+
+    for (int cz = lo2; cz <= hi2; ++cz) {
+      for (int cy = lo1; cy <= hi1; ++cy) {
+        const uint32_t rowc =
+            g.cell_base + ((uint32_t)cz * (uint32_t)g.dim[1] + (uint32_t)cy) * (uint32_t)g.dim[0];
+        if (lo0 > hi0) break;
+        const uint32_t s0 = a.cell_start[rowc + lo0];
+        const uint32_t s1 = a.cell_start[rowc + hi0 + 1];
+        for (uint32_t j = s0; j < s1; ++j) {
+          const float4 p = a.spts[j];
+          const float d2 = d2_of(qv.x, qv.y, qv.z, p.x, p.y, p.z);
+          const bool ok = KNN ? (d2 > 0.0f && d2 < R2) : (d2 < R2);
+          if (ok) {
+            const uint64_t key = pack_key(d2, __float_as_uint(p.w));
+            if (key < list[KC - 1]) topk_insert<KC>(list, key);
+          }
+        }
+      }
+    }
+
+    const uint32_t k = a.k;
+    const uint32_t base = a.local_index ? g.point_base : 0u;
+    int32_t* oi = a.out_idx + (uint64_t)row * k;
+    float* od = a.out_d2 ? a.out_d2 + (uint64_t)row * k : nullptr;
+#pragma unroll
+    for (int i = 0; i < KC; ++i) {
+      if ((uint32_t)i < k) {
+        const bool have = list[i] != ~0ull;
+        oi[i] = have ? (int32_t)(key_idx(list[i]) - base) : -1;
+        if (od) od[i] = have ? key_d2(list[i]) : __builtin_huge_valf();
+      }
+    }
+    if (a.short_rows) {
+      bool is_short = false;
+#pragma unroll
+      for (int i = 0; i < KC; ++i)
+        if ((uint32_t)i == k - 1) is_short = list[i] == ~0ull;
+      if (is_short) a.short_rows[atomicAdd(a.short_count, 1u)] = qs;
+    }
+  }
+}
+
+// --------------------------------------------------------------------------
+// G lanes per query
+
+template <int G>
+__device__ inline uint32_t group_incl_scan(uint32_t v, int l) {
+#pragma unroll
+  for (int o = 1; o < G; o <<= 1) {
+    uint32_t t = __shfl_up(v, o, G);
+    if (l >= o) v += t;
+  }
+  return v;
+}
+
+template <int G, int KB>
+struct GroupCfg {
+  static constexpr int NG = kTileQ / G;                  // groups per block
+  static constexpr int CAP = ((2 * KB + G) + G - 1) / G * G;  // LDS keys per group
+  static constexpr int EPL = CAP / G;                    // buffer entries per lane
+};
+
+// Ranks of this lane's buffer entries among the first cnt entries.
+template <int G, int EPL>
+__device__ inline void rank_entries(const uint64_t* buf, uint32_t cnt, int l,
+                                    uint64_t (&kv)[EPL], uint32_t (&rk)[EPL]) {
+#pragma unroll
+  for (int i = 0; i < EPL; ++i) {
+    const uint32_t e = (uint32_t)(l + i * G);
+    kv[i] = e < cnt ? buf[e] : ~0ull;
+    rk[i] = 0;
+  }
+  for (uint32_t j = 0; j < cnt; ++j) {
+    const uint64_t b = buf[j];
+#pragma unroll
+    for (int i = 0; i < EPL; ++i) rk[i] += b < kv[i] ? 1u : 0u;
+  }
+}
+
+template <int G, int KB, bool KNN>
+__global__ __launch_bounds__(kTileQ) void k_topk_group(const SearchArgs a) {
+  using C = GroupCfg<G, KB>;
+  __shared__ uint64_t s_buf[C::NG][C::CAP];
+  __shared__ uint32_t s_rs[C::NG][G];
+  __shared__ uint32_t s_ro[C::NG][G + 1];
+
+  const int gq = threadIdx.x / G;
+  const int l = threadIdx.x % G;
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t gmask = G == kWave ? ~0ull : (((1ull << G) - 1ull) << (lane - l));
+  uint64_t* buf = s_buf[gq];
+  uint32_t* rs = s_rs[gq];
+  uint32_t* ro = s_ro[gq];
+  const uint32_t k = a.k;
+
+  for (TileIter it(a); it.t < it.end; it.t += it.step) {
+    uint32_t q0, q1, gi;
+    tile_of(a, it.t, q0, q1, gi);
+    const GridDesc g = a.grids[gi];
+    for (uint32_t qs = q0 + gq; qs < q1; qs += C::NG) {
+      const float4 qv = a.queries[qs];
+      const uint32_t row = __float_as_uint(qv.w);
+      const float R = a.qradius ? a.qradius[row] : a.r;
+      const float R2 = R * R;
+      int lo0, hi0, lo1, hi1, lo2, hi2;
+      cell_range(g, qv.x, R, 0, lo0, hi0);
+      cell_range(g, qv.y, R, 1, lo1, hi1);
+      cell_range(g, qv.z, R, 2, lo2, hi2);
+      const int ny = hi1 - lo1 + 1, nz = hi2 - lo2 + 1;
+      const int nrows = (hi0 >= lo0 && ny > 0 && nz > 0) ? ny * nz : 0;
+
+      uint32_t cnt = 0;
+      uint64_t thr = ~0ull;
+      for (int rb = 0; rb < nrows; rb += G) {
+        const int j = rb + l;
+        uint32_t s0 = 0, len = 0;
+        if (j < nrows) {
+          const int cy = lo1 + j % ny, cz = lo2 + j / ny;
+          const uint32_t rowc = g.cell_base +
+                                ((uint32_t)cz * (uint32_t)g.dim[1] + (uint32_t)cy) *
+                                    (uint32_t)g.dim[0];
+          s0 = a.cell_start[rowc + lo0];
+          len = a.cell_start[rowc + hi0 + 1] - s0;
+        }
+        const uint32_t incl = group_incl_scan<G>(len, l);
+        const uint32_t total = __shfl(incl, G - 1, G);
+        rs[l] = s0;
+        ro[l + 1] = incl;
+        if (l == 0) ro[0] = 0;
+        wave_sync();
+        int rp = 0;
+        for (uint32_t v0 = 0; v0 < total; v0 += G) {
+          const uint32_t v = v0 + (uint32_t)l;
+          bool hit = false;
+          uint64_t key = 0;
+          if (v < total) {
+            while (ro[rp + 1] <= v) ++rp;
+            const float4 p = a.spts[rs[rp] + (v - ro[rp])];
+            const float d2 = d2_of(qv.x, qv.y, qv.z, p.x, p.y, p.z);
+            const bool ok = KNN ? (d2 > 0.0f && d2 < R2) : (d2 < R2);
+            key = pack_key(d2, __float_as_uint(p.w));
+            hit = ok && key < thr;
+          }
+          const uint64_t m = __ballot(hit) & gmask;
+          if (hit) buf[cnt + (uint32_t)__popcll(m & lanemask_lt64())] = key;
+          cnt += (uint32_t)__popcll(m);
+          if (cnt > (uint32_t)(C::CAP - G)) {
+            // Keep the k smallest; later hits must beat the k-th.
+            wave_sync();
+            uint64_t kv[C::EPL];
+            uint32_t rk[C::EPL];
+            rank_entries<G, C::EPL>(buf, cnt, l, kv, rk);
+            wave_sync();
+#pragma unroll
+            for (int i = 0; i < C::EPL; ++i)
+              if ((uint32_t)(l + i * G) < cnt && rk[i] < k) buf[rk[i]] = kv[i];
+            wave_sync();
+            cnt = k;
+            thr = buf[k - 1];
+          }
+        }
+        wave_sync();
+      }
+
+      // Final: rank-sort the buffer and write the row.
+      wave_sync();
+      uint64_t kv[C::EPL];
+      uint32_t rk[C::EPL];
+      rank_entries<G, C::EPL>(buf, cnt, l, kv, rk);
+      const uint32_t base = a.local_index ? g.point_base : 0u;
+      int32_t* oi = a.out_idx + (uint64_t)row * k;
+      float* od = a.out_d2 ? a.out_d2 + (uint64_t)row * k : nullptr;
+#pragma unroll
+      for (int i = 0; i < C::EPL; ++i) {
+        if ((uint32_t)(l + i * G) < cnt && rk[i] < k) {
+          oi[rk[i]] = (int32_t)(key_idx(kv[i]) - base);
+          if (od) od[rk[i]] = key_d2(kv[i]);
+        }
+      }
+      for (uint32_t s = cnt + (uint32_t)l; s < k; s += G) {
+        oi[s] = -1;
+        if (od) od[s] = __builtin_huge_valf();
+      }
+      if (a.short_rows && l == 0 && cnt < k) a.short_rows[atomicAdd(a.short_count, 1u)] = qs;
+      wave_sync();
+    }
+  }
+}
+
+// --------------------------------------------------------------------------
+// dispatch
+
+template <int KC>
+static void launch_thread(const SearchArgs& a, bool knn, dim3 grid, hipStream_t st) {
+  if (knn)
+    hipLaunchKernelGGL((k_topk_thread<KC, true>), grid, dim3(kTileQ), 0, st, a);
+  else
+    hipLaunchKernelGGL((k_topk_thread<KC, false>), grid, dim3(kTileQ), 0, st, a);
+}
+
+template <int G, int KB>
+static void launch_group(const SearchArgs& a, bool knn, dim3 grid, hipStream_t st) {
+  if (knn)
+    hipLaunchKernelGGL((k_topk_group<G, KB, true>), grid, dim3(kTileQ), 0, st, a);
+  else
+    hipLaunchKernelGGL((k_topk_group<G, KB, false>), grid, dim3(kTileQ), 0, st, a);
+}
+
+uint32_t max_k_for_group(int group_lanes) {
+  switch (group_lanes) {
+    case 1: return 64;
+    case 4: return 64;
+    case 16: return 256;
+    case 64: return 1024;
+    default: return 0;
+  }
+}
+
+hipError_t launch_search(const SearchArgs& a0, int knn_mode, int group_lanes, hipStream_t st,
+                         int num_cus) {
+  SearchArgs a = a0;
+  if (a.ntiles == 0) return hipSuccess;
+  const uint32_t k = a.k;
+  if (k == 0 || k > max_k_for_group(group_lanes)) return hipErrorInvalidValue;
+  uint32_t blocks = a.ntiles;
+  if (a.persistent) {
+    uint32_t cap = (uint32_t)(num_cus > 0 ? num_cus : 256) * 8u;
+    if (blocks > cap) blocks = cap;
+  }
+  const dim3 grid(blocks);
+  const bool knn = knn_mode != 0;
+  if (group_lanes == 1) {
+    if (k <= 1) launch_thread<1>(a, knn, grid, st);
+    else if (k <= 2) launch_thread<2>(a, knn, grid, st);
+    else if (k <= 4) launch_thread<4>(a, knn, grid, st);
+    else if (k <= 8) launch_thread<8>(a, knn, grid, st);
+    else if (k <= 16) launch_thread<16>(a, knn, grid, st);
+    else if (k <= 32) launch_thread<32>(a, knn, grid, st);
+    else launch_thread<64>(a, knn, grid, st);
+  } else if (group_lanes == 4) {
+    if (k <= 16) launch_group<4, 16>(a, knn, grid, st);
+    else launch_group<4, 64>(a, knn, grid, st);
+  } else if (group_lanes == 16) {
+    if (k <= 16) launch_group<16, 16>(a, knn, grid, st);
+    else if (k <= 64) launch_group<16, 64>(a, knn, grid, st);
+    else launch_group<16, 256>(a, knn, grid, st);
+  } else {
+    if (k <= 16) launch_group<64, 16>(a, knn, grid, st);
+    else if (k <= 64) launch_group<64, 64>(a, knn, grid, st);
+    else if (k <= 256) launch_group<64, 256>(a, knn, grid, st);
+    else launch_group<64, 1024>(a, knn, grid, st);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace rtnn

@@ -1,0 +1,226 @@
+"""GPU parity of the search path through the C ABI against the CPU oracle.
+
+Bar: indices bit-exact and d2 bit-exact (FP32, no FMA on either side), rows in
+(d2, index) order, -1 / +inf in empty slots.  Covers the reference's own data
+file, the workload shapes at oracle-checkable sizes, every kernel variant
+(thread-per-query and 4/16/64-lane groups, cell factors), edge cases (empty
+rows, duplicates, one hot cell, far and NaN queries, k above the hit count,
+buffer overflow of the group kernel), errors, batched clouds and determinism.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import rtnn_amd
+from conftest import golden_d2
+
+pytestmark = pytest.mark.gpu
+
+MODES = {"radius": oracle.MODE_RADIUS, "knn": oracle.MODE_KNN}
+
+
+def check(ctx, pts, qs, mode, r, k, qsel=None, **search_kw):
+    idx, d2, st = ctx.search(qs, mode=mode, radius=r, k=k, **search_kw)
+    if hasattr(idx, "cpu"):
+        idx, d2 = idx.cpu().numpy(), d2.cpu().numpy()
+    qarr = pts if qs is None else (qs.cpu().numpy() if hasattr(qs, "cpu") else qs)
+    parr = pts.cpu().numpy() if hasattr(pts, "cpu") else pts
+    wi, wd = oracle.grid_search(parr, qarr, MODES[mode], r, k, qsel=qsel, threads=8)
+    gi, gd = (idx, d2) if qsel is None else (idx[qsel], d2[qsel])
+    bad = np.nonzero((gi != wi).any(axis=1))[0]
+    assert len(bad) == 0, f"{len(bad)} rows differ, first {bad[:5]}: got {gi[bad[0]]} want {wi[bad[0]]}"
+    assert np.array_equal(gd, wd)
+    return st
+
+
+@pytest.fixture(autouse=True)
+def _reset_opts(ctx):
+    ctx.set_options()
+    yield
+    ctx.set_options()
+
+
+def test_samplepc_golden(ctx, golden, samplepc):
+    ctx.set_points(samplepc)
+    for case in golden["cases"]:
+        for qpw in (0, 1, 4, 16, 64):
+            if case["k"] > 64 and qpw in (1, 4):
+                continue
+            ctx.set_options(queries_per_wave=qpw)
+            idx, d2, _ = ctx.search(None, mode=case["mode"], radius=case["r"], k=case["k"])
+            assert idx.tolist() == case["idx"], (case["mode"], case["r"], case["k"], qpw)
+            assert np.array_equal(d2, golden_d2(case["d2"]))
+            idx, d2, _ = ctx.search(samplepc, mode=case["mode"], radius=case["r"], k=case["k"])
+            assert idx.tolist() == case["idx"]
+
+
+def test_device_tensors_in_and_out(ctx, samplepc):
+    import torch
+    p = torch.from_numpy(samplepc).cuda()
+    ctx.set_points(p)
+    idx, d2, _ = ctx.search(p, mode="knn", radius=10.0, k=3)
+    assert idx.is_cuda and idx.cpu().tolist()[0] == [2, 4, 6]
+
+
+def test_config1_range_full(ctx):
+    # 100,000 points U[0,50)^3, queries = points, r = 2, K = 50 (range).
+    pts = rtnn_amd.gen_uniform(1, 100000, 0.0, 50.0)
+    ctx.set_points(pts)
+    st = check(ctx, pts, None, "radius", 2.0, 50)
+    assert st.search_launches == 1
+    check(ctx, pts, None, "knn", 2.0, 50)
+
+
+@pytest.mark.parametrize("qpw", [1, 4, 16, 64])
+@pytest.mark.parametrize("cf", [0.5, 1.0, 2.0, 3.0])
+def test_kernel_variants_and_cell_factors(ctx, qpw, cf):
+    rng = np.random.default_rng(int(cf * 10) + qpw)
+    pts = rng.uniform(0, 20, (20000, 3)).astype(np.float32)
+    qs = np.concatenate([pts[:3000], rng.uniform(-3, 23, (2000, 3)).astype(np.float32)])
+    ctx.set_points(pts)
+    ctx.set_options(cell_factor=cf, queries_per_wave=qpw)
+    for mode in ("radius", "knn"):
+        for k in (1, 5, 16, 33, 64):
+            check(ctx, pts, qs, mode, 1.5, k)
+
+
+def test_large_k_group_kernel(ctx):
+    rng = np.random.default_rng(4)
+    pts = rng.uniform(0, 10, (30000, 3)).astype(np.float32)
+    ctx.set_points(pts)
+    for qpw, k in [(16, 100), (16, 256), (64, 300), (64, 1024), (0, 200)]:
+        ctx.set_options(queries_per_wave=qpw)
+        check(ctx, pts, pts[:2000], "radius", 1.2, k)
+        check(ctx, pts, pts[:500], "knn", 1.2, k)
+
+
+def test_group_buffer_overflow_path(ctx):
+    # Thousands of hits per query force the keep-k-smallest select of the LDS
+    # buffer many times before the final sort.
+    rng = np.random.default_rng(8)
+    pts = rng.normal(0, 1.0, (60000, 3)).astype(np.float32)
+    ctx.set_points(pts)
+    for qpw in (4, 16, 64):
+        ctx.set_options(queries_per_wave=qpw)
+        check(ctx, pts, pts[:1500], "radius", 2.0, 16)
+        check(ctx, pts, pts[:1500], "knn", 2.0, 64)
+
+
+def test_knn_separate_queries_gaussian_mixture(ctx):
+    # Config-2 shape at 1/10 size: clusters + background, separate queries,
+    # K = 8, r = 20; many background rows are short (sentinels).
+    pts = rtnn_amd.gen_gaussian_mixture(2, 20, 100000)
+    qs = rtnn_amd.gen_gaussian_mixture(2, 21, 100000)
+    ctx.set_points(pts)
+    check(ctx, pts, qs, "knn", 20.0, 8)
+    check(ctx, pts, qs, "radius", 20.0, 8)
+
+
+def test_spheres_knn16(ctx):
+    # Config-3 shape at 1/50 size: sphere surfaces, K = 16, r = 10.
+    pts = rtnn_amd.gen_spheres(3, 30, 200000, nspheres=40)
+    qs = rtnn_amd.gen_spheres(3, 31, 100000, nspheres=40)
+    ctx.set_points(pts)
+    check(ctx, pts, qs, "knn", 10.0, 16)
+
+
+def test_edge_cases(ctx):
+    rng = np.random.default_rng(12)
+    # duplicates + one hot cell
+    base = rng.uniform(0, 1e-3, (500, 3)).astype(np.float32)
+    pts = np.concatenate([base, base[:100], rng.uniform(0, 10, (500, 3)).astype(np.float32)])
+    ctx.set_points(pts)
+    for k in (3, 50, 128):
+        check(ctx, pts, pts, "radius", 0.5, k)
+        check(ctx, pts, pts, "knn", 0.5, k)
+    # far queries, a NaN query, a query on a grid corner
+    qs = np.array([[1e6, 1e6, 1e6], [-1e6, 0, 0], [np.nan, 0, 0], [0, 0, 0], [10, 10, 10],
+                   [5, 5, 5]], np.float32)
+    idx, d2, _ = ctx.search(qs, mode="radius", radius=2.0, k=8)
+    assert (idx[:3] == -1).all() and np.isinf(d2[:3]).all()
+    check(ctx, pts, qs[3:], "knn", 2.0, 8)
+    # radius larger than the whole box
+    check(ctx, pts, pts[:200], "knn", 1e4, 20)
+    # a single point
+    one = np.array([[1, 2, 3]], np.float32)
+    ctx.set_points(one)
+    idx, _, _ = ctx.search(None, mode="radius", radius=1.0, k=4)
+    assert idx.tolist() == [[0, -1, -1, -1]]
+    idx, _, _ = ctx.search(None, mode="knn", radius=1.0, k=4)
+    assert idx.tolist() == [[-1, -1, -1, -1]]
+
+
+def test_boundary_pairs_listed_and_exact(ctx):
+    # Lattice spacing 0.1 with r = 0.2: many pairs sit on the sphere boundary.
+    xyz, offs = rtnn_amd.gen_lattice_clouds(4, 1, 3000, 3000, jit=0.0)
+    ctx.set_points(xyz)
+    check(ctx, xyz, None, "radius", 0.2, 64)
+    cnt, _ = oracle.boundary_pairs(xyz, xyz, 0.2, qsel=np.arange(200))
+    assert cnt > 0  # the boundary is exercised (and the check above was exact)
+
+
+def test_errors(ctx, samplepc):
+    ctx.set_points(samplepc)
+    with pytest.raises(rtnn_amd.RtnnError):
+        ctx.search(None, mode="radius", radius=0.0, k=4)
+    with pytest.raises(rtnn_amd.RtnnError):
+        ctx.search(None, mode="radius", radius=1.0, k=0)
+    with pytest.raises(rtnn_amd.RtnnError):
+        ctx.search(None, mode="radius", radius=float("nan"), k=4)
+    with pytest.raises(rtnn_amd.RtnnError):
+        ctx.search(None, mode="radius", radius=1.0, k=5000)
+    bad = samplepc.copy()
+    bad[3, 1] = np.inf
+    with pytest.raises(rtnn_amd.RtnnError):
+        ctx.set_points(bad)
+    with pytest.raises(rtnn_amd.RtnnError):
+        ctx.set_options(queries_per_wave=3)
+
+
+def test_deterministic(ctx):
+    pts = rtnn_amd.gen_uniform(7, 50000, 0.0, 30.0)
+    ctx.set_points(pts)
+    a, ad, _ = ctx.search(None, mode="radius", radius=1.5, k=32)
+    b, bd, _ = ctx.search(None, mode="radius", radius=1.5, k=32)
+    assert np.array_equal(a, b) and np.array_equal(ad.view(np.int32), bd.view(np.int32))
+
+
+@pytest.mark.parametrize("qpw", [0, 1, 16])
+def test_batched_clouds(ctx, qpw):
+    import torch
+    xyz, offs = rtnn_amd.gen_lattice_clouds(9, 40, 1000, 8000)
+    ctx.set_options(queries_per_wave=qpw)
+    t = torch.from_numpy(xyz).cuda()
+    bi, bd, st = ctx.search_batched(t, offs, mode="radius", radius=0.2, k=64)
+    pi, pd, _ = ctx.search_percloud(t, offs, mode="radius", radius=0.2, k=64)
+    bi, bd = bi.cpu().numpy(), bd.cpu().numpy()
+    assert np.array_equal(bi, pi.cpu().numpy())
+    assert np.array_equal(bd, pd.cpu().numpy())
+    for c in (0, 17, 39):
+        a, b = int(offs[c]), int(offs[c + 1])
+        cloud = xyz[a:b]
+        wi, wd = oracle.grid_search(cloud, cloud, oracle.MODE_RADIUS, 0.2, 64, threads=8)
+        assert np.array_equal(bi[a:b], wi) and np.array_equal(bd[a:b], wd)
+    ki, kd, _ = ctx.search_batched(t, offs, mode="knn", radius=0.25, k=8)
+    a, b = int(offs[5]), int(offs[6])
+    wi, wd = oracle.grid_search(xyz[a:b], xyz[a:b], oracle.MODE_KNN, 0.25, 8, threads=8)
+    assert np.array_equal(ki.cpu().numpy()[a:b], wi)
+
+
+@pytest.mark.slow
+def test_config3_full_size_sampled(ctx):
+    # Full config-3 shape: 10M sphere-surface points, 10M queries, K=16, r=10;
+    # 4000 sampled rows checked bit-exactly, all rows checked for invariants.
+    pts = rtnn_amd.gen_spheres(3, 30, 10_000_000)
+    qs = rtnn_amd.gen_spheres(3, 31, 10_000_000)
+    ctx.set_points(pts)
+    idx, d2, st = ctx.search(qs, mode="knn", radius=10.0, k=16)
+    sel = np.random.default_rng(0).choice(len(qs), 4000, replace=False).astype(np.uint64)
+    wi, wd = oracle.grid_search(pts, qs, oracle.MODE_KNN, 10.0, 16, qsel=sel, threads=8)
+    s = sel.astype(np.int64)
+    assert np.array_equal(idx[s], wi) and np.array_equal(d2[s], wd)
+    # size-independent invariants over every row
+    assert (np.diff(d2, axis=1) >= 0).all()
+    valid = idx >= 0
+    assert (valid[:, 1:] <= valid[:, :-1]).all()  # sentinels only at the tail
+    assert (d2[valid] > 0).all() and (d2[valid] < np.float32(100)).all()
