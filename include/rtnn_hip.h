@@ -122,10 +122,12 @@ int rtnn_search_partitioned(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on
 int rtnn_mlp_train(rtnn_ctx ctx, float r, uint32_t k, uint32_t n_samples,
                    uint32_t steps, uint32_t batch, float lr, float tau,
                    uint64_t seed, float* final_loss);
-/* Weights as FP32: [w1 64x30 | b1 64 | w2 64x64 | b2 64 | w3 64 | b3 1] plus
- * normalisation [2] = 6275 floats. */
-#define RTNN_MLP_NPARAMS 6273
-#define RTNN_MLP_NBLOB   6275
+/* Weights as FP32, row-major [out][in]:
+ *   [w1 64x30 | b1 64 | w2 64x64 | b2 64 | w3 64 | b3 1]  (6209 parameters)
+ * followed by [count_scale, radius_margin] (feature scale of the log1p counts,
+ * multiplier applied to the proposed radius) = 6211 floats. */
+#define RTNN_MLP_NPARAMS 6209
+#define RTNN_MLP_NBLOB   6211
 int rtnn_mlp_get_weights(rtnn_ctx ctx, float* blob /* RTNN_MLP_NBLOB */);
 int rtnn_mlp_set_weights(rtnn_ctx ctx, const float* blob /* RTNN_MLP_NBLOB */);
 

@@ -21,7 +21,8 @@ LIB_PATH = os.path.join(HERE, "librtnn_hip.so")
 
 MODE_RADIUS = 0
 MODE_KNN = 1
-MLP_NBLOB = 6275
+MLP_NBLOB = 6211
+MLP_NPARAMS = 6209
 
 OK = 0
 ERR_INVALID = -1
@@ -302,6 +303,36 @@ class Context:
     def search_percloud(self, xyz, offsets, mode="radius", radius=0.2, k=64, with_d2=True):
         return self._batched(self._L.rtnn_search_percloud, xyz, offsets, mode, radius, k,
                              with_d2)
+
+    # MLP building blocks (device tensors)
+    def mlp_features(self, queries, radius):
+        import torch
+        q = _as_points(queries)
+        out = torch.empty((len(q), 32), dtype=torch.float32, device="cuda")
+        self._check(self._L.rtnn_mlp_features(self._h, ctypes.c_void_p(q.data_ptr()), len(q),
+                                              radius, ctypes.c_void_p(out.data_ptr())))
+        return out
+
+    def mlp_forward(self, feats):
+        import torch
+        feats = feats.to(torch.float32).contiguous()
+        out = torch.empty(feats.shape[0], dtype=torch.float32, device="cuda")
+        self._check(self._L.rtnn_mlp_forward(self._h, ctypes.c_void_p(feats.data_ptr()),
+                                             feats.shape[0], ctypes.c_void_p(out.data_ptr())))
+        return out
+
+    def mlp_backward(self, feats, target, tau=0.9):
+        import torch
+        feats = feats.to(torch.float32).contiguous()
+        target = target.to(torch.float32).contiguous()
+        grads = torch.empty(MLP_NPARAMS, dtype=torch.float32, device="cuda")
+        loss = ctypes.c_float(0)
+        self._check(self._L.rtnn_mlp_backward(self._h, ctypes.c_void_p(feats.data_ptr()),
+                                              ctypes.c_void_p(target.data_ptr()),
+                                              feats.shape[0], tau,
+                                              ctypes.c_void_p(grads.data_ptr()),
+                                              ctypes.byref(loss)))
+        return grads, float(loss.value)
 
     # building blocks (device tensors)
     def radix_sort_pairs(self, keys, vals, bits=32):

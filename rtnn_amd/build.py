@@ -30,7 +30,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall",
 HIP_FLAGS = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
              "-packed-fp32-ops", "-fno-gpu-rdc"]
 
-SOURCES = ["sort.hip", "grid.hip", "search.hip", "mlp.hip", "api.hip", "data.cpp"]
+SOURCES = ["sort.hip", "grid.hip", "search.hip", "partition.hip", "mlp.hip", "api.hip", "data.cpp"]
 
 
 def _run(cmd, verbose):
@@ -69,7 +69,9 @@ def build(verbose: bool = False, jobs: int = 6) -> str:
     cli_src = os.path.join(CSRC, "cli.cpp")
     if os.path.exists(cli_src):
         tmp = CLI + ".tmp"
-        _run(["g++", *COMMON, cli_src, "-o", tmp, "-L", HERE, "-lrtnn_hip",
+        knn = os.environ.get("RTNN_KNN", "50")  # compile-time K of KNN mode
+        _run(["g++", *COMMON, f"-DRTNN_KNN={int(knn)}", cli_src, "-o", tmp, "-L", HERE,
+              "-lrtnn_hip",
               f"-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib"], verbose)
         os.replace(tmp, CLI)
     return LIB

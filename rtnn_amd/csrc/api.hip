@@ -76,17 +76,23 @@ struct rtnn_ctx_s {
   GridDesc hgrid{};
   uint64_t ncells = 0;
   int sort_passes = 0;
+  bool spts_current = false;  // spts/cell_start describe ctx->pts
 
   // queries / outputs
   DevBuf qin, qkeys, qvals, qsorted, oidx, od2, shortrows, misc, tiles, offs, bboxes;
   hipEvent_t ev[8] = {};
 
+  // radius proposals (learned / partitioned)
+  DevBuf pgrid, pcounts, qradius;
+  GridDesc hpgrid{};
+  DevBuf tr_q, tr_idx, tr_d2, tr_t, tr_f, tr_out;
   MlpState mlp;
 
   ~rtnn_ctx_s() {
     DevBuf* all[] = {&pts, &keys, &vals, &counts, &cell_start, &spts, &tmp, &grids,
                      &qin, &qkeys, &qvals, &qsorted, &oidx, &od2, &shortrows, &misc,
-                     &tiles, &offs, &bboxes};
+                     &tiles, &offs, &bboxes, &pgrid, &pcounts, &qradius, &tr_q, &tr_idx,
+                     &tr_d2, &tr_t, &tr_f, &tr_out};
     for (DevBuf* b : all) b->release();
     mlp_release(mlp);
     for (auto& e : ev)
@@ -188,22 +194,19 @@ int build_grid(rtnn_ctx ctx, float cell) {
   CK(hipMemcpyAsync(ctx->grids.p, &ctx->hgrid, sizeof(GridDesc), hipMemcpyHostToDevice, ctx->st));
   CK(ctx->keys.ensure(4 * n));
   CK(ctx->vals.ensure(4 * n));
-  CK(ctx->counts.ensure(4 * (ctx->ncells + 1)));
   CK(ctx->cell_start.ensure(4 * (ctx->ncells + 1)));
   CK(ctx->spts.ensure(16 * std::max<uint64_t>(n, 1)));
-  uint64_t tmpw = std::max<uint64_t>(radix_tmp_words(n), (uint64_t)scan_tmp_words(ctx->ncells));
-  CK(ctx->tmp.ensure(4 * tmpw));
-  CK(fill_u32(ctx->counts.as<uint32_t>(), ctx->ncells, 0, ctx->st));
+  CK(ctx->tmp.ensure(4 * radix_tmp_words(n)));
   CK(cell_keys(ctx->pts.as<float>(), n, ctx->grids.as<GridDesc>(), nullptr, 1,
-               ctx->keys.as<uint32_t>(), ctx->vals.as<uint32_t>(), ctx->counts.as<uint32_t>(),
-               ctx->st));
-  CK(exclusive_scan(ctx->counts.as<uint32_t>(), ctx->cell_start.as<uint32_t>(), ctx->ncells,
-                    ctx->tmp.as<uint32_t>(), ctx->st));
+               ctx->keys.as<uint32_t>(), ctx->vals.as<uint32_t>(), nullptr, ctx->st));
   CK(radix_sort_pairs(ctx->keys.as<uint32_t>(), ctx->vals.as<uint32_t>(), n,
                       bits_for(ctx->ncells - 1), ctx->tmp.as<uint32_t>(), ctx->st,
                       &ctx->sort_passes));
+  CK(cell_bounds(ctx->keys.as<uint32_t>(), n, (uint32_t)ctx->ncells,
+                 ctx->cell_start.as<uint32_t>(), ctx->st));
   CK(gather_points(ctx->pts.as<float>(), ctx->vals.as<uint32_t>(), n, ctx->spts.as<float4>(),
                    ctx->st));
+  ctx->spts_current = true;
   return RTNN_OK;
 }
 
@@ -305,6 +308,7 @@ SearchArgs base_args(rtnn_ctx ctx, const float4* qproc, uint64_t nq, float r, ui
   a.nq = (uint32_t)nq;
   a.ntiles = (uint32_t)((nq + kTileQ - 1) / kTileQ);
   a.r = r;
+  a.qradius_scale = 1.0f;
   a.k = k;
   a.out_idx = o.idx;
   a.out_d2 = o.d2;
@@ -403,6 +407,7 @@ int rtnn_set_points(rtnn_ctx ctx, const float* xyz, uint64_t n, int on_device) {
   CK(hipMemcpyAsync(ctx->pts.p, xyz, 12 * n,
                     on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, ctx->st));
   ctx->n = n;
+  ctx->spts_current = false;
   CK(ctx->misc.ensure(4096 + 6 * 4 * kBboxBlocks));
   float* bb = ctx->misc.as<float>();
   uint32_t* bad = reinterpret_cast<uint32_t*>(bb + 8);
@@ -479,11 +484,279 @@ int rtnn_search(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_device, int
                      stats);
 }
 
+}  // extern "C"
+
+namespace {
+
+// Point-count grid over ctx->pts with edge `cell` (origin = points' bbox min).
+// grow = false: the edge is kept (error if the grid would exceed the budget).
+int build_count_grid(rtnn_ctx ctx, float cell, bool grow) {
+  GridDesc g;
+  const uint64_t cap = max_cells_for(ctx, ctx->n);
+  for (int tries = 0;; ++tries) {
+    plan_grid(ctx->bbox, cell, g);
+    if (grid_cells(g) <= cap && grid_cells(g) < 0xfffffff0ull) break;
+    if (!grow || tries > 200)
+      return fail(ctx, RTNN_ERR_UNSUPPORTED, "count grid of edge %g needs too many cells", cell);
+    cell *= 1.25f;
+  }
+  g.npoints = (uint32_t)ctx->n;
+  ctx->hpgrid = g;
+  const uint64_t nc = grid_cells(g);
+  CK(ctx->pgrid.ensure(sizeof(GridDesc)));
+  CK(hipMemcpyAsync(ctx->pgrid.p, &ctx->hpgrid, sizeof(GridDesc), hipMemcpyHostToDevice,
+                    ctx->st));
+  CK(ctx->pcounts.ensure(4 * nc));
+  CK(fill_u32(ctx->pcounts.as<uint32_t>(), nc, 0, ctx->st));
+  if (ctx->spts_current)  // points grouped by cell: one atomic per run
+    CK(count_sorted(ctx->spts.as<float4>(), ctx->n, ctx->pgrid.as<GridDesc>(),
+                    ctx->pcounts.as<uint32_t>(), ctx->st));
+  else
+    CK(count_cells(ctx->pts.as<float>(), ctx->n, ctx->pgrid.as<GridDesc>(),
+                   ctx->pcounts.as<uint32_t>(), ctx->st));
+  return RTNN_OK;
+}
+
+enum class Proposal { kLearned, kPartition };
+
+// KNN with a proposed radius r' <= r per query, then an exact fallback: the
+// rows with fewer than k hits within r' are searched again at r.  If a row
+// has >= k hits with d2 < r'^2 <= r^2, any other point has d2 >= r'^2, so
+// those k are the k nearest within r.
+int proposal_search(rtnn_ctx ctx, Proposal kind, const float* qxyz, uint64_t nq, int on_device,
+                    float r, uint32_t k, int32_t* out_idx, float* out_d2, int out_on_device,
+                    rtnn_stats* stats) {
+  int rc = check_search_args(ctx, RTNN_MODE_KNN, r, k, out_idx);
+  if (rc) return rc;
+  if (ctx->n == 0) return fail(ctx, RTNN_ERR_STATE, "rtnn_set_points has not been called");
+  if (kind == Proposal::kLearned && !ctx->mlp.ready)
+    return fail(ctx, RTNN_ERR_STATE, "the radius MLP has no weights (train or load it)");
+  if (k > 64) return fail(ctx, RTNN_ERR_UNSUPPORTED, "proposal search supports k <= 64");
+  if (!qxyz) nq = ctx->n;
+  if (nq == 0) return RTNN_OK;
+  if (nq >= 0xffffffffull) return fail(ctx, RTNN_ERR_UNSUPPORTED, "more than 2^32-2 queries");
+  CK(hipSetDevice(ctx->device));
+  CK(hipEventRecord(ctx->ev[0], ctx->st));
+  const float f = ctx->opt.cell_factor > 0 ? ctx->opt.cell_factor
+                                           : (kind == Proposal::kLearned ? 2.0f : 4.0f);
+  rc = build_grid(ctx, r / f);
+  if (rc) return rc;
+  const float4* qproc = nullptr;
+  rc = prepare_queries(ctx, qxyz, nq, on_device, &qproc);
+  if (rc) return rc;
+  CK(hipEventRecord(ctx->ev[1], ctx->st));
+  CK(ctx->qradius.ensure(4 * nq));
+  if (kind == Proposal::kLearned) {
+    rc = build_count_grid(ctx, r, false);
+    if (rc) return rc;
+    CK(mlp_propose(ctx->mlp, qproc, (uint32_t)nq, ctx->pgrid.as<GridDesc>(),
+                   ctx->pcounts.as<uint32_t>(), r, ctx->qradius.as<float>(), ctx->st,
+                   ctx->num_cus));
+  } else {
+    rc = build_count_grid(ctx, r / 8.0f, true);
+    if (rc) return rc;
+    CK(partition_radius(qproc, (uint32_t)nq, ctx->pgrid.as<GridDesc>(),
+                        ctx->pcounts.as<uint32_t>(), k, r, ctx->qradius.as<float>(), nullptr, 0,
+                        ctx->st));
+  }
+  CK(hipEventRecord(ctx->ev[2], ctx->st));
+  OutBufs o;
+  rc = output_targets(ctx, nq, k, out_idx, out_d2, out_on_device, o);
+  if (rc) return rc;
+  // Two short-row lists (positions into qproc) and their counters.
+  CK(ctx->shortrows.ensure(4 * (2 * nq + 64)));
+  uint32_t* short1 = ctx->shortrows.as<uint32_t>();
+  uint32_t* short2 = short1 + nq;
+  uint32_t* counters = short1 + 2 * nq;
+  CK(fill_u32(counters, 2, 0, ctx->st));
+  const int gl = group_lanes_from_opt(ctx, RTNN_MODE_KNN, k);
+  if (gl < 0 || k > max_k_for_group(gl))
+    return fail(ctx, RTNN_ERR_UNSUPPORTED, "k=%u not supported by the selected kernel", k);
+  // Stage 1: every query at its proposed radius r'.
+  SearchArgs a = base_args(ctx, qproc, nq, r, k, o);
+  a.qradius = ctx->qradius.as<float>();
+  a.short_rows = short1;
+  a.short_count = &counters[0];
+  CK(launch_search(a, 1, gl, ctx->st, ctx->num_cus));
+  CK(hipEventRecord(ctx->ev[3], ctx->st));
+  // Stage 2: short rows at min(r, 2 r'); stage 3: what is still short, at r.
+  SearchArgs s2 = base_args(ctx, qproc, nq, r, k, o);
+  s2.qsel = short1;
+  s2.nq_dev = &counters[0];
+  s2.qradius = ctx->qradius.as<float>();
+  s2.qradius_scale = 2.0f;
+  s2.short_rows = short2;
+  s2.short_count = &counters[1];
+  CK(launch_search(s2, 1, gl, ctx->st, ctx->num_cus));
+  SearchArgs s3 = base_args(ctx, qproc, nq, r, k, o);
+  s3.qsel = short2;
+  s3.nq_dev = &counters[1];
+  CK(launch_search(s3, 1, gl, ctx->st, ctx->num_cus));
+  CK(hipEventRecord(ctx->ev[4], ctx->st));
+  rc = copy_out(ctx, nq, k, out_idx, out_d2, out_on_device, o);
+  if (rc) return rc;
+  uint32_t nshort = 0;
+  CK(hipMemcpyAsync(&nshort, &counters[0], 4, hipMemcpyDeviceToHost, ctx->st));
+  CK(hipEventRecord(ctx->ev[5], ctx->st));
+  CK(hipStreamSynchronize(ctx->st));
+  if (stats) {
+    std::memset(stats, 0, sizeof *stats);
+    stats->ms_grid = ev_ms(ctx->ev[0], ctx->ev[1]);
+    stats->ms_mlp = ev_ms(ctx->ev[1], ctx->ev[2]);
+    stats->ms_search = ev_ms(ctx->ev[2], ctx->ev[3]);
+    stats->ms_fallback = ev_ms(ctx->ev[3], ctx->ev[4]);
+    stats->ms_total = ev_ms(ctx->ev[0], ctx->ev[5]);
+    stats->fallback_queries = nshort;
+    stats->search_launches = 3;
+    stats->lanes_per_query = (uint32_t)gl;
+    fill_stats(ctx, stats);
+  }
+  return RTNN_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int rtnn_search_partitioned(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_device,
                             float r, uint32_t k, int32_t* out_idx, float* out_d2,
                             int out_on_device, rtnn_stats* stats) {
-  return search_impl(ctx, qxyz, nq, on_device, RTNN_MODE_KNN, r, k, out_idx, out_d2,
-                     out_on_device, stats);
+  return proposal_search(ctx, Proposal::kPartition, qxyz, nq, on_device, r, k, out_idx, out_d2,
+                         out_on_device, stats);
+}
+
+int rtnn_search_learned(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_device, float r,
+                        uint32_t k, int32_t* out_idx, float* out_d2, int out_on_device,
+                        rtnn_stats* stats) {
+  return proposal_search(ctx, Proposal::kLearned, qxyz, nq, on_device, r, k, out_idx, out_d2,
+                         out_on_device, stats);
+}
+
+// ------------------------------------------------------------------ MLP --
+
+int rtnn_mlp_train(rtnn_ctx ctx, float r, uint32_t k, uint32_t n_samples, uint32_t steps,
+                   uint32_t batch, float lr, float tau, uint64_t seed, float* final_loss) {
+  if (!ctx) return RTNN_ERR_INVALID;
+  if (ctx->n == 0) return fail(ctx, RTNN_ERR_STATE, "rtnn_set_points has not been called");
+  if (!(r > 0.0f) || k == 0 || k > 64 || n_samples == 0 || batch == 0 || batch > (1u << 20) ||
+      !(lr > 0.0f) || !(tau > 0.0f && tau < 1.0f))
+    return fail(ctx, RTNN_ERR_INVALID, "invalid training arguments");
+  CK(hipSetDevice(ctx->device));
+  const uint64_t ns = n_samples;
+  // 1. training queries drawn from the points, their exact k-th distances
+  CK(ctx->tr_q.ensure(12 * ns));
+  CK(mlp_sample_points(ctx->pts.as<float>(), ctx->n, ns, seed, ctx->tr_q.as<float>(), ctx->st));
+  CK(ctx->tr_idx.ensure(4 * ns * k));
+  CK(ctx->tr_d2.ensure(4 * ns * k));
+  rtnn_options saved = ctx->opt;
+  ctx->opt.query_order = 0;
+  int rc = search_impl(ctx, ctx->tr_q.as<float>(), ns, 1, RTNN_MODE_KNN, r, k,
+                       ctx->tr_idx.as<int32_t>(), ctx->tr_d2.as<float>(), 1, nullptr);
+  ctx->opt = saved;
+  if (rc) return rc;
+  CK(ctx->tr_t.ensure(4 * ns));
+  CK(mlp_targets(ctx->tr_d2.as<float>(), k, ns, r, ctx->tr_t.as<float>(), ctx->st));
+  // 2. features on the edge-r count grid
+  rc = build_count_grid(ctx, r, false);
+  if (rc) return rc;
+  CK(ctx->tr_f.ensure(4ull * kMlpFP * ns));
+  CK(mlp_features(ctx->tr_q.as<float>(), ns, ctx->pgrid.as<GridDesc>(),
+                  ctx->pcounts.as<uint32_t>(), ctx->mlp.count_scale, ctx->tr_f.as<float>(),
+                  ctx->st));
+  // 3. Adam on the pinball loss
+  const uint32_t nb = (batch + kGradBlock - 1) / kGradBlock;
+  const uint32_t eval_n = (uint32_t)std::min<uint64_t>(ns, 1u << 16);
+  const uint32_t eval_nb = (eval_n + kGradBlock - 1) / kGradBlock;
+  CK(mlp_alloc(ctx->mlp, std::max(nb, eval_nb)));
+  std::vector<float> init(kMlpNP);
+  mlp_init_host(init.data(), seed ^ 0x5eedull);
+  CK(hipMemcpyAsync(ctx->mlp.params, init.data(), 4 * kMlpNP, hipMemcpyHostToDevice, ctx->st));
+  CK(fill_u32(reinterpret_cast<uint32_t*>(ctx->mlp.adam_m), kMlpNP, 0, ctx->st));
+  CK(fill_u32(reinterpret_cast<uint32_t*>(ctx->mlp.adam_v), kMlpNP, 0, ctx->st));
+  for (uint32_t s = 1; s <= steps; ++s) {
+    uint64_t bseed = (seed * 0x9e3779b97f4a7c15ull + s) | 1ull;
+    CK(mlp_grad(ctx->mlp, ctx->tr_f.as<float>(), ctx->tr_t.as<float>(), ns, batch, bseed, tau,
+                1.0f / (float)batch, ctx->mlp.partial, ctx->st));
+    CK(mlp_adam(ctx->mlp, nb, lr, (int)s, ctx->st));
+  }
+  // 4. loss on the first eval_n samples
+  CK(ctx->tr_out.ensure(4 * (kMlpNP + 8)));
+  CK(mlp_grad(ctx->mlp, ctx->tr_f.as<float>(), ctx->tr_t.as<float>(), ns, eval_n, 0, tau,
+              1.0f / (float)eval_n, ctx->mlp.partial, ctx->st));
+  CK(mlp_reduce(ctx->mlp.partial, eval_nb, ctx->tr_out.as<float>(),
+                ctx->tr_out.as<float>() + kMlpNP, ctx->st));
+  float loss = 0;
+  CK(hipMemcpyAsync(&loss, ctx->tr_out.as<float>() + kMlpNP, 4, hipMemcpyDeviceToHost, ctx->st));
+  CK(hipStreamSynchronize(ctx->st));
+  if (final_loss) *final_loss = loss;
+  ctx->mlp.ready = true;
+  return RTNN_OK;
+}
+
+int rtnn_mlp_get_weights(rtnn_ctx ctx, float* blob) {
+  if (!ctx || !blob) return RTNN_ERR_INVALID;
+  if (!ctx->mlp.ready) return fail(ctx, RTNN_ERR_STATE, "the radius MLP has no weights");
+  CK(hipSetDevice(ctx->device));
+  CK(hipMemcpyAsync(blob, ctx->mlp.params, 4 * kMlpNP, hipMemcpyDeviceToHost, ctx->st));
+  CK(hipStreamSynchronize(ctx->st));
+  blob[kMlpNP] = ctx->mlp.count_scale;
+  blob[kMlpNP + 1] = ctx->mlp.margin;
+  return RTNN_OK;
+}
+
+int rtnn_mlp_set_weights(rtnn_ctx ctx, const float* blob) {
+  if (!ctx || !blob) return RTNN_ERR_INVALID;
+  for (int i = 0; i < kMlpNBlob; ++i)
+    if (!std::isfinite(blob[i])) return fail(ctx, RTNN_ERR_INVALID, "non-finite weight %d", i);
+  if (!(blob[kMlpNP] > 0.0f) || !(blob[kMlpNP + 1] > 0.0f))
+    return fail(ctx, RTNN_ERR_INVALID, "count_scale and radius_margin must be > 0");
+  CK(hipSetDevice(ctx->device));
+  CK(mlp_alloc(ctx->mlp, 1));
+  CK(hipMemcpyAsync(ctx->mlp.params, blob, 4 * kMlpNP, hipMemcpyHostToDevice, ctx->st));
+  CK(hipStreamSynchronize(ctx->st));
+  ctx->mlp.count_scale = blob[kMlpNP];
+  ctx->mlp.margin = blob[kMlpNP + 1];
+  ctx->mlp.ready = true;
+  return RTNN_OK;
+}
+
+int rtnn_mlp_forward(rtnn_ctx ctx, const float* feats, uint64_t n, float* out) {
+  if (!ctx || ((!feats || !out) && n)) return RTNN_ERR_INVALID;
+  if (!ctx->mlp.ready) return fail(ctx, RTNN_ERR_STATE, "the radius MLP has no weights");
+  CK(hipSetDevice(ctx->device));
+  CK(mlp_forward(ctx->mlp, feats, n, out, ctx->st, ctx->num_cus));
+  CK(hipStreamSynchronize(ctx->st));
+  return RTNN_OK;
+}
+
+int rtnn_mlp_backward(rtnn_ctx ctx, const float* feats, const float* target, uint64_t n,
+                      float tau, float* grads, float* loss) {
+  if (!ctx || !feats || !target || !grads || n == 0 || n > (1u << 24)) return RTNN_ERR_INVALID;
+  if (!ctx->mlp.ready) return fail(ctx, RTNN_ERR_STATE, "the radius MLP has no weights");
+  CK(hipSetDevice(ctx->device));
+  const uint32_t nb = (uint32_t)((n + kGradBlock - 1) / kGradBlock);
+  CK(mlp_alloc(ctx->mlp, nb));
+  CK(ctx->tr_out.ensure(4 * (kMlpNP + 8)));
+  CK(mlp_grad(ctx->mlp, feats, target, n, (uint32_t)n, 0, tau, 1.0f / (float)n,
+              ctx->mlp.partial, ctx->st));
+  CK(mlp_reduce(ctx->mlp.partial, nb, grads, ctx->tr_out.as<float>() + kMlpNP, ctx->st));
+  float l = 0;
+  CK(hipMemcpyAsync(&l, ctx->tr_out.as<float>() + kMlpNP, 4, hipMemcpyDeviceToHost, ctx->st));
+  CK(hipStreamSynchronize(ctx->st));
+  if (loss) *loss = l;
+  return RTNN_OK;
+}
+
+int rtnn_mlp_features(rtnn_ctx ctx, const float* qxyz, uint64_t nq, float r, float* feats) {
+  if (!ctx || ((!qxyz || !feats) && nq) || !(r > 0.0f)) return RTNN_ERR_INVALID;
+  if (ctx->n == 0) return fail(ctx, RTNN_ERR_STATE, "rtnn_set_points has not been called");
+  CK(hipSetDevice(ctx->device));
+  int rc = build_count_grid(ctx, r, false);
+  if (rc) return rc;
+  CK(mlp_features(qxyz, nq, ctx->pgrid.as<GridDesc>(), ctx->pcounts.as<uint32_t>(),
+                  ctx->mlp.count_scale, feats, ctx->st));
+  CK(hipStreamSynchronize(ctx->st));
+  return RTNN_OK;
 }
 
 // ------------------------------------------------------------- batched ----
@@ -550,19 +823,17 @@ int rtnn_search_batched(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets,
                     ctx->st));
   CK(ctx->keys.ensure(4 * n));
   CK(ctx->vals.ensure(4 * n));
-  CK(ctx->counts.ensure(4 * (total + 1)));
   CK(ctx->cell_start.ensure(4 * (total + 1)));
   CK(ctx->spts.ensure(16 * n));
-  CK(ctx->tmp.ensure(4 * std::max<uint64_t>(radix_tmp_words(n), (uint64_t)scan_tmp_words(total))));
-  CK(fill_u32(ctx->counts.as<uint32_t>(), total, 0, ctx->st));
+  CK(ctx->tmp.ensure(4 * radix_tmp_words(n)));
+  ctx->spts_current = false;  // spts now holds the clouds, not ctx->pts
   CK(cell_keys(xyz, n, ctx->grids.as<GridDesc>(), ctx->offs.as<uint64_t>(), nclouds,
-               ctx->keys.as<uint32_t>(), ctx->vals.as<uint32_t>(), ctx->counts.as<uint32_t>(),
-               ctx->st));
-  CK(exclusive_scan(ctx->counts.as<uint32_t>(), ctx->cell_start.as<uint32_t>(), total,
-                    ctx->tmp.as<uint32_t>(), ctx->st));
+               ctx->keys.as<uint32_t>(), ctx->vals.as<uint32_t>(), nullptr, ctx->st));
   CK(radix_sort_pairs(ctx->keys.as<uint32_t>(), ctx->vals.as<uint32_t>(), n,
                       bits_for(total - 1), ctx->tmp.as<uint32_t>(), ctx->st,
                       &ctx->sort_passes));
+  CK(cell_bounds(ctx->keys.as<uint32_t>(), n, (uint32_t)total, ctx->cell_start.as<uint32_t>(),
+                 ctx->st));
   CK(gather_points(xyz, ctx->vals.as<uint32_t>(), n, ctx->spts.as<float4>(), ctx->st));
 
   // Tiles: runs of <= kTileQ sorted queries inside one cloud.  Cell keys are

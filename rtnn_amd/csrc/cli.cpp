@@ -1,0 +1,334 @@
+// rtnn -- command-line driver with the optixNSearch flag surface.
+//
+// Flags, defaults and their meaning follow the reference driver
+// (src/optixNSearch/util.cpp:137-383, state.h:70-96, main.cpp:47-161):
+//   -f/--pfile, -q/--qfile (queries default to the points), -sm/--searchmode
+//   radius|knn, -r/--radius (default 2), -k/--knn (default 50; ignored in KNN
+//   mode, where K is fixed at build time: -DRTNN_KNN=<K>, the reference's
+//   cmake -DKNN), -d/--device, -c/--check, -p/--partition, and the advanced
+//   flags, which are accepted (see below).  The reference's flag bugs are not
+//   reproduced: -a takes 0/1/2, and --crStep has no trailing space.
+// Added: -o <file> writes the result (format below), --learned 0|1 selects the
+// learned-radius KNN, --mlp-in/--mlp-out load/save its weights, --cell-factor
+// and --qpw set the grid cell factor and queries per wave.
+//
+// Output file (little endian): "RTNNRES1", u64 nq, u32 k, u32 flags (bit 0:
+// d2 present), then int32 idx[nq][k] (empty slot = -1), then float d2[nq][k].
+//
+// All GPU work goes through the C ABI of librtnn_hip.so (include/rtnn_hip.h).
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <queue>
+#include <string>
+#include <vector>
+
+#include "rtnn_data.h"
+#include "rtnn_hip.h"
+
+#ifndef RTNN_KNN
+#define RTNN_KNN 50
+#endif
+
+namespace {
+
+struct Args {
+  std::string pfile, qfile, out, mlp_in, mlp_out, mode = "radius";
+  float radius = 2.0f;
+  unsigned knn = 50;
+  int device = 0;
+  bool check = false, partition = true, learned = false;
+  int approx = 2;
+  float cell_factor = 0.0f;
+  int qpw = 0;
+};
+
+struct Timer {
+  std::chrono::high_resolution_clock::time_point t0 = std::chrono::high_resolution_clock::now();
+  // Same line format as the reference's Timing::stopTiming (sutil/Timing.h:69).
+  double stop(const char* name) {
+    double ms = std::chrono::duration<double, std::milli>(
+                    std::chrono::high_resolution_clock::now() - t0).count();
+    std::printf("time %s: %g ms\n\n", name, ms);
+    std::fflush(stdout);
+    return ms;
+  }
+};
+
+[[noreturn]] void usage(const char* argv0, int code) {
+  std::fprintf(stderr,
+               "Usage: %s [options]\n\n"
+               "Basic Options:\n"
+               "  --pfile      | -f   File for search points (also the queries unless -q).\n"
+               "  --qfile      | -q   File for queries.\n"
+               "  --searchmode | -sm  \"knn\" or \"radius\" (default radius).\n"
+               "  --radius     | -r   Search radius (default 2).\n"
+               "  --knn        | -k   Max K returned in radius mode (default 50); KNN mode uses "
+               "the build-time K (%d).\n"
+               "  --device     | -d   GPU id (default 0).\n"
+               "  --check      | -c   Sanity check on the CPU (default 0).\n"
+               "  --help       | -h   This message.\n\n"
+               "Advanced Options (accepted for compatibility):\n"
+               "  --partition | -p   Density partition for KNN (exact; default 1).\n"
+               "  --approx | -a, --autobatch | -ab, --numbatch | -nb, --gassort | -s,\n"
+               "  --gsrRatio | -sg, --gather | -g, --pointsort | -ps, --querysort | -qs,\n"
+               "  --autocrratio | -ac, --crratio | -cr, --gpumemused | -gmu, --crStep | -crs,\n"
+               "  --metacellScale | -mc, --estgassize | -egs, --interleave | -i, --msr | -m,\n"
+               "  --deferFree | -df, --filterQueries | -fq\n\n"
+               "Added Options:\n"
+               "  -o <file>          Write results (RTNNRES1 binary).\n"
+               "  --learned 0|1      Learned-radius KNN (trains the MLP unless --mlp-in).\n"
+               "  --mlp-in <file>    Load MLP weights;  --mlp-out <file>  save them.\n"
+               "  --cell-factor <f>  Grid cell edge = r / f;  --qpw <n>  queries per wave.\n",
+               argv0, RTNN_KNN);
+  std::exit(code);
+}
+
+Args parse(int argc, char** argv) {
+  Args a;
+  auto need = [&](int& i) -> const char* {
+    if (i >= argc - 1) usage(argv[0], 1);
+    return argv[++i];
+  };
+  for (int i = 1; i < argc; ++i) {
+    std::string s = argv[i];
+    if (s == "--help" || s == "-h") usage(argv[0], 0);
+    else if (s == "--pfile" || s == "-f") a.pfile = need(i);
+    else if (s == "--qfile" || s == "-q") a.qfile = need(i);
+    else if (s == "--knn" || s == "-k") a.knn = (unsigned)std::atoi(need(i));
+    else if (s == "--searchmode" || s == "-sm") {
+      a.mode = need(i);
+      if (a.mode != "knn" && a.mode != "radius") usage(argv[0], 1);
+    } else if (s == "--radius" || s == "-r") a.radius = std::stof(need(i));
+    else if (s == "--device" || s == "-d") a.device = std::atoi(need(i));
+    else if (s == "--check" || s == "-c") a.check = std::atoi(need(i)) != 0;
+    else if (s == "--partition" || s == "-p") a.partition = std::atoi(need(i)) != 0;
+    else if (s == "--approx" || s == "-a") a.approx = std::atoi(need(i));
+    else if (s == "--gassort" || s == "-s") {
+      int v = std::atoi(need(i));
+      if (v < 0 || v > 2) usage(argv[0], 1);
+    } else if (s == "--gsrRatio" || s == "-sg") {
+      if (!(std::stof(need(i)) > 0)) usage(argv[0], 1);
+    } else if (s == "--estgassize" || s == "-egs") {
+      if (std::stof(need(i)) < 0) usage(argv[0], 1);
+    } else if (s == "--crStep" || s == "--crStep " || s == "-crs") {
+      if (!(std::stof(need(i)) > 1)) {
+        std::fprintf(stderr, "crStep has to be greater than 1.\n");
+        usage(argv[0], 1);
+      }
+    } else if (s == "--msr" || s == "-m" || s == "--deferFree" || s == "-df" ||
+               s == "--filterQueries" || s == "-fq" || s == "--numbatch" || s == "-nb" ||
+               s == "--autobatch" || s == "-ab" || s == "--autocrratio" || s == "-ac" ||
+               s == "--interleave" || s == "-i" || s == "--pointsort" || s == "-ps" ||
+               s == "--querysort" || s == "-qs" || s == "--crratio" || s == "-cr" ||
+               s == "--gpumemused" || s == "-gmu" || s == "--gather" || s == "-g" ||
+               s == "--metacellScale" || s == "-mc") {
+      need(i);  // scheduling knobs of the BVH pipeline: no effect on results here
+    } else if (s == "-o" || s == "--out") a.out = need(i);
+    else if (s == "--learned") a.learned = std::atoi(need(i)) != 0;
+    else if (s == "--mlp-in") a.mlp_in = need(i);
+    else if (s == "--mlp-out") a.mlp_out = need(i);
+    else if (s == "--cell-factor") a.cell_factor = std::stof(need(i));
+    else if (s == "--qpw") a.qpw = std::atoi(need(i));
+    else {
+      std::fprintf(stderr, "Unknown option '%s'\n", argv[i]);
+      usage(argv[0], 1);
+    }
+  }
+  if (a.mode == "knn") a.knn = RTNN_KNN;  // util.cpp:368-369
+  if (a.pfile.empty()) usage(argv[0], 1);
+  return a;
+}
+
+std::vector<float> read_file(const std::string& path) {
+  uint64_t n = 0;
+  int rc = rtnn_read_points(path.c_str(), nullptr, &n);
+  if (rc == -2) {
+    std::fprintf(stderr, "Could not read the frame data %s\n", path.c_str());
+    std::exit(1);
+  }
+  std::vector<float> v(3 * n);
+  rc = rtnn_read_points(path.c_str(), v.data(), &n);
+  if (rc == -1) {
+    std::fprintf(stderr, "%s: line %llu does not hold three numbers\n", path.c_str(),
+                 (unsigned long long)n);
+    std::exit(1);
+  }
+  return v;
+}
+
+float d2_of(const float* q, const float* p) {
+  volatile float dx = q[0] - p[0], dy = q[1] - p[1], dz = q[2] - p[2];
+  volatile float sx = dx * dx, sy = dy * dy, sz = dz * dz;
+  volatile float s = sx + sy;
+  return s + sz;
+}
+
+// Sanity check with the reference's rules (check.cpp:20-139): KNN compares 5
+// sampled queries against a brute force; radius checks every returned index is
+// within r and reports the average neighbour count.  This build also checks
+// the KNN indices and their (d2, index) order, and radius completeness for the
+// sampled queries.
+bool sanity_check(const Args& a, const std::vector<float>& P, const std::vector<float>& Q,
+                  const std::vector<int32_t>& idx, unsigned k) {
+  const uint64_t np = P.size() / 3, nq = Q.size() / 3;
+  const float r2 = a.radius * a.radius;
+  const bool knn = a.mode == "knn";
+  bool ok = true;
+  if (!knn) {
+    uint64_t total = 0;
+    for (uint64_t q = 0; q < nq && ok; ++q)
+      for (unsigned n = 0; n < k; ++n) {
+        int32_t p = idx[q * k + n];
+        if (p < 0) break;
+        ++total;
+        if (!(d2_of(&Q[3 * q], &P[3 * (uint64_t)p]) < r2)) {
+          std::printf("Point %d is not a neighbor of query %llu.\n", p, (unsigned long long)q);
+          ok = false;
+          break;
+        }
+      }
+    std::fprintf(stderr, "Avg neighbor/query: %g\n", (double)total / (double)nq);
+  }
+  uint64_t seed = 0x1234567ull;
+  for (int t = 0; t < 5 && ok; ++t) {
+    seed = seed * 6364136223846793005ull + 1442695040888963407ull;
+    const uint64_t q = (seed >> 17) % nq;
+    std::priority_queue<std::pair<float, uint32_t>> heap;
+    for (uint64_t p = 0; p < np; ++p) {
+      float d2 = d2_of(&Q[3 * q], &P[3 * p]);
+      bool keep = knn ? (d2 > 0 && d2 < r2) : (d2 < r2);
+      if (!keep) continue;
+      std::pair<float, uint32_t> e(d2, (uint32_t)p);
+      if (heap.size() < k) heap.push(e);
+      else if (e < heap.top()) { heap.pop(); heap.push(e); }
+    }
+    std::vector<int32_t> want(k, -1);
+    for (int i = (int)heap.size() - 1; i >= 0; --i) { want[i] = (int32_t)heap.top().second; heap.pop(); }
+    for (unsigned n = 0; n < k; ++n)
+      if (want[n] != idx[q * k + n]) {
+        std::printf("Incorrect query [%llu] %f, %f, %f\n", (unsigned long long)q, Q[3 * q],
+                    Q[3 * q + 1], Q[3 * q + 2]);
+        ok = false;
+        break;
+      }
+  }
+  if (ok) std::fprintf(stderr, "Sanity check done.\n");
+  return ok;
+}
+
+int check_rc(rtnn_ctx ctx, int rc, const char* what) {
+  if (rc != RTNN_OK) {
+    std::fprintf(stderr, "Caught exception: %s failed (%d): %s\n", what, rc,
+                 ctx ? rtnn_last_error(ctx) : rtnn_create_error());
+    std::exit(1);
+  }
+  return rc;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  Args a = parse(argc, argv);
+  std::vector<float> P = read_file(a.pfile);
+  const bool samepq = a.qfile.empty() || a.qfile == a.pfile;
+  std::vector<float> Q = samepq ? P : read_file(a.qfile);
+  const uint64_t np = P.size() / 3, nq = Q.size() / 3;
+  if (np == 0 || nq == 0) {  // util.cpp:401-404
+    std::printf("empty query and/or points\n");
+    return 0;
+  }
+  const bool knn = a.mode == "knn";
+  const unsigned k = a.knn;
+  std::printf("========================================\n");
+  std::printf("numPoints: %llu\nnumQueries: %llu\nsearchMode: %s\nradius: %g\nK: %u\n",
+              (unsigned long long)np, (unsigned long long)nq, a.mode.c_str(), a.radius, k);
+  std::printf("Same P and Q? %s\nQuery partition? %s\nLearned radius? %s\n",
+              samepq ? "true" : "false", a.partition ? "true" : "false",
+              a.learned ? "true" : "false");
+  std::printf("Device: %s\n========================================\n\n", rtnn_version());
+
+  rtnn_ctx ctx = nullptr;
+  check_rc(nullptr, rtnn_create(a.device, &ctx), "rtnn_create");
+  rtnn_options opt{};
+  opt.cell_factor = a.cell_factor;
+  opt.queries_per_wave = a.qpw;
+  check_rc(ctx, rtnn_set_options(ctx, &opt), "rtnn_set_options");
+  {
+    Timer t;
+    check_rc(ctx, rtnn_set_points(ctx, P.data(), np, 0), "rtnn_set_points");
+    t.stop("upload points and/or queries");
+  }
+  if (knn && a.learned) {
+    if (!a.mlp_in.empty()) {
+      std::vector<float> blob(RTNN_MLP_NBLOB);
+      FILE* f = std::fopen(a.mlp_in.c_str(), "rb");
+      if (!f || std::fread(blob.data(), 4, blob.size(), f) != blob.size()) {
+        std::fprintf(stderr, "cannot read MLP weights %s\n", a.mlp_in.c_str());
+        return 1;
+      }
+      std::fclose(f);
+      check_rc(ctx, rtnn_mlp_set_weights(ctx, blob.data()), "rtnn_mlp_set_weights");
+    } else {
+      Timer t;
+      float loss = 0;
+      check_rc(ctx, rtnn_mlp_train(ctx, a.radius, k, 1u << 21, 2000, 4096, 1e-3f, 0.9f, 1, &loss),
+               "rtnn_mlp_train");
+      t.stop("train radius MLP");
+      std::printf("MLP pinball loss: %g\n", loss);
+    }
+    if (!a.mlp_out.empty()) {
+      std::vector<float> blob(RTNN_MLP_NBLOB);
+      check_rc(ctx, rtnn_mlp_get_weights(ctx, blob.data()), "rtnn_mlp_get_weights");
+      FILE* f = std::fopen(a.mlp_out.c_str(), "wb");
+      if (!f) return 1;
+      std::fwrite(blob.data(), 4, blob.size(), f);
+      std::fclose(f);
+    }
+  }
+  std::vector<int32_t> idx(nq * (uint64_t)k);
+  std::vector<float> d2(a.out.empty() ? 0 : nq * (uint64_t)k);
+  float* d2p = d2.empty() ? nullptr : d2.data();
+  const float* qp = samepq ? nullptr : Q.data();
+  rtnn_stats st{};
+  {
+    Timer t;
+    int rc;
+    if (knn && a.learned)
+      rc = rtnn_search_learned(ctx, qp, nq, 0, a.radius, k, idx.data(), d2p, 0, &st);
+    else if (knn && a.partition)
+      rc = rtnn_search_partitioned(ctx, qp, nq, 0, a.radius, k, idx.data(), d2p, 0, &st);
+    else
+      rc = rtnn_search(ctx, qp, nq, 0, knn ? RTNN_MODE_KNN : RTNN_MODE_RADIUS, a.radius, k,
+                       idx.data(), d2p, 0, &st);
+    check_rc(ctx, rc, "search");
+    t.stop("total search time");
+  }
+  std::printf("time grid build: %g ms\n\ntime search compute: %g ms\n\n", st.ms_grid,
+              st.ms_search + st.ms_fallback);
+  if (knn && (a.learned || a.partition))
+    std::printf("fallback queries: %llu (%.3f%%)\n", (unsigned long long)st.fallback_queries,
+                100.0 * (double)st.fallback_queries / (double)nq);
+  if (!a.out.empty()) {
+    FILE* f = std::fopen(a.out.c_str(), "wb");
+    if (!f) {
+      std::fprintf(stderr, "cannot write %s\n", a.out.c_str());
+      return 1;
+    }
+    uint64_t hq = nq;
+    uint32_t hk = k, flags = 1;
+    std::fwrite("RTNNRES1", 1, 8, f);
+    std::fwrite(&hq, 8, 1, f);
+    std::fwrite(&hk, 4, 1, f);
+    std::fwrite(&flags, 4, 1, f);
+    std::fwrite(idx.data(), 4, idx.size(), f);
+    std::fwrite(d2.data(), 4, d2.size(), f);
+    std::fclose(f);
+  }
+  int code = 0;
+  if (a.check && !sanity_check(a, P, Q, idx, k)) code = 1;
+  rtnn_destroy(ctx);
+  return code;
+}

@@ -43,15 +43,18 @@ struct SearchArgs {
   const uint32_t* cell_start;  // global cell table, cells + 1 entries
   const GridDesc* grids;
   const float4* queries;     // processing order: (x, y, z, bits(output row))
+  const uint32_t* qsel;      // nullable: position i of this launch reads queries[qsel[i]]
+  const uint32_t* nq_dev;    // nullable: query count read on the device (persistent)
   const Tile* tiles;         // nullable: implicit tiles of kTileQ queries, grid 0
   uint32_t ntiles;
   uint32_t nq;
   float r;                   // search radius (per-query radius overrides when set)
   uint32_t k;                // output columns
-  const float* qradius;      // nullable; indexed by output row
+  const float* qradius;      // nullable; indexed by processing position (queries[])
+  float qradius_scale;       // per-query radius = min(r, qradius[.] * qradius_scale)
   int32_t* out_idx;          // [rows x k]
   float* out_d2;             // nullable
-  uint32_t* short_rows;      // nullable: rows with < k hits are appended here
+  uint32_t* short_rows;      // nullable: positions (in queries[]) with < k hits
   uint32_t* short_count;     // counter for short_rows
   int local_index;           // batched: report cloud-local point indices
   int persistent;            // grid-stride over tiles (XCD-chunked)

@@ -157,7 +157,7 @@ __global__ __launch_bounds__(256) void k_cell_keys(const float* __restrict__ xyz
                                    (uint32_t)g.dim[0] + (uint32_t)cx;
   keys[i] = key;
   vals[i] = (uint32_t)i;
-  atomicAdd(&counts[key], 1u);
+  if (counts) atomicAdd(&counts[key], 1u);
 }
 
 hipError_t cell_keys(const float* xyz, uint64_t n, const GridDesc* grids,
@@ -166,6 +166,94 @@ hipError_t cell_keys(const float* xyz, uint64_t n, const GridDesc* grids,
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_cell_keys, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st, xyz, n,
                      grids, offsets, nclouds, keys, vals, counts);
+  return hipGetLastError();
+}
+
+// cell_start[c] = first sorted position whose key is >= c, for c in
+// [0, ncells]; thread j (0 <= j <= n) owns the cells (key[j-1], key[j]] (with
+// key[-1] = -1 and key[n] = ncells).  Every cell is written exactly once, so
+// no counts, atomics, fill or scan are needed.  Long gaps (empty space) are
+// spread over the wave: the wave scans its gap lengths and its 64 lanes write
+// the concatenated gaps in coalesced strides.
+__global__ __launch_bounds__(256) void k_cell_bounds(const uint32_t* __restrict__ keys,
+                                                     uint64_t n, uint32_t ncells,
+                                                     uint32_t* __restrict__ cs) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t j = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  uint32_t lo = 0, len = 0;
+  if (j <= n) {
+    lo = j == 0 ? 0u : keys[j - 1] + 1u;
+    const uint32_t hi = j == n ? ncells : keys[j];
+    len = hi + 1u > lo ? hi + 1u - lo : 0u;
+  }
+  uint32_t incl = len;
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    uint32_t t = __shfl_up(incl, o, kWave);
+    if (lane >= o) incl += t;
+  }
+  const uint32_t excl = incl - len;
+  const uint32_t total = __shfl(incl, kWave - 1, kWave);
+  const uint64_t j0 = j - (uint64_t)lane;
+  for (uint32_t base = 0; base < total; base += kWave) {
+    const uint32_t t = base + (uint32_t)lane;
+    // owner = the last lane whose exclusive offset is <= t
+    int owner = 0;
+#pragma unroll
+    for (int step = kWave / 2; step > 0; step >>= 1) {
+      const uint32_t e = __shfl(excl, owner + step, kWave);
+      if (e <= t) owner += step;
+    }
+    const uint32_t olo = __shfl(lo, owner, kWave);
+    const uint32_t oex = __shfl(excl, owner, kWave);
+    if (t < total) cs[olo + (t - oex)] = (uint32_t)(j0 + (uint64_t)owner);
+  }
+}
+
+hipError_t cell_bounds(const uint32_t* sorted_keys, uint64_t n, uint32_t ncells,
+                       uint32_t* cell_start, hipStream_t st) {
+  hipLaunchKernelGGL(k_cell_bounds, dim3((uint32_t)((n + 1 + 255) / 256)), dim3(256), 0, st,
+                     sorted_keys, n, ncells, cell_start);
+  return hipGetLastError();
+}
+
+// counts[cell of spts[j]] over the sorted points; consecutive points share a
+// cell, so each wave adds one count per run of equal cells (one atomic per
+// run instead of per point).
+__global__ __launch_bounds__(256) void k_count_sorted(const float4* __restrict__ spts,
+                                                      uint64_t n,
+                                                      const GridDesc* __restrict__ grid,
+                                                      uint32_t* __restrict__ counts) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t j = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  const GridDesc g = *grid;
+  uint32_t key = 0xffffffffu;
+  if (j < n) {
+    const float4 p = spts[j];
+    const int cx = cell_of(p.x, g.mn[0], g.inv, g.dim[0]);
+    const int cy = cell_of(p.y, g.mn[1], g.inv, g.dim[1]);
+    const int cz = cell_of(p.z, g.mn[2], g.inv, g.dim[2]);
+    key = ((uint32_t)cz * (uint32_t)g.dim[1] + (uint32_t)cy) * (uint32_t)g.dim[0] + (uint32_t)cx;
+  }
+  const uint32_t prev = __shfl_up(key, 1, kWave);
+  const bool head = j < n && (lane == 0 || prev != key);
+  const uint64_t heads = __ballot(head);
+  if (head) {
+    // run ends at the next head (or at the end of the wave / of the data)
+    const uint64_t after = lane == kWave - 1 ? 0ull : (heads >> (lane + 1)) << (lane + 1);
+    const int next = after ? __ffsll((long long)after) - 1 : kWave;
+    const uint64_t j0 = j - (uint64_t)lane;
+    const uint64_t valid = min((uint64_t)kWave, n - j0);  // lanes holding points
+    const uint32_t len = (uint32_t)(min((uint64_t)next, valid) - (uint64_t)lane);
+    atomicAdd(&counts[key], len);
+  }
+}
+
+hipError_t count_sorted(const float4* spts, uint64_t n, const GridDesc* g, uint32_t* counts,
+                        hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_count_sorted, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st, spts,
+                     n, g, counts);
   return hipGetLastError();
 }
 

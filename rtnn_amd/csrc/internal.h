@@ -26,10 +26,16 @@ hipError_t bbox(const float* xyz, uint64_t n, float* part, float* out, uint32_t*
 // Per-cloud bbox: out[6 * c .. 6 * c + 6) for cloud c (offsets: nclouds + 1).
 hipError_t bbox_segmented(const float* xyz, const uint64_t* offsets, uint32_t nclouds,
                           float* out, uint32_t* nonfinite, hipStream_t st);
-// keys[i] = grid cell of point i, vals[i] = i, counts[key] += 1.
+// keys[i] = grid cell of point i, vals[i] = i, counts[key] += 1 (counts nullable).
 hipError_t cell_keys(const float* xyz, uint64_t n, const GridDesc* grids,
                      const uint64_t* offsets, uint32_t nclouds, uint32_t* keys,
                      uint32_t* vals, uint32_t* counts, hipStream_t st);
+// cell_start[0..ncells] from the sorted keys (first position with key >= c).
+hipError_t cell_bounds(const uint32_t* sorted_keys, uint64_t n, uint32_t ncells,
+                       uint32_t* cell_start, hipStream_t st);
+// counts[cell of spts[j]] += 1 on grid g, one atomic per run of equal cells.
+hipError_t count_sorted(const float4* spts, uint64_t n, const GridDesc* g, uint32_t* counts,
+                        hipStream_t st);
 // spts[j] = (xyz[vals[j]], bits(vals[j])).
 hipError_t gather_points(const float* xyz, const uint32_t* vals, uint64_t n, float4* spts,
                          hipStream_t st);
@@ -43,6 +49,14 @@ hipError_t iota_u32(uint32_t* p, uint64_t n, hipStream_t st);
 // Coarse count grid: counts[cell of point] += 1 (grid g, single cloud).
 hipError_t count_cells(const float* xyz, uint64_t n, const GridDesc* g, uint32_t* counts,
                        hipStream_t st);
+
+// ---- partition.hip
+// Launch radius per processing position from the point-count grid (edge
+// grid->cell): min(r, sqrt(3)*(i+1)*cell) for the smallest megacell i holding
+// >= k+1 points, else r.  iter_hist (nullable) counts queries per i.
+hipError_t partition_radius(const float4* qproc, uint32_t nq, const GridDesc* grid,
+                            const uint32_t* counts, uint32_t k, float r, float* qradius,
+                            uint32_t* iter_hist, int hist_bins, hipStream_t st);
 
 // ---- search.hip
 // Runs the top-k search.  group_lanes = 1 selects the thread-per-query kernel

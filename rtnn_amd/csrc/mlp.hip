@@ -1,28 +1,551 @@
-// Learned radius proposal (placeholder until the MLP kernels land).
+// Learned radius proposal for KNN on gfx950.
+//
+// Inference (k_mlp_propose, k_mlp_forward): MFMA v_mfma_f32_16x16x16_bf16.
+// The layers are computed transposed, H^T = W * X^T: the weights are the A
+// operand (held in VGPRs for the whole persistent loop) and the activations
+// of 16 queries are the B operand.  For this instruction lane l holds
+//   A[l % 16][4 * (l / 16) + j],  B[4 * (l / 16) + j][l % 16],
+//   D[4 * (l / 16) + j][l % 16]      (j = 0..3),
+// so a layer's output tile is already laid out as the next layer's B operand:
+// GELU and the BF16 rounding happen in registers, no LDS transpose.  The last
+// layer (64 -> 1) is a VALU dot product reduced over the four lane groups.
+//
+// Training (k_mlp_grad, k_mlp_adam): FP32 VALU, one sample per lane, the
+// same BF16 rounding of weights and activations in the forward pass as the
+// MFMA path (straight-through gradients to the FP32 master weights).
+// Per-block gradient partials are summed in a fixed order, so training is
+// deterministic.
+//
+// GELU is the tanh form of Hendrycks & Gimpel (2016),
+//   gelu(x) = 0.5 x (1 + tanh(sqrt(2/pi) (x + 0.044715 x^3))),
+// with tanh(u) = 1 - 2 / (1 + exp(2u)) evaluated through the hardware exp.
+#include <math.h>
+
 #include "../../include/rtnn_hip.h"
+#include "common.h"
+#include "internal.h"
 #include "mlp.h"
 
 namespace rtnn {
-void mlp_release(MlpState& m) {
-  if (m.dev) (void)hipFree(m.dev);
-  m.dev = nullptr;
-  m.bytes = 0;
-  m.ready = false;
-}
-}  // namespace rtnn
 
-extern "C" {
-int rtnn_mlp_train(rtnn_ctx, float, uint32_t, uint32_t, uint32_t, uint32_t, float, float,
-                   uint64_t, float*) { return RTNN_ERR_UNSUPPORTED; }
-int rtnn_mlp_get_weights(rtnn_ctx, float*) { return RTNN_ERR_UNSUPPORTED; }
-int rtnn_mlp_set_weights(rtnn_ctx, const float*) { return RTNN_ERR_UNSUPPORTED; }
-int rtnn_search_learned(rtnn_ctx, const float*, uint64_t, int, float, uint32_t, int32_t*,
-                        float*, int, rtnn_stats*) { return RTNN_ERR_UNSUPPORTED; }
-int rtnn_mlp_forward(rtnn_ctx, const float*, uint64_t, float*) { return RTNN_ERR_UNSUPPORTED; }
-int rtnn_mlp_backward(rtnn_ctx, const float*, const float*, uint64_t, float, float*, float*) {
-  return RTNN_ERR_UNSUPPORTED;
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// FP32 -> BF16 with round-to-nearest-even (finite inputs).
+__host__ __device__ inline uint16_t f2bf(float f) {
+  uint32_t u;
+#ifdef __HIP_DEVICE_COMPILE__
+  u = __float_as_uint(f);
+#else
+  __builtin_memcpy(&u, &f, 4);
+#endif
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
 }
-int rtnn_mlp_features(rtnn_ctx, const float*, uint64_t, float, float*) {
-  return RTNN_ERR_UNSUPPORTED;
+__device__ inline float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+__device__ inline float round_bf(float f) { return bf2f(f2bf(f)); }
+
+constexpr float kGeluC = 0.7978845608028654f;  // sqrt(2 / pi)
+constexpr float kGeluA = 0.044715f;
+
+__device__ inline float tanh_fast(float u) {
+  return 1.0f - 2.0f / (1.0f + __expf(2.0f * u));
 }
+__device__ inline float gelu(float x) {
+  float u = kGeluC * (x + kGeluA * x * x * x);
+  return 0.5f * x * (1.0f + tanh_fast(u));
 }
+__device__ inline float gelu_grad(float x) {
+  float u = kGeluC * (x + kGeluA * x * x * x);
+  float t = tanh_fast(u);
+  return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * kGeluC * (1.0f + 3.0f * kGeluA * x * x);
+}
+
+// ------------------------------------------------------------ features --
+
+// Feature f of a query: f < 27 -> log1p(count of coarse cell c + (dx,dy,dz)),
+// (dx,dy,dz) = (f%3-1, f/3%3-1, f/9-1), times count_scale; 27..29 -> offset of
+// the query inside its coarse cell; 30, 31 -> 0.
+struct QueryCell {
+  int cx, cy, cz;
+  float ox, oy, oz;
+};
+
+// Coarse cell (clamped to one cell outside the grid for far or NaN queries)
+// and the offset of the query inside it, clamped to [0, 1].
+__device__ inline void axis_cell(float p, float mn, float inv, int dim, int& c, float& off) {
+  float t = (p - mn) * inv;
+  float f = floorf(t);
+  f = fminf(fmaxf(f, -2.0f), (float)dim + 1.0f);
+  c = (int)f;
+  float o = t - f;
+  off = (o >= 0.0f && o <= 1.0f) ? o : (o > 1.0f ? 1.0f : 0.0f);
+}
+
+__device__ inline QueryCell query_cell(const GridDesc& g, float x, float y, float z) {
+  QueryCell qc;
+  axis_cell(x, g.mn[0], g.inv, g.dim[0], qc.cx, qc.ox);
+  axis_cell(y, g.mn[1], g.inv, g.dim[1], qc.cy, qc.oy);
+  axis_cell(z, g.mn[2], g.inv, g.dim[2], qc.cz, qc.oz);
+  return qc;
+}
+
+__device__ inline float feature(const GridDesc& g, const uint32_t* __restrict__ counts,
+                                const QueryCell& qc, int f, float count_scale) {
+  if (f < 27) {
+    int x = qc.cx + f % 3 - 1, y = qc.cy + (f / 3) % 3 - 1, z = qc.cz + f / 9 - 1;
+    if (x < 0 || y < 0 || z < 0 || x >= g.dim[0] || y >= g.dim[1] || z >= g.dim[2])
+      return 0.0f;
+    uint32_t c = counts[((uint32_t)z * (uint32_t)g.dim[1] + (uint32_t)y) * (uint32_t)g.dim[0] +
+                        (uint32_t)x];
+    return __logf(1.0f + (float)c) * count_scale;
+  }
+  if (f == 27) return qc.ox;
+  if (f == 28) return qc.oy;
+  if (f == 29) return qc.oz;
+  return 0.0f;
+}
+
+__global__ __launch_bounds__(256) void k_mlp_features(const float* __restrict__ q, uint64_t n,
+                                                      const GridDesc* __restrict__ coarse,
+                                                      const uint32_t* __restrict__ counts,
+                                                      float count_scale,
+                                                      float* __restrict__ feats) {
+  uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const GridDesc g = *coarse;
+  QueryCell qc = query_cell(g, q[3 * i], q[3 * i + 1], q[3 * i + 2]);
+  for (int f = 0; f < kMlpFP; ++f) feats[i * kMlpFP + f] = feature(g, counts, qc, f, count_scale);
+}
+
+// ------------------------------------------------------- MFMA inference --
+
+struct MfmaWeights {
+  bf16x4 a1[4][2];  // layer 1: M tile s (16 outputs), K step ks (16 inputs)
+  bf16x4 a2[4][4];  // layer 2
+  float b1[4][4];   // bias of output 16s + 4*(l/16) + j
+  float b2[4][4];
+  float w3[4][4];   // BF16-rounded output weights
+  float b3;
+};
+
+__device__ inline void load_weights(const float* __restrict__ P, MfmaWeights& w) {
+  const int l = threadIdx.x & 63;
+  const int row = l & 15, grp = l >> 4;
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+      bf16x4 v;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        int col = 16 * ks + 4 * grp + j;
+        float x = col < kMlpF ? P[kOW1 + (16 * s + row) * kMlpF + col] : 0.0f;
+        v[j] = (short)f2bf(x);
+      }
+      w.a1[s][ks] = v;
+    }
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+      bf16x4 v;
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        v[j] = (short)f2bf(P[kOW2 + (16 * s + row) * kMlpH + 16 * ks + 4 * grp + j]);
+      w.a2[s][ks] = v;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      int o = 16 * s + 4 * grp + j;
+      w.b1[s][j] = P[kOB1 + o];
+      w.b2[s][j] = P[kOB2 + o];
+      w.w3[s][j] = round_bf(P[kOW3 + o]);
+    }
+  }
+  w.b3 = P[kOB3];
+}
+
+// Forward of one 16-query tile given the B operands of layer 1 (x[ks] holds
+// features 16ks + 4*(l/16) + j of query l%16).  Returns o for query l%16 in
+// every lane of the column.
+__device__ inline float mlp_tile(const MfmaWeights& w, const bf16x4 (&x)[2]) {
+  f32x4 acc[4];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    acc[s] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks)
+      acc[s] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w.a1[s][ks], x[ks], acc[s], 0, 0, 0);
+  }
+  bf16x4 h1[4];
+#pragma unroll
+  for (int s = 0; s < 4; ++s)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) h1[s][j] = (short)f2bf(gelu(acc[s][j] + w.b1[s][j]));
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    acc[s] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks)
+      acc[s] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w.a2[s][ks], h1[ks], acc[s], 0, 0, 0);
+  }
+  float part = 0.0f;
+#pragma unroll
+  for (int s = 0; s < 4; ++s)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) part += w.w3[s][j] * round_bf(gelu(acc[s][j] + w.b2[s][j]));
+  part += __shfl_xor(part, 16, 64);
+  part += __shfl_xor(part, 32, 64);
+  return part + w.b3;
+}
+
+// Persistent: each wave handles tiles of 16 rows, grid-strided.
+__global__ __launch_bounds__(256) void k_mlp_forward(const float* __restrict__ P,
+                                                     const float* __restrict__ feats, uint64_t n,
+                                                     float* __restrict__ out) {
+  MfmaWeights w;
+  load_weights(P, w);
+  const int l = threadIdx.x & 63, row = l & 15, grp = l >> 4;
+  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64;
+  const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / 64;
+  const uint64_t ntiles = (n + 15) / 16;
+  for (uint64_t t = wave; t < ntiles; t += nwaves) {
+    const uint64_t qi = t * 16 + row;
+    bf16x4 x[2];
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        x[ks][j] = (short)(qi < n ? f2bf(feats[qi * kMlpFP + 16 * ks + 4 * grp + j]) : 0);
+    float o = mlp_tile(w, x);
+    if (grp == 0 && qi < n) out[qi] = o;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_mlp_propose(const float* __restrict__ P,
+                                                     const float4* __restrict__ qproc,
+                                                     uint32_t nq,
+                                                     const GridDesc* __restrict__ coarse,
+                                                     const uint32_t* __restrict__ counts,
+                                                     float count_scale, float margin, float r,
+                                                     float* __restrict__ qradius) {
+  MfmaWeights w;
+  load_weights(P, w);
+  const GridDesc g = *coarse;
+  const int l = threadIdx.x & 63, row = l & 15, grp = l >> 4;
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / 64;
+  const uint32_t nwaves = gridDim.x * blockDim.x / 64;
+  const uint32_t ntiles = (nq + 15) / 16;
+  for (uint32_t t = wave; t < ntiles; t += nwaves) {
+    const uint32_t qi = t * 16 + row;
+    bf16x4 x[2];
+    if (qi < nq) {
+      const float4 q = qproc[qi];
+      QueryCell qc = query_cell(g, q.x, q.y, q.z);
+#pragma unroll
+      for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          x[ks][j] = (short)f2bf(feature(g, counts, qc, 16 * ks + 4 * grp + j, count_scale));
+    } else {
+#pragma unroll
+      for (int ks = 0; ks < 2; ++ks) x[ks] = bf16x4{0, 0, 0, 0};
+    }
+    float o = mlp_tile(w, x);
+    if (grp == 0 && qi < nq) {
+      float frac = fminf(fmaxf(o * margin, kMinFrac), 1.0f);
+      qradius[qi] = fminf(r * frac, r);
+    }
+  }
+}
+
+// ------------------------------------------------------------ training --
+
+__device__ inline uint64_t mix64(uint64_t z) {
+  z += 0x9e3779b97f4a7c15ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+constexpr int kLd = 65;  // padded row stride (conflict-free column access)
+
+// One sample per lane (kGradBlock lanes).  partial[block] receives this
+// block's gradient sums (kMlpNP floats) and loss sum.
+__global__ __launch_bounds__(kGradBlock) void k_mlp_grad(const float* __restrict__ P,
+                                                         const float* __restrict__ feats,
+                                                         const float* __restrict__ targets,
+                                                         uint64_t n, uint32_t count,
+                                                         uint64_t seed, float tau, float norm,
+                                                         float* __restrict__ partial) {
+  __shared__ float sW1[kMlpH][kMlpF + 1];
+  __shared__ float sW2[kMlpH][kLd];
+  __shared__ float sB1[kMlpH], sB2[kMlpH], sW3[kMlpH];
+  __shared__ float sX[kGradBlock][kMlpF + 1];
+  __shared__ float sZ1[kGradBlock][kLd];
+  __shared__ float sH1[kGradBlock][kLd];  // bf16(gelu(z1)), the input of layer 2
+  __shared__ float sZ2[kGradBlock][kLd];
+  __shared__ float sD2[kGradBlock][kLd];
+  __shared__ float sD1[kGradBlock][kLd];
+  __shared__ float sG[kGradBlock];
+  __shared__ float sLoss[kGradBlock];
+  const int t = threadIdx.x;
+
+  for (int i = t; i < kMlpH * kMlpF; i += kGradBlock) sW1[i / kMlpF][i % kMlpF] = round_bf(P[kOW1 + i]);
+  for (int i = t; i < kMlpH * kMlpH; i += kGradBlock) sW2[i / kMlpH][i % kMlpH] = round_bf(P[kOW2 + i]);
+  sB1[t] = P[kOB1 + t];
+  sB2[t] = P[kOB2 + t];
+  sW3[t] = round_bf(P[kOW3 + t]);
+  const float b3 = P[kOB3];
+  __syncthreads();
+
+  const uint32_t si = blockIdx.x * kGradBlock + t;
+  const bool valid = si < count;
+  uint64_t row = si;
+  if (seed) row = mix64(seed ^ ((uint64_t)si * 0x100000001b3ull)) % n;
+  float x[kMlpF];
+#pragma unroll
+  for (int f = 0; f < kMlpF; ++f) {
+    x[f] = valid ? round_bf(feats[row * kMlpFP + f]) : 0.0f;
+    sX[t][f] = x[f];
+  }
+  // forward
+  for (int o = 0; o < kMlpH; ++o) {
+    float z = 0.0f;
+#pragma unroll
+    for (int f = 0; f < kMlpF; ++f) z += sW1[o][f] * x[f];
+    z += sB1[o];
+    sZ1[t][o] = z;
+    sH1[t][o] = round_bf(gelu(z));
+  }
+  for (int o = 0; o < kMlpH; ++o) {
+    float z = 0.0f;
+    for (int i = 0; i < kMlpH; ++i) z += sW2[o][i] * sH1[t][i];
+    sZ2[t][o] = z + sB2[o];
+  }
+  float out = 0.0f;
+  for (int o = 0; o < kMlpH; ++o) out += sW3[o] * round_bf(gelu(sZ2[t][o]));
+  out += b3;
+  const float tgt = valid ? targets[row] : 0.0f;
+  const float diff = tgt - out;
+  const float loss = valid ? (diff >= 0.0f ? tau * diff : (tau - 1.0f) * diff) : 0.0f;
+  const float g = valid ? norm * (out > tgt ? (1.0f - tau) : (out < tgt ? -tau : 0.0f)) : 0.0f;
+  sG[t] = g;
+  sLoss[t] = loss;
+  // backward through layer 3 and 2
+  for (int o = 0; o < kMlpH; ++o) sD2[t][o] = g * sW3[o] * gelu_grad(sZ2[t][o]);
+  for (int i = 0; i < kMlpH; ++i) {
+    float d = 0.0f;
+    for (int o = 0; o < kMlpH; ++o) d += sW2[o][i] * sD2[t][o];
+    sD1[t][i] = d * gelu_grad(sZ1[t][i]);
+  }
+  __syncthreads();
+
+  // weight gradients: lane t owns output unit t of every layer
+  float* dst = partial + (uint64_t)blockIdx.x * (kMlpNP + 1);
+  {
+    float gb2 = 0.0f, gw3 = 0.0f, gb1 = 0.0f;
+    for (int s = 0; s < kGradBlock; ++s) {
+      gb2 += sD2[s][t];
+      gb1 += sD1[s][t];
+      gw3 += sG[s] * round_bf(gelu(sZ2[s][t]));
+    }
+    dst[kOB2 + t] = gb2;
+    dst[kOB1 + t] = gb1;
+    dst[kOW3 + t] = gw3;
+    for (int i = 0; i < kMlpH; ++i) {
+      float acc = 0.0f;
+      for (int s = 0; s < kGradBlock; ++s) acc += sD2[s][t] * sH1[s][i];
+      dst[kOW2 + t * kMlpH + i] = acc;
+    }
+    for (int f = 0; f < kMlpF; ++f) {
+      float acc = 0.0f;
+      for (int s = 0; s < kGradBlock; ++s) acc += sD1[s][t] * sX[s][f];
+      dst[kOW1 + t * kMlpF + f] = acc;
+    }
+  }
+  if (t == 0) {
+    float gb3 = 0.0f, ls = 0.0f;
+    for (int s = 0; s < kGradBlock; ++s) {
+      gb3 += sG[s];
+      ls += sLoss[s];
+    }
+    dst[kOB3] = gb3;
+    dst[kMlpNP] = ls * norm;
+  }
+}
+
+// grads[p] = sum over blocks of partial[b][p] (fixed order); loss likewise.
+__global__ __launch_bounds__(256) void k_mlp_reduce(const float* __restrict__ partial,
+                                                    uint32_t nblocks, float* __restrict__ grads,
+                                                    float* __restrict__ loss) {
+  uint32_t p = blockIdx.x * 256 + threadIdx.x;
+  if (p > (uint32_t)kMlpNP) return;
+  float s = 0.0f;
+  for (uint32_t b = 0; b < nblocks; ++b) s += partial[(uint64_t)b * (kMlpNP + 1) + p];
+  if (p < (uint32_t)kMlpNP) grads[p] = s;
+  else if (loss) *loss = s;
+}
+
+__global__ __launch_bounds__(256) void k_mlp_adam(float* __restrict__ P, float* __restrict__ m,
+                                                  float* __restrict__ v,
+                                                  const float* __restrict__ partial,
+                                                  uint32_t nblocks, float lr, float bc1,
+                                                  float bc2) {
+  uint32_t p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= (uint32_t)kMlpNP) return;
+  float g = 0.0f;
+  for (uint32_t b = 0; b < nblocks; ++b) g += partial[(uint64_t)b * (kMlpNP + 1) + p];
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  float mm = b1 * m[p] + (1.0f - b1) * g;
+  float vv = b2 * v[p] + (1.0f - b2) * g * g;
+  m[p] = mm;
+  v[p] = vv;
+  P[p] -= lr * (mm / bc1) / (sqrtf(vv / bc2) + eps);
+}
+
+__global__ __launch_bounds__(256) void k_mlp_targets(const float* __restrict__ d2, uint32_t k,
+                                                     uint64_t n, float r,
+                                                     float* __restrict__ t) {
+  uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  float d = d2[i * k + (k - 1)];
+  t[i] = (d < __builtin_huge_valf()) ? sqrtf(d) / r : 1.0f;
+}
+
+__global__ __launch_bounds__(256) void k_sample_points(const float* __restrict__ pts,
+                                                       uint64_t npts, uint64_t n, uint64_t seed,
+                                                       float* __restrict__ out) {
+  uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  uint64_t j = mix64(seed * 0x9e3779b97f4a7c15ull + i) % npts;
+  out[3 * i] = pts[3 * j];
+  out[3 * i + 1] = pts[3 * j + 1];
+  out[3 * i + 2] = pts[3 * j + 2];
+}
+
+// ------------------------------------------------------------- host --
+
+hipError_t mlp_alloc(MlpState& m, uint32_t max_grad_blocks) {
+  hipError_t e;
+  if (!m.params) {
+    if ((e = hipMalloc(&m.params, 4 * kMlpNP)) != hipSuccess) return e;
+    if ((e = hipMalloc(&m.adam_m, 4 * kMlpNP)) != hipSuccess) return e;
+    if ((e = hipMalloc(&m.adam_v, 4 * kMlpNP)) != hipSuccess) return e;
+  }
+  if (m.partial_blocks < max_grad_blocks) {
+    if (m.partial) {
+      (void)hipDeviceSynchronize();
+      (void)hipFree(m.partial);
+      m.partial = nullptr;
+    }
+    if ((e = hipMalloc(&m.partial, 4ull * (kMlpNP + 1) * max_grad_blocks)) != hipSuccess) {
+      m.partial_blocks = 0;
+      return e;
+    }
+    m.partial_blocks = max_grad_blocks;
+  }
+  return hipSuccess;
+}
+
+void mlp_release(MlpState& m) {
+  float* ptrs[] = {m.params, m.adam_m, m.adam_v, m.partial};
+  for (float* p : ptrs)
+    if (p) (void)hipFree(p);
+  m = MlpState{};
+}
+
+void mlp_init_host(float* P, uint64_t seed) {
+  uint64_t s = seed;
+  auto u01 = [&s]() {
+    uint64_t z = (s += 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * 0x1.0p-53;
+  };
+  auto fill = [&](int off, int fan_out, int fan_in) {
+    double lim = std::sqrt(6.0 / (fan_in + fan_out));
+    for (int i = 0; i < fan_out * fan_in; ++i) P[off + i] = (float)((2 * u01() - 1) * lim);
+  };
+  for (int i = 0; i < kMlpNP; ++i) P[i] = 0.0f;
+  fill(kOW1, kMlpH, kMlpF);
+  fill(kOW2, kMlpH, kMlpH);
+  fill(kOW3, 1, kMlpH);
+  P[kOB3] = 0.3f;
+}
+
+static uint32_t blocks_for(uint64_t n) { return (uint32_t)((n + 255) / 256); }
+
+hipError_t mlp_features(const float* qxyz, uint64_t n, const GridDesc* coarse,
+                        const uint32_t* counts, float count_scale, float* feats,
+                        hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_mlp_features, dim3(blocks_for(n)), dim3(256), 0, st, qxyz, n, coarse,
+                     counts, count_scale, feats);
+  return hipGetLastError();
+}
+
+static dim3 persistent_grid(uint64_t tiles16, int num_cus) {
+  uint64_t want = (tiles16 + 3) / 4;  // 4 waves per block
+  uint64_t cap = (uint64_t)(num_cus > 0 ? num_cus : 256) * 8;
+  return dim3((uint32_t)(want < cap ? (want ? want : 1) : cap));
+}
+
+hipError_t mlp_forward(const MlpState& m, const float* feats, uint64_t n, float* out,
+                       hipStream_t st, int num_cus) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_mlp_forward, persistent_grid((n + 15) / 16, num_cus), dim3(256), 0, st,
+                     m.params, feats, n, out);
+  return hipGetLastError();
+}
+
+hipError_t mlp_propose(const MlpState& m, const float4* qproc, uint32_t nq,
+                       const GridDesc* coarse, const uint32_t* counts, float r, float* qradius,
+                       hipStream_t st, int num_cus) {
+  if (nq == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_mlp_propose, persistent_grid((nq + 15) / 16, num_cus), dim3(256), 0, st,
+                     m.params, qproc, nq, coarse, counts, m.count_scale, m.margin, r, qradius);
+  return hipGetLastError();
+}
+
+hipError_t mlp_grad(const MlpState& m, const float* feats, const float* targets, uint64_t n,
+                    uint32_t count, uint64_t rand_seed, float tau, float norm, float* partial,
+                    hipStream_t st) {
+  uint32_t nb = (count + kGradBlock - 1) / kGradBlock;
+  if (nb == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_mlp_grad, dim3(nb), dim3(kGradBlock), 0, st, m.params, feats, targets, n,
+                     count, rand_seed, tau, norm, partial);
+  return hipGetLastError();
+}
+
+hipError_t mlp_reduce(const float* partial, uint32_t nblocks, float* grads, float* loss,
+                      hipStream_t st) {
+  hipLaunchKernelGGL(k_mlp_reduce, dim3(blocks_for(kMlpNP + 1)), dim3(256), 0, st, partial,
+                     nblocks, grads, loss);
+  return hipGetLastError();
+}
+
+hipError_t mlp_adam(MlpState& m, uint32_t nblocks, float lr, int t, hipStream_t st) {
+  float bc1 = 1.0f - std::pow(0.9f, (float)t);
+  float bc2 = 1.0f - std::pow(0.999f, (float)t);
+  hipLaunchKernelGGL(k_mlp_adam, dim3(blocks_for(kMlpNP)), dim3(256), 0, st, m.params, m.adam_m,
+                     m.adam_v, m.partial, nblocks, lr, bc1, bc2);
+  return hipGetLastError();
+}
+
+hipError_t mlp_targets(const float* d2, uint32_t k, uint64_t n, float r, float* t,
+                       hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_mlp_targets, dim3(blocks_for(n)), dim3(256), 0, st, d2, k, n, r, t);
+  return hipGetLastError();
+}
+
+hipError_t mlp_sample_points(const float* pts, uint64_t npts, uint64_t n, uint64_t seed,
+                             float* out, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sample_points, dim3(blocks_for(n)), dim3(256), 0, st, pts, npts, n, seed,
+                     out);
+  return hipGetLastError();
+}
+
+}  // namespace rtnn

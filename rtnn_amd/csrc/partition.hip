@@ -1,0 +1,95 @@
+// Density-based launch radius per query: the reference's query partitioning
+// (src/optixNSearch/grid.cu:33-44,84-211 calcSearchSize; sort.cpp:346-391
+// genBatches; util.cpp:408-426) as one kernel per query instead of one thread
+// per occupied cell plus a histogram and per-batch BVHs.
+//
+// For the query's cell c on a count grid of edge `cell`, grow the cube of
+// (2i+1)^3 cells shell by shell until it holds >= k+1 points (the count may
+// include the query itself) or its width (2i+2)*cell exceeds 2r/sqrt(3) (the
+// cube inscribed in the radius-r sphere).  Every point of that cube lies within
+// sqrt(3)*(i+1)*cell of the query, so the k nearest others do too: the launch
+// radius is min(r, sqrt(3)*(i+1)*cell) (the reference's exact mode,
+// radiusFromMegacell with approxMode 0).  Queries that still see fewer than k
+// hits are re-run at r by the caller, so the result stays exact.
+#include "common.h"
+#include "internal.h"
+
+namespace rtnn {
+
+__device__ inline uint32_t cell_count(const GridDesc& g, const uint32_t* __restrict__ counts,
+                                      int x, int y, int z) {
+  if (x < 0 || y < 0 || z < 0 || x >= g.dim[0] || y >= g.dim[1] || z >= g.dim[2]) return 0;
+  return counts[((uint32_t)z * (uint32_t)g.dim[1] + (uint32_t)y) * (uint32_t)g.dim[0] +
+                (uint32_t)x];
+}
+
+// Point count of the cells at Chebyshev distance exactly i from (cx, cy, cz).
+__device__ inline uint64_t shell_count(const GridDesc& g, const uint32_t* __restrict__ counts,
+                                       int cx, int cy, int cz, int i) {
+  if (i == 0) return cell_count(g, counts, cx, cy, cz);
+  uint64_t s = 0;
+  for (int dz = -i; dz <= i; ++dz) {
+    const int z = cz + dz;
+    if (z < 0 || z >= g.dim[2]) continue;
+    for (int dy = -i; dy <= i; ++dy) {
+      const int y = cy + dy;
+      if (y < 0 || y >= g.dim[1]) continue;
+      const bool face = (dz == -i || dz == i || dy == -i || dy == i);
+      if (face) {
+        for (int dx = -i; dx <= i; ++dx) s += cell_count(g, counts, cx + dx, y, z);
+      } else {
+        s += cell_count(g, counts, cx - i, y, z) + cell_count(g, counts, cx + i, y, z);
+      }
+    }
+  }
+  return s;
+}
+
+__global__ __launch_bounds__(256) void k_partition_radius(const float4* __restrict__ qproc,
+                                                          uint32_t nq,
+                                                          const GridDesc* __restrict__ grid,
+                                                          const uint32_t* __restrict__ counts,
+                                                          uint32_t k, float r,
+                                                          float* __restrict__ qradius,
+                                                          uint32_t* __restrict__ iter_hist,
+                                                          int hist_bins) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= nq) return;
+  const GridDesc g = *grid;
+  const float4 q = qproc[i];
+  int c[3];
+  const float p[3] = {q.x, q.y, q.z};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    float f = floorf((p[a] - g.mn[a]) * g.inv);
+    f = fminf(fmaxf(f, -1.0f), (float)g.dim[a]);  // outside / NaN: border-adjacent
+    c[a] = (int)f;
+  }
+  const float max_width = 2.0f * r / 1.7320508f;
+  uint64_t count = 0;
+  int it = 0;
+  float R = r;
+  for (;;) {
+    count += shell_count(g, counts, c[0], c[1], c[2], it);
+    const float width = (float)(2 * it + 2) * g.cell;
+    if (width > max_width) break;  // megacell too wide: search at r
+    if (count >= (uint64_t)k + 1) {
+      R = fminf(r, 1.7320508f * (float)(it + 1) * g.cell * (1.0f + 1e-6f));
+      break;
+    }
+    ++it;
+  }
+  qradius[i] = R;
+  if (iter_hist) atomicAdd(&iter_hist[min(it, hist_bins - 1)], 1u);
+}
+
+hipError_t partition_radius(const float4* qproc, uint32_t nq, const GridDesc* grid,
+                            const uint32_t* counts, uint32_t k, float r, float* qradius,
+                            uint32_t* iter_hist, int hist_bins, hipStream_t st) {
+  if (nq == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_partition_radius, dim3((nq + 255) / 256), dim3(256), 0, st, qproc, nq,
+                     grid, counts, k, r, qradius, iter_hist, hist_bins);
+  return hipGetLastError();
+}
+
+}  // namespace rtnn

@@ -43,28 +43,34 @@ __device__ inline uint64_t lanemask_lt64() {
 }
 
 // Resolves tile t into its query range and grid.
-__device__ inline void tile_of(const SearchArgs& a, uint32_t t, uint32_t& q0, uint32_t& q1,
-                               uint32_t& gi) {
+__device__ inline void tile_of(const SearchArgs& a, uint32_t nq, uint32_t t, uint32_t& q0,
+                               uint32_t& q1, uint32_t& gi) {
   if (a.tiles) {
     Tile tl = a.tiles[t];
     q0 = tl.q0; q1 = tl.q1; gi = tl.grid;
   } else {
     q0 = t * (uint32_t)kTileQ;
-    q1 = min(q0 + (uint32_t)kTileQ, a.nq);
+    q1 = min(q0 + (uint32_t)kTileQ, nq);
     gi = 0;
   }
+}
+
+// Query count of this launch (device-resident for the fallback pass).
+__device__ inline uint32_t launch_nq(const SearchArgs& a) {
+  return a.nq_dev ? *a.nq_dev : a.nq;
 }
 
 // Persistent launches walk tiles in XCD-contiguous chunks; plain launches map
 // each block to one tile through xcd_swizzle.
 struct TileIter {
   uint32_t t, end, step;
-  __device__ TileIter(const SearchArgs& a) {
+  __device__ TileIter(const SearchArgs& a, uint32_t nq) {
+    uint32_t ntiles = a.nq_dev ? (nq + kTileQ - 1) / kTileQ : a.ntiles;
     if (a.persistent) {
       constexpr uint32_t kXcd = 8;
       uint32_t x = blockIdx.x % kXcd, j = blockIdx.x / kXcd;
       uint32_t per = (gridDim.x + kXcd - 1 - x) / kXcd;  // blocks on this XCD
-      uint32_t q = a.ntiles / kXcd, rem = a.ntiles % kXcd;
+      uint32_t q = ntiles / kXcd, rem = ntiles % kXcd;
       uint32_t start = x * q + (x < rem ? x : rem);
       end = start + q + (x < rem ? 1u : 0u);
       t = start + j;
@@ -93,43 +99,78 @@ __device__ inline void topk_insert(uint64_t (&list)[KC], uint64_t key) {
 
 template <int KC, bool KNN>
 __global__ __launch_bounds__(kTileQ) void k_topk_thread(const SearchArgs a) {
-  for (TileIter it(a); it.t < it.end; it.t += it.step) {
+  const uint32_t nq = launch_nq(a);
+  for (TileIter it(a, nq); it.t < it.end; it.t += it.step) {
     uint32_t q0, q1, gi;
-    tile_of(a, it.t, q0, q1, gi);
-    const uint32_t qs = q0 + threadIdx.x;
-    if (qs >= q1) continue;
+    tile_of(a, nq, it.t, q0, q1, gi);
+    const uint32_t qi = q0 + threadIdx.x;
+    const bool active = qi < q1;
+    const uint32_t qs = active ? (a.qsel ? a.qsel[qi] : qi) : 0u;
     const GridDesc g = a.grids[gi];
-    const float4 qv = a.queries[qs];
+    const float4 qv = active ? a.queries[qs] : make_float4(0.f, 0.f, 0.f, 0.f);
     const uint32_t row = __float_as_uint(qv.w);
-    const float R = a.qradius ? a.qradius[row] : a.r;
+    const float R = a.qradius && active ? fminf(a.r, a.qradius[qs] * a.qradius_scale) : a.r;
     const float R2 = R * R;
     int lo0, hi0, lo1, hi1, lo2, hi2;
     cell_range(g, qv.x, R, 0, lo0, hi0);
     cell_range(g, qv.y, R, 1, lo1, hi1);
     cell_range(g, qv.z, R, 2, lo2, hi2);
+    const int ny = hi1 - lo1 + 1, nz = hi2 - lo2 + 1;
+    const int nrows = (active && hi0 >= lo0 && ny > 0 && nz > 0) ? ny * nz : 0;
 
     uint64_t list[KC];
 #pragma unroll
     for (int i = 0; i < KC; ++i) list[i] = ~0ull;
+    // Hits below the current k-th wait in a 4-entry queue; the insertion
+    // network runs when some lane's queue is full, so its cost is paid once per
+    // four hits instead of on every candidate where any lane hits.
+    uint64_t pq0 = ~0ull, pq1 = ~0ull, pq2 = ~0ull, pq3 = ~0ull;
+    int pn = 0;
 
-    for (int cz = lo2; cz <= hi2; ++cz) {
-      for (int cy = lo1; cy <= hi1; ++cy) {
-        const uint32_t rowc =
-            g.cell_base + ((uint32_t)cz * (uint32_t)g.dim[1] + (uint32_t)cy) * (uint32_t)g.dim[0];
-        if (lo0 > hi0) break;
-        const uint32_t s0 = a.cell_start[rowc + lo0];
-        const uint32_t s1 = a.cell_start[rowc + hi0 + 1];
-        for (uint32_t j = s0; j < s1; ++j) {
-          const float4 p = a.spts[j];
-          const float d2 = d2_of(qv.x, qv.y, qv.z, p.x, p.y, p.z);
-          const bool ok = KNN ? (d2 > 0.0f && d2 < R2) : (d2 < R2);
-          if (ok) {
-            const uint64_t key = pack_key(d2, __float_as_uint(p.w));
-            if (key < list[KC - 1]) topk_insert<KC>(list, key);
-          }
+    // The rows of the block are walked as one flattened candidate stream; the
+    // bounds of the next row are loaded one row ahead.
+    auto row_bounds = [&](int rr, uint32_t& s, uint32_t& e) {
+      const int cy = lo1 + rr % ny, cz = lo2 + rr / ny;
+      const uint32_t rowc =
+          g.cell_base + ((uint32_t)cz * (uint32_t)g.dim[1] + (uint32_t)cy) * (uint32_t)g.dim[0];
+      s = a.cell_start[rowc + lo0];
+      e = a.cell_start[rowc + hi0 + 1];
+    };
+    uint32_t s = 0, e = 0, ns = 0, ne = 0;
+    int rr = 0;  // row held in (ns, ne)
+    if (nrows > 0) row_bounds(0, ns, ne);
+    for (;;) {
+      if (s == e) {
+        if (rr >= nrows) break;
+        s = ns;
+        e = ne;
+        ++rr;
+        if (rr < nrows) row_bounds(rr, ns, ne);
+        continue;
+      }
+      const float4 p = a.spts[s++];
+      const float d2 = d2_of(qv.x, qv.y, qv.z, p.x, p.y, p.z);
+      const bool ok = KNN ? (d2 > 0.0f && d2 < R2) : (d2 < R2);
+      const uint64_t key = pack_key(d2, __float_as_uint(p.w));
+      if (ok && key < list[KC - 1]) {
+        pq3 = pq2; pq2 = pq1; pq1 = pq0; pq0 = key;
+        ++pn;
+      }
+      if (__any(pn == 4)) {
+#pragma unroll 1
+        for (int f = 0; f < 4; ++f) {
+          topk_insert<KC>(list, pq0);
+          pq0 = pq1; pq1 = pq2; pq2 = pq3; pq3 = ~0ull;
         }
+        pn = 0;
       }
     }
+#pragma unroll 1
+    for (int f = 0; f < 4; ++f) {
+      topk_insert<KC>(list, pq0);
+      pq0 = pq1; pq1 = pq2; pq2 = pq3; pq3 = ~0ull;
+    }
+    if (!active) continue;
 
     const uint32_t k = a.k;
     const uint32_t base = a.local_index ? g.point_base : 0u;
@@ -206,14 +247,16 @@ __global__ __launch_bounds__(kTileQ) void k_topk_group(const SearchArgs a) {
   uint32_t* ro = s_ro[gq];
   const uint32_t k = a.k;
 
-  for (TileIter it(a); it.t < it.end; it.t += it.step) {
+  const uint32_t nq = launch_nq(a);
+  for (TileIter it(a, nq); it.t < it.end; it.t += it.step) {
     uint32_t q0, q1, gi;
-    tile_of(a, it.t, q0, q1, gi);
+    tile_of(a, nq, it.t, q0, q1, gi);
     const GridDesc g = a.grids[gi];
-    for (uint32_t qs = q0 + gq; qs < q1; qs += C::NG) {
+    for (uint32_t qi = q0 + gq; qi < q1; qi += C::NG) {
+      const uint32_t qs = a.qsel ? a.qsel[qi] : qi;
       const float4 qv = a.queries[qs];
       const uint32_t row = __float_as_uint(qv.w);
-      const float R = a.qradius ? a.qradius[row] : a.r;
+      const float R = a.qradius ? fminf(a.r, a.qradius[qs] * a.qradius_scale) : a.r;
       const float R2 = R * R;
       int lo0, hi0, lo1, hi1, lo2, hi2;
       cell_range(g, qv.x, R, 0, lo0, hi0);
@@ -333,6 +376,7 @@ hipError_t launch_search(const SearchArgs& a0, int knn_mode, int group_lanes, hi
                          int num_cus) {
   SearchArgs a = a0;
   if (a.ntiles == 0) return hipSuccess;
+  if (a.nq_dev) a.persistent = 1;  // tile count known only on the device
   const uint32_t k = a.k;
   if (k == 0 || k > max_k_for_group(group_lanes)) return hipErrorInvalidValue;
   uint32_t blocks = a.ntiles;

@@ -1,0 +1,78 @@
+"""The rtnn command-line driver: flag surface of the reference's optixNSearch
+(util.cpp:137-383) and its result file."""
+import os
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, golden_d2
+
+CLI = os.path.join(ROOT, "rtnn_amd", "rtnn")
+SAMPLE = os.path.join(ROOT, "tests", "golden", "samplepc.txt")
+
+
+def run(*args):
+    return subprocess.run([CLI, *args], stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                          text=True, timeout=300)
+
+
+def read_result(path):
+    with open(path, "rb") as f:
+        assert f.read(8) == b"RTNNRES1"
+        nq, k, flags = struct.unpack("<QII", f.read(16))
+        idx = np.frombuffer(f.read(4 * nq * k), np.int32).reshape(nq, k)
+        d2 = np.frombuffer(f.read(4 * nq * k), np.float32).reshape(nq, k) if flags & 1 else None
+    return idx, d2
+
+
+def test_help_and_bad_flags():
+    r = run("-h")
+    assert r.returncode == 0 and "--searchmode" in r.stderr
+    assert run("-f", SAMPLE, "--bogus", "1").returncode != 0
+    assert run("-f", SAMPLE, "-sm", "nearest").returncode != 0
+    assert run("-f", SAMPLE, "-crs", "1.0").returncode != 0
+    assert run("-sm", "knn").returncode != 0  # no point file
+
+
+def test_missing_and_malformed_files(tmp_path):
+    r = run("-f", str(tmp_path / "nope.txt"))
+    assert r.returncode != 0
+    bad = tmp_path / "bad.txt"
+    bad.write_text("1,2,3\n4,5\n")
+    assert run("-f", str(bad)).returncode != 0
+    empty = tmp_path / "empty.txt"
+    empty.write_text("")
+    r = run("-f", str(empty))
+    assert r.returncode == 0 and "empty query and/or points" in r.stdout
+
+
+@pytest.mark.gpu
+def test_cli_matches_golden(golden, tmp_path):
+    out = str(tmp_path / "res.bin")
+    r = run("-f", SAMPLE, "-r", "2", "-k", "50", "-c", "1", "-o", out)
+    assert r.returncode == 0, r.stderr
+    assert "Sanity check done." in r.stderr and "time total search time" in r.stdout
+    idx, d2 = read_result(out)
+    case = golden["cases"][0]
+    assert idx.tolist() == case["idx"]
+    assert np.array_equal(d2, golden_d2(case["d2"]))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("extra", [[], ["-p", "0"], ["--qpw", "16"]])
+def test_cli_knn_separate_queries(tmp_path, extra):
+    import oracle
+    rng = np.random.default_rng(3)
+    p = rng.uniform(0, 20, (5000, 3)).astype(np.float32)
+    q = rng.uniform(0, 20, (700, 3)).astype(np.float32)
+    pf, qf, out = tmp_path / "p.txt", tmp_path / "q.txt", str(tmp_path / "o.bin")
+    np.savetxt(pf, p, fmt="%.9g", delimiter=",")
+    np.savetxt(qf, q, fmt="%.9g", delimiter=",")
+    r = run("-f", str(pf), "-q", str(qf), "-sm", "knn", "-r", "3", "-c", "1", "-o", out, *extra)
+    assert r.returncode == 0, r.stderr
+    idx, d2 = read_result(out)
+    k = idx.shape[1]
+    wi, wd = oracle.bruteforce(p, q, oracle.MODE_KNN, 3.0, k, threads=4)
+    assert np.array_equal(idx, wi) and np.array_equal(d2, wd)

@@ -1,0 +1,136 @@
+"""GPU tests of the learned radius proposal and the density-partitioned KNN.
+
+Numerics (tolerances written per test):
+  * features: GPU log1p via the hardware log vs numpy log1p, atol 1e-5;
+  * MFMA forward: BF16 weights/activations with FP32 accumulation vs the
+    float64 reference with the same BF16 rounding; accumulation order can
+    flip an activation's BF16 rounding, so atol 2e-2 (a wrong MFMA operand
+    layout gives O(1) errors);
+  * backward: FP32 GPU gradient vs the float64 straight-through gradient,
+    relative L2 error < 2e-2 and elementwise atol 2e-3 * max|g|.
+Exactness: learned and partitioned KNN must equal the plain KNN bit for bit,
+including when every query falls back (r' forced to its floor) and when none
+does (r' forced to r).
+"""
+import numpy as np
+import pytest
+
+import rtnn_amd
+from oracle import mlp_ref as M
+
+pytestmark = pytest.mark.gpu
+
+
+def _t(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def rand_blob(seed, scale=0.25):
+    rng = np.random.default_rng(seed)
+    p = rng.normal(0, scale, M.NP).astype(np.float32)
+    p[M.OB3] = 0.3
+    return np.concatenate([p, [0.25, 1.0]]).astype(np.float32)
+
+
+def test_features_match_reference(ctx):
+    rng = np.random.default_rng(1)
+    pts = rng.uniform(0, 60, (50000, 3)).astype(np.float32)
+    qs = np.concatenate([pts[:2000], rng.uniform(-20, 80, (2000, 3)).astype(np.float32)])
+    ctx.set_points(pts)
+    got = ctx.mlp_features(_t(qs), 5.0).cpu().numpy()
+    want = M.features(pts, qs, 5.0, 0.25)
+    assert np.abs(got - want).max() < 1e-5
+
+
+def test_mfma_forward_matches_reference(ctx):
+    blob = rand_blob(2)
+    ctx.set_mlp_weights(blob)
+    rng = np.random.default_rng(3)
+    feats = rng.uniform(0, 2, (5000, 32)).astype(np.float32)
+    feats[:, 30:] = 0
+    got = ctx.mlp_forward(_t(feats)).cpu().numpy()
+    want = M.forward(blob[:M.NP], feats)
+    err = np.abs(got - want)
+    assert err.max() < 2e-2, err.max()
+    assert np.median(err) < 2e-3
+    # weights round trip
+    w = ctx.mlp_weights()
+    assert np.array_equal(w, blob)
+
+
+def test_backward_matches_reference(ctx):
+    blob = rand_blob(4)
+    ctx.set_mlp_weights(blob)
+    rng = np.random.default_rng(5)
+    n = 1000
+    feats = rng.uniform(0, 2, (n, 32)).astype(np.float32)
+    feats[:, 30:] = 0
+    o = M.forward(blob[:M.NP], feats)
+    t = (o + np.where(rng.random(n) < 0.5, -0.3, 0.3)).astype(np.float32)
+    g, loss = ctx.mlp_backward(_t(feats), _t(t), tau=0.9)
+    g = g.cpu().numpy().astype(np.float64)
+    want = M.grad(blob[:M.NP], feats, t, 0.9)
+    assert np.linalg.norm(g - want) < 2e-2 * np.linalg.norm(want)
+    assert np.abs(g - want).max() < 2e-3 * np.abs(want).max()
+    assert abs(loss - M.loss(blob[:M.NP], feats, t, 0.9)) < 1e-4
+
+
+def _plain_vs(ctx, qs, r, k, fn):
+    import torch
+    pi, pd, _ = ctx.search(qs, mode="knn", radius=r, k=k)
+    li, ld, st = fn(qs, radius=r, k=k)
+    assert np.array_equal(pi, li)
+    assert np.array_equal(pd, ld)
+    return st
+
+
+def test_learned_training_and_exactness(ctx):
+    pts = rtnn_amd.gen_spheres(3, 30, 300000, nspheres=60, box=600.0)
+    qs = rtnn_amd.gen_spheres(3, 31, 100000, nspheres=60, box=600.0)
+    ctx.set_points(pts)
+    with rtnn_amd.Context(0) as fresh:
+        fresh.set_points(pts)
+        with pytest.raises(rtnn_amd.RtnnError, match="no weights"):
+            fresh.search_learned(qs, radius=10.0, k=16)
+    loss0 = ctx.mlp_train(10.0, 16, n_samples=1 << 15, steps=1, batch=4096, seed=7)
+    loss = ctx.mlp_train(10.0, 16, n_samples=1 << 15, steps=400, batch=4096, seed=7)
+    assert loss < loss0, (loss0, loss)
+    st = _plain_vs(ctx, qs, 10.0, 16, ctx.search_learned)
+    frac = st.fallback_queries / len(qs)
+    assert frac < 0.5, frac
+    # training is deterministic
+    w1 = ctx.mlp_weights()
+    ctx.mlp_train(10.0, 16, n_samples=1 << 15, steps=400, batch=4096, seed=7)
+    assert np.array_equal(w1, ctx.mlp_weights())
+
+
+def test_learned_forced_fallback_and_no_fallback(ctx):
+    rng = np.random.default_rng(9)
+    pts = rng.uniform(0, 40, (60000, 3)).astype(np.float32)
+    qs = rng.uniform(-5, 45, (20000, 3)).astype(np.float32)
+    ctx.set_points(pts)
+    blob = np.zeros(M.NP + 2, np.float32)
+    blob[M.NP] = 0.25
+    blob[M.NP + 1] = 1.0
+    ctx.set_mlp_weights(blob)  # o = 0 -> r' at its floor -> (almost) all fall back
+    st = _plain_vs(ctx, qs, 3.0, 8, ctx.search_learned)
+    assert st.fallback_queries > 0.9 * len(qs)
+    blob[M.OB3] = 10.0  # o = 10 -> r' = r -> nothing beyond the plain short rows
+    ctx.set_mlp_weights(blob)
+    st = _plain_vs(ctx, qs, 3.0, 8, ctx.search_learned)
+    pi, _, _ = ctx.search(qs, mode="knn", radius=3.0, k=8)
+    assert st.fallback_queries == int((pi[:, -1] < 0).sum())
+
+
+@pytest.mark.parametrize("k", [1, 8, 16, 32])
+def test_partitioned_equals_plain(ctx, k):
+    pts = rtnn_amd.gen_gaussian_mixture(2, 20, 100000)
+    qs = rtnn_amd.gen_gaussian_mixture(2, 21, 50000)
+    ctx.set_points(pts)
+    st = _plain_vs(ctx, qs, 20.0, k, ctx.search_partitioned)
+    assert st.search_launches == 3
+    rng = np.random.default_rng(k)
+    u = rng.uniform(0, 30, (40000, 3)).astype(np.float32)
+    ctx.set_points(u)
+    _plain_vs(ctx, u[:5000], 2.0, k, ctx.search_partitioned)

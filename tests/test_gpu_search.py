@@ -17,6 +17,8 @@ from conftest import golden_d2
 pytestmark = pytest.mark.gpu
 
 MODES = {"radius": oracle.MODE_RADIUS, "knn": oracle.MODE_KNN}
+# lanes per query (64 / queries_per_wave) -> largest k of that kernel
+MAX_K = {1: 64, 4: 64, 16: 256, 64: 1024}
 
 
 def check(ctx, pts, qs, mode, r, k, qsel=None, **search_kw):
@@ -44,7 +46,7 @@ def test_samplepc_golden(ctx, golden, samplepc):
     ctx.set_points(samplepc)
     for case in golden["cases"]:
         for qpw in (0, 1, 4, 16, 64):
-            if case["k"] > 64 and qpw in (1, 4):
+            if qpw and case["k"] > MAX_K[64 // qpw]:
                 continue
             ctx.set_options(queries_per_wave=qpw)
             idx, d2, _ = ctx.search(None, mode=case["mode"], radius=case["r"], k=case["k"])
@@ -88,7 +90,8 @@ def test_large_k_group_kernel(ctx):
     rng = np.random.default_rng(4)
     pts = rng.uniform(0, 10, (30000, 3)).astype(np.float32)
     ctx.set_points(pts)
-    for qpw, k in [(16, 100), (16, 256), (64, 300), (64, 1024), (0, 200)]:
+    # queries_per_wave 4 -> 16 lanes per query, 1 -> 64 lanes per query
+    for qpw, k in [(4, 100), (4, 256), (1, 300), (1, 1024), (0, 200)]:
         ctx.set_options(queries_per_wave=qpw)
         check(ctx, pts, pts[:2000], "radius", 1.2, k)
         check(ctx, pts, pts[:500], "knn", 1.2, k)
@@ -100,7 +103,7 @@ def test_group_buffer_overflow_path(ctx):
     rng = np.random.default_rng(8)
     pts = rng.normal(0, 1.0, (60000, 3)).astype(np.float32)
     ctx.set_points(pts)
-    for qpw in (4, 16, 64):
+    for qpw in (1, 4, 16):  # 64, 16 and 4 lanes per query
         ctx.set_options(queries_per_wave=qpw)
         check(ctx, pts, pts[:1500], "radius", 2.0, 16)
         check(ctx, pts, pts[:1500], "knn", 2.0, 64)
