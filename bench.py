@@ -56,7 +56,39 @@ def workload(name):
                     cfg={"workload": "KNN K=8 r=20: 1M Gaussian-mixture points (64 x sigma 5 "
                                      "+ 20% background), 1M queries",
                          "points": 1_000_000, "queries": 1_000_000, "k": 8, "radius": 20.0})
+    if name == "c5":
+        return dict(pts=rtnn_amd.gen_levy(5, 30_000_000, alpha=1.5, min_step=0.01, box=1000.0),
+                    qs=rtnn_amd.gen_uniform(51, 30_000_000, 0.0, 1000.0), mode="knn", r=100.0,
+                    k=32, methods=["partitioned", "learned"],
+                    cfg={"workload": "KNN K=32 r=100: 30M Levy-flight points (Pareto alpha 1.5, "
+                                     "min step 0.01, periodic [0,1000]^3), 30M uniform queries",
+                         "points": 30_000_000, "queries": 30_000_000, "k": 32, "radius": 100.0})
+    if name == "c4":
+        xyz, offs = rtnn_amd.gen_lattice_clouds(4, 4096, 1000, 200000, h=0.1, jit=0.02)
+        return dict(pts=xyz, qs=None, offsets=offs, mode="radius", r=0.2, k=64,
+                    methods=["percloud", "batched"],
+                    cfg={"workload": "batched range r=0.2 K=64: 4096 jittered cubic lattices "
+                                     "(h=0.1, jitter 0.02h), sizes log-uniform in [1e3, 2e5], "
+                                     "queries = points",
+                         "clouds": 4096, "points": int(offs[-1]), "k": 64, "radius": 0.2})
     raise SystemExit(f"unknown workload {name}")
+
+
+def c27_bytes_clouds(xyz, offs, r, k, sample, seed=5):
+    """Range roofline bytes for many clouds: the exact C27 (counted inside each
+    query's own cloud) of `sample` random queries, its mean scaled to all."""
+    rng = np.random.default_rng(seed)
+    n = int(offs[-1])
+    qi = np.sort(rng.choice(n, min(sample, n), replace=False))
+    cloud = np.searchsorted(offs, qi, side="right") - 1
+    tot = 0
+    for c in np.unique(cloud):
+        a, b = int(offs[c]), int(offs[c + 1])
+        sel = qi[cloud == c]
+        pts = xyz[a:b]
+        tot += c27_bytes(pts, xyz[sel], r, k) - len(sel) * (12 + 4 * k)
+    c27_mean = tot / 12 / len(qi)
+    return float(n * (12 + 4 * k) + 12 * c27_mean * n), c27_mean
 
 
 def c27_bytes(pts, qs, r, k):
@@ -112,6 +144,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--train-samples", type=int, default=1 << 21)
     ap.add_argument("--train-steps", type=int, default=2000)
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="time the default configuration instead of the autotuned one")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (scale factor for HBM counters)")
     args = ap.parse_args()
@@ -142,7 +176,8 @@ def main():
     dp = torch.from_numpy(pts).cuda()
     dq = torch.from_numpy(qs).cuda() if qs is not None else None
     ctx = rtnn_amd.Context(0)
-    ctx.set_points(dp)
+    if "offsets" not in w:
+        ctx.set_points(dp)
 
     results = {}
     outputs = {}
@@ -152,6 +187,12 @@ def main():
             fn = lambda: ctx.search(dq, mode=mode, radius=r, k=k, out_device=True)
         elif m == "partitioned":
             fn = lambda: ctx.search_partitioned(dq, radius=r, k=k, out_device=True)
+        elif m == "percloud":
+            fn = lambda: ctx.search_percloud(dp, w["offsets"], mode=mode, radius=r, k=k,
+                                             with_d2=False)
+        elif m == "batched":
+            fn = lambda: ctx.search_batched(dp, w["offsets"], mode=mode, radius=r, k=k,
+                                            with_d2=False)
         else:
             t0 = time.perf_counter()
             loss = ctx.mlp_train(r, k, n_samples=args.train_samples, steps=args.train_steps,
@@ -161,25 +202,40 @@ def main():
                      "lr": 1e-3, "tau": 0.9}
             log(f"[bench] MLP trained on the GPU in {train['seconds']} s, loss {loss:.5f}")
             fn = lambda: ctx.search_learned(dq, radius=r, k=k, out_device=True)
+        ctx.set_options()
+        tuned = None
+        if args.autotune and m in ("plain", "partitioned", "learned"):
+            t0 = time.perf_counter()
+            opt, tms = ctx.autotune(dq, method=m, mode=mode, radius=r, k=k)
+            tuned = dict(opt, search_ms=round(tms, 4), seconds=round(time.perf_counter() - t0, 3))
+            log(f"[bench] {m}: autotuned {tuned}")
         ms, agg, out = timed(fn, args.steps, args.warmup, m)
         results[m] = dict(ms_per_step=round(ms, 4), queries_per_s=round(nq / (ms * 1e-3), 1),
                           **{kk: (round(v, 4) if isinstance(v, float) else v)
                              for kk, v in agg.items()})
+        if tuned:
+            results[m]["autotuned"] = tuned
         outputs[m] = (out[0], out[1])
 
-    # every row of every method equals the plain grid result
+    # every row of every method equals the first (plain / per-cloud) result
     base_i, base_d = outputs[w["methods"][0]]
     for m in w["methods"][1:]:
-        same = bool(torch.equal(outputs[m][0], base_i) and torch.equal(outputs[m][1], base_d))
-        results[m]["identical_to_plain_all_rows"] = same
-        log(f"[bench] {m} identical to plain on all {nq} rows: {same}")
+        same = bool(torch.equal(outputs[m][0], base_i) and
+                    (base_d is None or torch.equal(outputs[m][1], base_d)))
+        results[m]["identical_to_" + w["methods"][0] + "_all_rows"] = same
+        log(f"[bench] {m} identical to {w['methods'][0]} on all {nq} rows: {same}")
 
     head = w["methods"][-1]
     # roofline of the dominant search kernel of the headline method
     kms = results[head]["ms_search"]
+    c27_note = None
     if mode == "knn":
         algo = nq * (12 + 12 * len(pts) / nq + 8 * k)
         algo_def = "KNN compulsory bytes per query 12 + 12*N/Q + 8*K"
+    elif "offsets" in w:
+        algo, c27m = c27_bytes_clouds(pts, w["offsets"], r, k, sample=2_000_000)
+        algo_def = "range bytes per query 12 + 12*C27(q) + 4*K"
+        c27_note = f"C27 mean {c27m:.2f} from 2M sampled queries, scaled to all queries"
     else:
         algo = c27_bytes(pts, pts if qs is None else qs, r, k)
         algo_def = "range bytes per query 12 + 12*C27(q) + 4*K"
@@ -188,33 +244,61 @@ def main():
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": kname,
             "kernel_ms": round(kms, 4), "algorithmic_bytes": algo, "bytes_definition": algo_def}
+    if c27_note:
+        roof["c27"] = c27_note
 
     # exactness on a sample + CPU baseline (oracle brute force on the sample)
     import oracle
-    qarr = pts if qs is None else qs
     om = oracle.MODE_KNN if mode == "knn" else oracle.MODE_RADIUS
-    ns = min(args.cpu_sample, nq)
-    sel = np.random.default_rng(1234).choice(nq, ns, replace=False).astype(np.uint64)
     cores = min(16, os.cpu_count() or 1)
     cpu, mism = None, None
     if not args.no_cpu:
-        t = time.perf_counter()
-        wi, wd = oracle.bruteforce(pts, qarr, om, r, k, qsel=sel, threads=cores)
-        dt = time.perf_counter() - t
-        s = torch.from_numpy(sel.astype(np.int64)).cuda()
-        mism = {}
-        for m, (oi, od) in outputs.items():
-            gi, gd = oi[s].cpu().numpy(), od[s].cpu().numpy()
-            mism[m] = int(((gi != wi).any(axis=1) | (gd != wd).any(axis=1)).sum())
+        rng = np.random.default_rng(1234)
+        mism = {m: 0 for m in outputs}
+        if "offsets" in w:  # brute force inside sampled clouds
+            offs = w["offsets"]
+            clouds = rng.choice(len(offs) - 1, 40, replace=False)
+            per = max(1, args.cpu_sample // len(clouds))
+            t = time.perf_counter()
+            ns = 0
+            for c in clouds:
+                a, b = int(offs[c]), int(offs[c + 1])
+                sel = rng.choice(b - a, min(per, b - a), replace=False).astype(np.uint64)
+                wi, wd = oracle.bruteforce(pts[a:b], pts[a:b], om, r, k, qsel=sel, threads=cores)
+                rows = torch.from_numpy(sel.astype(np.int64) + a).cuda()
+                for m, (oi, od) in outputs.items():
+                    mism[m] += int((oi[rows].cpu().numpy() != wi).any(axis=1).sum())
+                ns += len(sel)
+            dt = time.perf_counter() - t
+            sample = (f"{ns} random queries from 40 random clouds, brute force over their "
+                      f"cloud ({dt:.1f} s)")
+        else:
+            qarr = pts if qs is None else qs
+            ns = min(args.cpu_sample, nq)
+            sel = rng.choice(nq, ns, replace=False).astype(np.uint64)
+            t = time.perf_counter()
+            wi, wd = oracle.bruteforce(pts, qarr, om, r, k, qsel=sel, threads=cores)
+            dt = time.perf_counter() - t
+            rows = torch.from_numpy(sel.astype(np.int64)).cuda()
+            for m, (oi, od) in outputs.items():
+                bad = (oi[rows].cpu().numpy() != wi).any(axis=1)
+                if od is not None:
+                    bad |= (od[rows].cpu().numpy() != wd).any(axis=1)
+                mism[m] = int(bad.sum())
+            sample = (f"{ns} random queries of this workload, brute force over all "
+                      f"{len(pts)} points ({dt:.1f} s)")
         cpu = {"value": round(ns / dt, 3), "unit": "queries/s", "cores": cores, "kind": "port",
-               "sample": f"{ns} random queries of this workload, brute force over all "
-                         f"{len(pts)} points ({dt:.1f} s)"}
+               "sample": sample}
         log(f"[bench] cpu baseline {ns / dt:.1f} q/s on {cores} threads; mismatches {mism}")
 
     value = results[head]["queries_per_s"]
-    cfg = dict(w["cfg"], method=head, step="grid build + query sort + radius proposal + search "
-                                          "+ exact fallback" if head == "learned" else
-                                          "grid build + query sort + search")
+    steps_desc = {"learned": "grid build + query sort + radius proposal + search + exact fallback",
+                  "partitioned": "grid build + query sort + density radius + search + exact "
+                                 "fallback",
+                  "batched": "per-cloud bbox + one grid build for all clouds + one persistent "
+                             "search launch",
+                  "plain": "grid build + query sort + search"}
+    cfg = dict(w["cfg"], method=head, step=steps_desc.get(head, head))
     out = {
         "metric": METRIC if args.workload == "c3" else f"{mode} queries/s ({args.workload})",
         "value": value, "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
@@ -223,9 +307,10 @@ def main():
         "data": "synthetic", "config": cfg, "roofline": roof, "cpu_baseline": cpu,
         "methods": results, "sampled_mismatches": mism, "mlp_training": train,
     }
-    if "plain" in results and head != "plain":
-        out["speedup_vs_plain"] = round(results["plain"]["ms_per_step"] /
-                                        results[head]["ms_per_step"], 3)
+    base = w["methods"][0]
+    if head != base:
+        out["speedup_vs_" + base] = round(results[base]["ms_per_step"] /
+                                          results[head]["ms_per_step"], 3)
     print(json.dumps(out), flush=True)
     ctx.close()
 

@@ -55,6 +55,11 @@ typedef struct {
   int   query_order;      /* 0 auto, 1 input order, 2 Morton order of query cells  */
   int   time_stages;      /* 1: record hipEvent timings into rtnn_stats             */
   uint32_t max_cells;     /* cap on grid cells (cell edge grows to respect it)      */
+  int   grid_levels;      /* learned/partitioned KNN: 1 = one raster grid of edge
+                             r / cell_factor; >= 2 = hierarchical Morton grid with
+                             that many levels (edges r, r/2, ...); 0 = auto       */
+  float level_alpha;      /* hierarchical: a query of radius R walks the finest
+                             level whose cell edge is >= R * level_alpha (0: 1.0) */
 } rtnn_options;
 
 /* Filled by search entry points when a non-NULL pointer is passed. */
@@ -153,6 +158,22 @@ int rtnn_search_batched(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets,
 int rtnn_search_percloud(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets,
                          uint32_t nclouds, int mode, float r, uint32_t k,
                          int32_t* out_idx, float* out_d2, rtnn_stats* stats);
+
+/* ---- autotuner --------------------------------------------------------- */
+
+#define RTNN_METHOD_PLAIN       0
+#define RTNN_METHOD_PARTITIONED 1
+#define RTNN_METHOD_LEARNED     2
+
+/* Times every (cell_factor, queries_per_wave) candidate of the method on this
+ * exact workload (same queries, r, k) and keeps the fastest in the context's
+ * options (also returned in *best; *best_ms = its time per search).  Replaces
+ * the reference's analytic cell-size and batching models (calcCRRatio,
+ * util.cpp:551-676; autoBatchingRange/KNN, sort.cpp:221-320) with
+ * measurement.  out_idx/out_d2 are scratch outputs of size nq x k (device). */
+int rtnn_autotune(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_device, int method,
+                  int mode, float r, uint32_t k, int32_t* out_idx, float* out_d2,
+                  rtnn_options* best, double* best_ms);
 
 /* ---- building blocks (exported for parity tests; device pointers) ---- */
 

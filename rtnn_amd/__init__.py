@@ -45,7 +45,9 @@ class Options(ctypes.Structure):
                 ("queries_per_wave", ctypes.c_int),
                 ("query_order", ctypes.c_int),
                 ("time_stages", ctypes.c_int),
-                ("max_cells", ctypes.c_uint32)]
+                ("max_cells", ctypes.c_uint32),
+                ("grid_levels", ctypes.c_int),
+                ("level_alpha", ctypes.c_float)]
 
 
 class Stats(ctypes.Structure):
@@ -97,6 +99,8 @@ SIGNATURES = {
                                  ctypes.POINTER(Stats)]),
     "rtnn_search_percloud": (_I, [_VP, _VP, _VP, _U32, _I, _F, _U32, _VP, _VP,
                                   ctypes.POINTER(Stats)]),
+    "rtnn_autotune": (_I, [_VP, _VP, _U64, _I, _I, _I, _F, _U32, _VP, _VP,
+                           ctypes.POINTER(Options), ctypes.POINTER(ctypes.c_double)]),
     "rtnn_radix_sort_pairs": (_I, [_VP, _VP, _VP, _U64, _U32]),
     "rtnn_exclusive_scan": (_I, [_VP, _VP, _VP, _U64]),
     "rtnn_morton3": (_I, [_VP, _VP, _VP, _U64]),
@@ -202,8 +206,9 @@ class Context:
             raise RtnnError(rc, self._L.rtnn_last_error(self._h).decode())
 
     def set_options(self, cell_factor=0.0, queries_per_wave=0, query_order=0,
-                    time_stages=0, max_cells=0):
-        o = Options(cell_factor, queries_per_wave, query_order, time_stages, max_cells)
+                    time_stages=0, max_cells=0, grid_levels=0, level_alpha=0.0):
+        o = Options(cell_factor, queries_per_wave, query_order, time_stages, max_cells,
+                    grid_levels, level_alpha)
         self._check(self._L.rtnn_set_options(self._h, ctypes.byref(o)))
 
     def synchronize(self):
@@ -303,6 +308,28 @@ class Context:
     def search_percloud(self, xyz, offsets, mode="radius", radius=0.2, k=64, with_d2=True):
         return self._batched(self._L.rtnn_search_percloud, xyz, offsets, mode, radius, k,
                              with_d2)
+
+    def autotune(self, queries=None, method="plain", mode="knn", radius=10.0, k=16):
+        """Times the candidate configurations on this workload and keeps the
+        fastest; returns (options dict, ms per search)."""
+        import torch
+        meth = {"plain": 0, "partitioned": 1, "learned": 2}[method]
+        if queries is None:
+            nq, qptr, qdev, keep = len(self._points), None, 0, None
+        else:
+            keep = _as_points(queries)
+            qptr, qdev = _ptr(keep)
+            nq = len(keep)
+        idx = torch.empty((nq, k), dtype=torch.int32, device="cuda")
+        d2 = torch.empty((nq, k), dtype=torch.float32, device="cuda")
+        best = Options()
+        ms = ctypes.c_double(0)
+        self._check(self._L.rtnn_autotune(self._h, qptr, nq, qdev, meth, _mode(mode), radius, k,
+                                          ctypes.c_void_p(idx.data_ptr()),
+                                          ctypes.c_void_p(d2.data_ptr()), ctypes.byref(best),
+                                          ctypes.byref(ms)))
+        return ({"cell_factor": best.cell_factor, "queries_per_wave": best.queries_per_wave,
+                 "grid_levels": best.grid_levels, "level_alpha": best.level_alpha}, ms.value)
 
     # MLP building blocks (device tensors)
     def mlp_features(self, queries, radius):

@@ -77,6 +77,8 @@ struct rtnn_ctx_s {
   uint64_t ncells = 0;
   int sort_passes = 0;
   bool spts_current = false;  // spts/cell_start describe ctx->pts
+  int hlevels = 0;            // > 0: grids/cell_start hold a hierarchical Morton grid
+  int hlevel_min = 0;
 
   // queries / outputs
   DevBuf qin, qkeys, qvals, qsorted, oidx, od2, shortrows, misc, tiles, offs, bboxes;
@@ -196,17 +198,91 @@ int build_grid(rtnn_ctx ctx, float cell) {
   CK(ctx->vals.ensure(4 * n));
   CK(ctx->cell_start.ensure(4 * (ctx->ncells + 1)));
   CK(ctx->spts.ensure(16 * std::max<uint64_t>(n, 1)));
-  CK(ctx->tmp.ensure(4 * radix_tmp_words(n)));
+  CK(ctx->tmp.ensure(4 * std::max(radix_tmp_words(n), cell_bounds_tmp_words(ctx->ncells))));
   CK(cell_keys(ctx->pts.as<float>(), n, ctx->grids.as<GridDesc>(), nullptr, 1,
                ctx->keys.as<uint32_t>(), ctx->vals.as<uint32_t>(), nullptr, ctx->st));
   CK(radix_sort_pairs(ctx->keys.as<uint32_t>(), ctx->vals.as<uint32_t>(), n,
                       bits_for(ctx->ncells - 1), ctx->tmp.as<uint32_t>(), ctx->st,
                       &ctx->sort_passes));
   CK(cell_bounds(ctx->keys.as<uint32_t>(), n, (uint32_t)ctx->ncells,
-                 ctx->cell_start.as<uint32_t>(), ctx->st));
+                 ctx->cell_start.as<uint32_t>(), ctx->tmp.as<uint32_t>(), ctx->st));
   CK(gather_points(ctx->pts.as<float>(), ctx->vals.as<uint32_t>(), n, ctx->spts.as<float4>(),
                    ctx->st));
   ctx->spts_current = true;
+  ctx->hlevels = 0;
+  return RTNN_OK;
+}
+
+// Hierarchical Morton grid over ctx->pts: level l has cell edge r * 2^(l-L+1)
+// (the top level's edge is exactly r) on a cubic grid of 2^(b-l) cells per
+// axis.  Points are sorted once by their level-0 Morton key; a level-l cell is
+// the key range sharing key >> 3l, so every level's table is built from the
+// same sorted keys.  Level tables larger than 2^27 cells are skipped (queries
+// use the finest built level).
+int build_hgrid(rtnn_ctx ctx, float r, int want_levels) {
+  int L = std::max(2, std::min(want_levels, 8));
+  float c0 = 0, inv0 = 0;
+  int b = 0;
+  for (;;) {
+    c0 = std::ldexp(r, -(L - 1));
+    inv0 = 1.0f / c0;
+    int dmax = 1;
+    for (int a = 0; a < 3; ++a) {
+      volatile float ext = ctx->bbox[a + 3] - ctx->bbox[a];
+      volatile float e = ext * inv0;
+      dmax = std::max(dmax, (int)std::min(std::floor((float)e) + 1.0f, 2.0e9f));
+    }
+    b = std::max(bits_for((uint64_t)(dmax - 1)), L - 1);
+    if (b <= 10) break;
+    if (--L < 1) return fail(ctx, RTNN_ERR_UNSUPPORTED, "radius too small for a Morton grid");
+  }
+  const uint64_t kMaxTable = 1ull << 27;
+  int lmin = 0;
+  while (lmin < L - 1 && (1ull << (3 * (b - lmin))) > kMaxTable) ++lmin;
+  std::vector<GridDesc> lv(L);
+  uint64_t total = 0;
+  for (int l = 0; l < L; ++l) {
+    GridDesc& g = lv[l];
+    std::memset(&g, 0, sizeof g);
+    for (int a = 0; a < 3; ++a) {
+      g.mn[a] = ctx->bbox[a];
+      g.dim[a] = 1 << (b - l);
+    }
+    g.cell = std::ldexp(c0, l);
+    g.inv = std::ldexp(inv0, -l);  // exactly 2^-l * inv0: same cells as key >> 3l
+    g.npoints = (uint32_t)ctx->n;
+    g.morton = 1;
+    if (l >= lmin) {
+      g.cell_base = (uint32_t)total;
+      total += (1ull << (3 * (b - l))) + 1;
+    }
+  }
+  if (total >= 0xfffffff0ull) return fail(ctx, RTNN_ERR_UNSUPPORTED, "Morton tables too large");
+  const uint64_t n = ctx->n;
+  ctx->hgrid = lv[0];
+  CK(ctx->grids.ensure(sizeof(GridDesc) * L));
+  CK(hipMemcpyAsync(ctx->grids.p, lv.data(), sizeof(GridDesc) * L, hipMemcpyHostToDevice,
+                    ctx->st));
+  CK(ctx->keys.ensure(4 * n));
+  CK(ctx->vals.ensure(4 * n));
+  CK(ctx->cell_start.ensure(4 * total));
+  CK(ctx->spts.ensure(16 * std::max<uint64_t>(n, 1)));
+  CK(ctx->tmp.ensure(
+      4 * std::max(radix_tmp_words(n), cell_bounds_tmp_words(1ull << (3 * (b - lmin))))));
+  CK(morton_keys(ctx->pts.as<float>(), n, ctx->grids.as<GridDesc>(), ctx->keys.as<uint32_t>(),
+                 ctx->vals.as<uint32_t>(), ctx->st));
+  CK(radix_sort_pairs(ctx->keys.as<uint32_t>(), ctx->vals.as<uint32_t>(), n, 3 * b,
+                      ctx->tmp.as<uint32_t>(), ctx->st, &ctx->sort_passes));
+  for (int l = lmin; l < L; ++l)
+    CK(cell_bounds(ctx->keys.as<uint32_t>(), n, (uint32_t)(1ull << (3 * (b - l))),
+                   ctx->cell_start.as<uint32_t>() + lv[l].cell_base, ctx->tmp.as<uint32_t>(),
+                   ctx->st, 3 * l));
+  CK(gather_points(ctx->pts.as<float>(), ctx->vals.as<uint32_t>(), n, ctx->spts.as<float4>(),
+                   ctx->st));
+  ctx->ncells = total;
+  ctx->spts_current = true;
+  ctx->hlevels = L;
+  ctx->hlevel_min = lmin;
   return RTNN_OK;
 }
 
@@ -312,6 +388,11 @@ SearchArgs base_args(rtnn_ctx ctx, const float4* qproc, uint64_t nq, float r, ui
   a.k = k;
   a.out_idx = o.idx;
   a.out_d2 = o.d2;
+  if (ctx->hlevels > 0) {
+    a.nlevels = ctx->hlevels;
+    a.level_min = ctx->hlevel_min;
+    a.level_alpha = ctx->opt.level_alpha > 0 ? ctx->opt.level_alpha : 1.0f;
+  }
   return a;
 }
 
@@ -387,6 +468,10 @@ int rtnn_set_options(rtnn_ctx ctx, const rtnn_options* opt) {
     return fail(ctx, RTNN_ERR_INVALID, "queries_per_wave must be 0, 1, 4, 16 or 64");
   if (opt->query_order < 0 || opt->query_order > 2)
     return fail(ctx, RTNN_ERR_INVALID, "query_order must be 0, 1 or 2");
+  if (opt->grid_levels < 0 || opt->grid_levels > 8)
+    return fail(ctx, RTNN_ERR_INVALID, "grid_levels must be in [0, 8]");
+  if (!(opt->level_alpha >= 0.0f) || !std::isfinite(opt->level_alpha))
+    return fail(ctx, RTNN_ERR_INVALID, "level_alpha must be >= 0");
   ctx->opt = *opt;
   return RTNN_OK;
 }
@@ -539,7 +624,10 @@ int proposal_search(rtnn_ctx ctx, Proposal kind, const float* qxyz, uint64_t nq,
   CK(hipEventRecord(ctx->ev[0], ctx->st));
   const float f = ctx->opt.cell_factor > 0 ? ctx->opt.cell_factor
                                            : (kind == Proposal::kLearned ? 2.0f : 4.0f);
-  rc = build_grid(ctx, r / f);
+  const int levels = ctx->opt.grid_levels > 0 ? ctx->opt.grid_levels
+                                              : (kind == Proposal::kLearned ? 1 : 5);
+  const bool hier = levels >= 2;
+  rc = hier ? build_hgrid(ctx, r, levels) : build_grid(ctx, r / f);
   if (rc) return rc;
   const float4* qproc = nullptr;
   rc = prepare_queries(ctx, qxyz, nq, on_device, &qproc);
@@ -547,11 +635,25 @@ int proposal_search(rtnn_ctx ctx, Proposal kind, const float* qxyz, uint64_t nq,
   CK(hipEventRecord(ctx->ev[1], ctx->st));
   CK(ctx->qradius.ensure(4 * nq));
   if (kind == Proposal::kLearned) {
-    rc = build_count_grid(ctx, r, false);
-    if (rc) return rc;
-    CK(mlp_propose(ctx->mlp, qproc, (uint32_t)nq, ctx->pgrid.as<GridDesc>(),
-                   ctx->pcounts.as<uint32_t>(), r, ctx->qradius.as<float>(), ctx->st,
-                   ctx->num_cus));
+    // Features: point counts of the edge-r cells.  On the hierarchical grid
+    // they are the top level's table (end - start); otherwise a count grid.
+    const GridDesc* coarse;
+    const uint32_t* counts;
+    if (hier) {
+      coarse = ctx->grids.as<GridDesc>() + (ctx->hlevels - 1);
+      counts = ctx->cell_start.as<uint32_t>();
+    } else {
+      rc = build_count_grid(ctx, r, false);
+      if (rc) return rc;
+      coarse = ctx->pgrid.as<GridDesc>();
+      counts = ctx->pcounts.as<uint32_t>();
+    }
+    CK(mlp_propose(ctx->mlp, qproc, (uint32_t)nq, coarse, counts, r, ctx->qradius.as<float>(),
+                   ctx->st, ctx->num_cus));
+  } else if (hier) {
+    CK(partition_hier(qproc, (uint32_t)nq, ctx->grids.as<GridDesc>(), ctx->hlevels,
+                      ctx->hlevel_min, ctx->cell_start.as<uint32_t>(), k, r,
+                      ctx->qradius.as<float>(), ctx->st));
   } else {
     rc = build_count_grid(ctx, r / 8.0f, true);
     if (rc) return rc;
@@ -825,7 +927,7 @@ int rtnn_search_batched(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets,
   CK(ctx->vals.ensure(4 * n));
   CK(ctx->cell_start.ensure(4 * (total + 1)));
   CK(ctx->spts.ensure(16 * n));
-  CK(ctx->tmp.ensure(4 * radix_tmp_words(n)));
+  CK(ctx->tmp.ensure(4 * std::max(radix_tmp_words(n), cell_bounds_tmp_words(total))));
   ctx->spts_current = false;  // spts now holds the clouds, not ctx->pts
   CK(cell_keys(xyz, n, ctx->grids.as<GridDesc>(), ctx->offs.as<uint64_t>(), nclouds,
                ctx->keys.as<uint32_t>(), ctx->vals.as<uint32_t>(), nullptr, ctx->st));
@@ -833,7 +935,7 @@ int rtnn_search_batched(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets,
                       bits_for(total - 1), ctx->tmp.as<uint32_t>(), ctx->st,
                       &ctx->sort_passes));
   CK(cell_bounds(ctx->keys.as<uint32_t>(), n, (uint32_t)total, ctx->cell_start.as<uint32_t>(),
-                 ctx->st));
+                 ctx->tmp.as<uint32_t>(), ctx->st));
   CK(gather_points(xyz, ctx->vals.as<uint32_t>(), n, ctx->spts.as<float4>(), ctx->st));
 
   // Tiles: runs of <= kTileQ sorted queries inside one cloud.  Cell keys are
@@ -898,6 +1000,77 @@ int rtnn_search_percloud(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets
     stats->ms_total = ev_ms(ctx->ev[6], ctx->ev[7]);
     stats->search_launches = launches;
   }
+  return RTNN_OK;
+}
+
+// ------------------------------------------------------------ autotune --
+
+int rtnn_autotune(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_device, int method,
+                  int mode, float r, uint32_t k, int32_t* out_idx, float* out_d2,
+                  rtnn_options* best, double* best_ms) {
+  if (!ctx || !out_idx) return RTNN_ERR_INVALID;
+  if (method < RTNN_METHOD_PLAIN || method > RTNN_METHOD_LEARNED)
+    return fail(ctx, RTNN_ERR_INVALID, "unknown method %d", method);
+  const rtnn_options base = ctx->opt;
+  static const float kPlainCf[] = {1.0f, 2.0f, 3.0f};
+  static const float kPropCf[] = {1.5f, 2.0f, 2.5f, 3.0f};
+  static const int kQpwKnn[] = {0, 16, 4};
+  static const int kQpwRange[] = {64, 16, 4, 1};
+  const bool plain = method == RTNN_METHOD_PLAIN;
+  const float* cfs = plain ? kPlainCf : kPropCf;
+  const int ncf = plain ? 3 : 4;
+  const bool range = plain && mode == RTNN_MODE_RADIUS;
+  const int* qpws = range ? kQpwRange : kQpwKnn;
+  const int nqpw = range ? 4 : 3;
+  double best_t = 1e300;
+  rtnn_options best_opt = base;
+  // Candidates: raster grids (cell factor x lanes per query); for the radius
+  // proposal methods also hierarchical Morton grids (levels x level alpha).
+  struct Cand {
+    float cf;
+    int qpw, levels;
+    float alpha;
+  };
+  std::vector<Cand> cands;
+  for (int i = 0; i < ncf; ++i)
+    for (int j = 0; j < nqpw; ++j) cands.push_back({cfs[i], qpws[j], plain ? 0 : 1, 0.0f});
+  if (!plain)
+    for (int lv : {3, 5})
+      for (float al : {1.0f, 0.5f})
+        for (int qpw : {0, 16}) cands.push_back({0.0f, qpw, lv, al});
+  for (const Cand& cand : cands) {
+    {
+      const int qpw = cand.qpw;
+      if (qpw > 0 && k > max_k_for_group(64 / qpw)) continue;
+      if (qpw == 0 && k > 64) continue;
+      ctx->opt = base;
+      ctx->opt.cell_factor = cand.cf;
+      ctx->opt.queries_per_wave = qpw;
+      ctx->opt.grid_levels = cand.levels;
+      ctx->opt.level_alpha = cand.alpha;
+      rtnn_stats st{};
+      int rc = RTNN_OK;
+      for (int rep = 0; rep < 2 && rc == RTNN_OK; ++rep) {  // first run sizes the buffers
+        if (plain)
+          rc = search_impl(ctx, qxyz, nq, on_device, mode, r, k, out_idx, out_d2, 1, &st);
+        else
+          rc = proposal_search(ctx, method == RTNN_METHOD_LEARNED ? Proposal::kLearned
+                                                                   : Proposal::kPartition,
+                               qxyz, nq, on_device, r, k, out_idx, out_d2, 1, &st);
+      }
+      if (rc != RTNN_OK) {
+        ctx->opt = base;
+        return rc;
+      }
+      if (st.ms_total < best_t) {
+        best_t = st.ms_total;
+        best_opt = ctx->opt;
+      }
+    }
+  }
+  ctx->opt = best_opt;
+  if (best) *best = best_opt;
+  if (best_ms) *best_ms = best_t;
   return RTNN_OK;
 }
 

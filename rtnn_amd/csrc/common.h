@@ -26,7 +26,7 @@ struct GridDesc {
   uint32_t cell_base;   // first cell of this grid in the global cell table
   uint32_t point_base;  // first point of this cloud in the input order
   uint32_t npoints;
-  uint32_t pad_;
+  uint32_t morton;      // 1: cells numbered cell_base + Morton code (cubic grid)
 };
 static_assert(sizeof(GridDesc) == 48, "GridDesc layout");
 
@@ -45,7 +45,8 @@ struct SearchArgs {
   const float4* queries;     // processing order: (x, y, z, bits(output row))
   const uint32_t* qsel;      // nullable: position i of this launch reads queries[qsel[i]]
   const uint32_t* nq_dev;    // nullable: query count read on the device (persistent)
-  const Tile* tiles;         // nullable: implicit tiles of kTileQ queries, grid 0
+  const Tile* tiles;         // nullable: implicit tiles of tile_q queries, grid 0
+  uint32_t tile_q;           // queries per implicit tile (set by launch_search)
   uint32_t ntiles;
   uint32_t nq;
   float r;                   // search radius (per-query radius overrides when set)
@@ -58,6 +59,13 @@ struct SearchArgs {
   uint32_t* short_count;     // counter for short_rows
   int local_index;           // batched: report cloud-local point indices
   int persistent;            // grid-stride over tiles (XCD-chunked)
+  // Hierarchical Morton grid (nlevels > 0): grids[l], l = 0 (finest) ..
+  // nlevels-1, cubic with 2^(bits-l) cells per axis and cell 2^l * grids[0].cell;
+  // tables below level_min are not built.  A query of radius R walks the
+  // finest built level whose cell is >= R * level_alpha.
+  int nlevels;
+  int level_min;
+  float level_alpha;
 };
 
 constexpr int kTileQ = 256;  // queries per tile (= threads of a KNN block)
@@ -72,6 +80,21 @@ __host__ __device__ inline float d2_of(float qx, float qy, float qz, float px, f
   float sz = dz * dz;
   float s = sx + sy;
   return s + sz;
+}
+
+// 3D Morton (Z-order) code of 10-bit cell coordinates, the bit layout of the
+// reference's MortonCode3 (helper_mortonCode.h:19-34): bit 3i of the code is
+// bit i of x, 3i+1 of y, 3i+2 of z.
+__host__ __device__ inline uint32_t spread3(uint32_t x) {
+  x &= 0x3ffu;
+  x = (x | (x << 16)) & 0xff0000ffu;
+  x = (x | (x << 8)) & 0x0300f00fu;
+  x = (x | (x << 4)) & 0x030c30c3u;
+  x = (x | (x << 2)) & 0x09249249u;
+  return x;
+}
+__host__ __device__ inline uint32_t morton3(uint32_t x, uint32_t y, uint32_t z) {
+  return (spread3(z) << 2) | (spread3(y) << 1) | spread3(x);
 }
 
 // (d2, index) packed so unsigned 64-bit order is the lexicographic order

@@ -169,51 +169,103 @@ hipError_t cell_keys(const float* xyz, uint64_t n, const GridDesc* grids,
   return hipGetLastError();
 }
 
-// cell_start[c] = first sorted position whose key is >= c, for c in
-// [0, ncells]; thread j (0 <= j <= n) owns the cells (key[j-1], key[j]] (with
-// key[-1] = -1 and key[n] = ncells).  Every cell is written exactly once, so
-// no counts, atomics, fill or scan are needed.  Long gaps (empty space) are
-// spread over the wave: the wave scans its gap lengths and its 64 lanes write
-// the concatenated gaps in coalesced strides.
-__global__ __launch_bounds__(256) void k_cell_bounds(const uint32_t* __restrict__ keys,
-                                                     uint64_t n, uint32_t ncells,
-                                                     uint32_t* __restrict__ cs) {
-  const int lane = threadIdx.x & (kWave - 1);
-  const uint64_t j = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-  uint32_t lo = 0, len = 0;
-  if (j <= n) {
-    lo = j == 0 ? 0u : keys[j - 1] + 1u;
-    const uint32_t hi = j == n ? ncells : keys[j];
-    len = hi + 1u > lo ? hi + 1u - lo : 0u;
+// cell_start[c] = first sorted position whose key (>> shift) is >= c, for c
+// in [0, ncells] (entry ncells holds n), so a cell's points are
+// [cell_start[c], cell_start[c+1]) and no counts, atomics or scan are needed.
+// The cells are cut into tiles of kCellTile: k_tile_bounds finds each tile's
+// first sorted position by binary search (one thread per tile), then one
+// block per tile writes its cells, each a lower_bound over the tile's keys
+// staged in LDS.  Every block writes exactly one tile, however the points are
+// spread (empty space costs the same as occupied space).
+constexpr uint32_t kCellTile = 4096;
+constexpr uint32_t kCellTileKeys = 4096;  // keys staged in LDS per tile
+
+__device__ inline uint64_t lower_bound_shifted(const uint32_t* __restrict__ keys, uint64_t lo,
+                                               uint64_t hi, uint64_t c, int shift) {
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) >> 1;
+    if ((uint64_t)(keys[mid] >> shift) < c) lo = mid + 1; else hi = mid;
   }
-  uint32_t incl = len;
-#pragma unroll
-  for (int o = 1; o < kWave; o <<= 1) {
-    uint32_t t = __shfl_up(incl, o, kWave);
-    if (lane >= o) incl += t;
-  }
-  const uint32_t excl = incl - len;
-  const uint32_t total = __shfl(incl, kWave - 1, kWave);
-  const uint64_t j0 = j - (uint64_t)lane;
-  for (uint32_t base = 0; base < total; base += kWave) {
-    const uint32_t t = base + (uint32_t)lane;
-    // owner = the last lane whose exclusive offset is <= t
-    int owner = 0;
-#pragma unroll
-    for (int step = kWave / 2; step > 0; step >>= 1) {
-      const uint32_t e = __shfl(excl, owner + step, kWave);
-      if (e <= t) owner += step;
+  return lo;
+}
+
+__global__ __launch_bounds__(256) void k_tile_bounds(const uint32_t* __restrict__ keys, uint64_t n,
+                                                     uint64_t ncells1, uint32_t ntiles, int shift,
+                                                     uint64_t* __restrict__ tb) {
+  const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+  if (t > ntiles) return;
+  const uint64_t c = min((uint64_t)t * kCellTile, ncells1);
+  tb[t] = lower_bound_shifted(keys, 0, n, c, shift);
+}
+
+__global__ __launch_bounds__(256) void k_cell_fill(const uint32_t* __restrict__ keys,
+                                                   uint64_t ncells1, int shift,
+                                                   const uint64_t* __restrict__ tb,
+                                                   uint32_t* __restrict__ cs) {
+  __shared__ uint32_t s_keys[kCellTileKeys];
+  const uint32_t t = blockIdx.x;
+  const uint64_t c0 = (uint64_t)t * kCellTile;
+  const uint64_t j0 = tb[t], j1 = tb[t + 1];
+  const uint64_t m = j1 - j0;
+  const bool staged = m <= kCellTileKeys;
+  if (staged)
+    for (uint32_t i = threadIdx.x; i < m; i += 256) s_keys[i] = keys[j0 + i] >> shift;
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < kCellTile; i += 256) {
+    const uint64_t c = c0 + i;
+    if (c >= ncells1) break;
+    uint64_t pos;
+    if (staged) {
+      uint32_t lo = 0, hi = (uint32_t)m;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if ((uint64_t)s_keys[mid] < c) lo = mid + 1; else hi = mid;
+      }
+      pos = j0 + lo;
+    } else {
+      pos = lower_bound_shifted(keys, j0, j1, c, shift);
     }
-    const uint32_t olo = __shfl(lo, owner, kWave);
-    const uint32_t oex = __shfl(excl, owner, kWave);
-    if (t < total) cs[olo + (t - oex)] = (uint32_t)(j0 + (uint64_t)owner);
+    cs[c] = (uint32_t)pos;
   }
 }
 
+uint64_t cell_bounds_tmp_words(uint64_t ncells) {
+  return 2 * ((ncells + 1 + kCellTile - 1) / kCellTile + 1) + 2;
+}
+
 hipError_t cell_bounds(const uint32_t* sorted_keys, uint64_t n, uint32_t ncells,
-                       uint32_t* cell_start, hipStream_t st) {
-  hipLaunchKernelGGL(k_cell_bounds, dim3((uint32_t)((n + 1 + 255) / 256)), dim3(256), 0, st,
-                     sorted_keys, n, ncells, cell_start);
+                       uint32_t* cell_start, uint32_t* tmp, hipStream_t st, int shift) {
+  const uint64_t ncells1 = (uint64_t)ncells + 1;
+  const uint32_t ntiles = (uint32_t)((ncells1 + kCellTile - 1) / kCellTile);
+  uint64_t* tb = reinterpret_cast<uint64_t*>(tmp);
+  hipLaunchKernelGGL(k_tile_bounds, dim3((ntiles + 1 + 255) / 256), dim3(256), 0, st,
+                     sorted_keys, n, ncells1, ntiles, shift, tb);
+  hipLaunchKernelGGL(k_cell_fill, dim3(ntiles), dim3(256), 0, st, sorted_keys, ncells1, shift,
+                     tb, cell_start);
+  return hipGetLastError();
+}
+
+// Level-0 Morton key of each point's cell on grid g (cubic, 2^bits cells per
+// axis); the sort by this key makes every coarser level's cells contiguous.
+__global__ __launch_bounds__(256) void k_morton_keys(const float* __restrict__ xyz, uint64_t n,
+                                                     const GridDesc* __restrict__ grid,
+                                                     uint32_t* __restrict__ keys,
+                                                     uint32_t* __restrict__ vals) {
+  uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const GridDesc g = *grid;
+  uint32_t cx = (uint32_t)cell_of(xyz[3 * i], g.mn[0], g.inv, g.dim[0]);
+  uint32_t cy = (uint32_t)cell_of(xyz[3 * i + 1], g.mn[1], g.inv, g.dim[1]);
+  uint32_t cz = (uint32_t)cell_of(xyz[3 * i + 2], g.mn[2], g.inv, g.dim[2]);
+  keys[i] = morton3(cx, cy, cz);
+  vals[i] = (uint32_t)i;
+}
+
+hipError_t morton_keys(const float* xyz, uint64_t n, const GridDesc* g0, uint32_t* keys,
+                       uint32_t* vals, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_morton_keys, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st, xyz,
+                     n, g0, keys, vals);
   return hipGetLastError();
 }
 
@@ -295,20 +347,6 @@ hipError_t gather_points(const float* xyz, const uint32_t* vals, uint64_t n, flo
   hipLaunchKernelGGL(k_gather, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st, xyz, vals,
                      n, spts);
   return hipGetLastError();
-}
-
-// Part1By2 of helper_mortonCode.h:19-27: spread the low 10 bits of x so that
-// two zero bits follow each one.
-__host__ __device__ inline uint32_t spread3(uint32_t x) {
-  x &= 0x3ffu;
-  x = (x | (x << 16)) & 0xff0000ffu;
-  x = (x | (x << 8)) & 0x0300f00fu;
-  x = (x | (x << 4)) & 0x030c30c3u;
-  x = (x | (x << 2)) & 0x09249249u;
-  return x;
-}
-__host__ __device__ inline uint32_t morton3(uint32_t x, uint32_t y, uint32_t z) {
-  return (spread3(z) << 2) | (spread3(y) << 1) | spread3(x);
 }
 
 __global__ __launch_bounds__(256) void k_query_morton(const float* __restrict__ q, uint64_t nq,

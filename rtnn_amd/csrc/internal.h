@@ -30,9 +30,15 @@ hipError_t bbox_segmented(const float* xyz, const uint64_t* offsets, uint32_t nc
 hipError_t cell_keys(const float* xyz, uint64_t n, const GridDesc* grids,
                      const uint64_t* offsets, uint32_t nclouds, uint32_t* keys,
                      uint32_t* vals, uint32_t* counts, hipStream_t st);
-// cell_start[0..ncells] from the sorted keys (first position with key >= c).
+// cell_start[0..ncells] from the sorted keys (first position with
+// (key >> shift) >= c).  tmp: cell_bounds_tmp_words(ncells) words, 8-byte
+// aligned.
+uint64_t cell_bounds_tmp_words(uint64_t ncells);
 hipError_t cell_bounds(const uint32_t* sorted_keys, uint64_t n, uint32_t ncells,
-                       uint32_t* cell_start, hipStream_t st);
+                       uint32_t* cell_start, uint32_t* tmp, hipStream_t st, int shift = 0);
+// keys[i] = Morton code of point i's cell on the cubic level-0 grid g0.
+hipError_t morton_keys(const float* xyz, uint64_t n, const GridDesc* g0, uint32_t* keys,
+                       uint32_t* vals, hipStream_t st);
 // counts[cell of spts[j]] += 1 on grid g, one atomic per run of equal cells.
 hipError_t count_sorted(const float4* spts, uint64_t n, const GridDesc* g, uint32_t* counts,
                         hipStream_t st);
@@ -57,6 +63,11 @@ hipError_t count_cells(const float* xyz, uint64_t n, const GridDesc* g, uint32_t
 hipError_t partition_radius(const float4* qproc, uint32_t nq, const GridDesc* grid,
                             const uint32_t* counts, uint32_t k, float r, float* qradius,
                             uint32_t* iter_hist, int hist_bins, hipStream_t st);
+
+// Same rule on a hierarchical Morton grid (see partition.hip).
+hipError_t partition_hier(const float4* qproc, uint32_t nq, const GridDesc* levels, int nlevels,
+                          int level_min, const uint32_t* table, uint32_t k, float r,
+                          float* qradius, hipStream_t st);
 
 // ---- search.hip
 // Runs the top-k search.  group_lanes = 1 selects the thread-per-query kernel

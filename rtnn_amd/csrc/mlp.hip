@@ -96,8 +96,14 @@ __device__ inline float feature(const GridDesc& g, const uint32_t* __restrict__ 
     int x = qc.cx + f % 3 - 1, y = qc.cy + (f / 3) % 3 - 1, z = qc.cz + f / 9 - 1;
     if (x < 0 || y < 0 || z < 0 || x >= g.dim[0] || y >= g.dim[1] || z >= g.dim[2])
       return 0.0f;
-    uint32_t c = counts[((uint32_t)z * (uint32_t)g.dim[1] + (uint32_t)y) * (uint32_t)g.dim[0] +
-                        (uint32_t)x];
+    uint32_t c;
+    if (g.morton) {  // cell table of a Morton level: count = end - start
+      const uint32_t id = g.cell_base + morton3((uint32_t)x, (uint32_t)y, (uint32_t)z);
+      c = counts[id + 1] - counts[id];
+    } else {
+      c = counts[((uint32_t)z * (uint32_t)g.dim[1] + (uint32_t)y) * (uint32_t)g.dim[0] +
+                 (uint32_t)x];
+    }
     return __logf(1.0f + (float)c) * count_scale;
   }
   if (f == 27) return qc.ox;

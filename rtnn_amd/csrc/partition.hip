@@ -83,6 +83,75 @@ __global__ __launch_bounds__(256) void k_partition_radius(const float4* __restri
   if (iter_hist) atomicAdd(&iter_hist[min(it, hist_bins - 1)], 1u);
 }
 
+// The same megacell rule on a hierarchical Morton grid (levels l, cell
+// c_l = 2^l c_0, point counts from the level tables): at the finest level
+// where the query's own cell holds >= k+1 points the radius is sqrt(3) c_l
+// (every point of a cell containing the query is within its diagonal); where
+// the 3x3x3 block around it does, 2 sqrt(3) c_l.  Queries outside the grid box
+// and queries no level satisfies get r.
+__global__ __launch_bounds__(256) void k_partition_hier(const float4* __restrict__ qproc,
+                                                        uint32_t nq,
+                                                        const GridDesc* __restrict__ levels,
+                                                        int nlevels, int level_min,
+                                                        const uint32_t* __restrict__ table,
+                                                        uint32_t k, float r,
+                                                        float* __restrict__ qradius) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= nq) return;
+  const GridDesc g0 = levels[0];
+  const float4 q = qproc[i];
+  const float p[3] = {q.x, q.y, q.z};
+  int c[3];
+  bool inside = true;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float f = floorf((p[a] - g0.mn[a]) * g0.inv);
+    inside = inside && f >= 0.0f && f < (float)g0.dim[a];
+    c[a] = inside ? (int)f : 0;
+  }
+  float R = r;
+  if (inside) {
+    const float kSqrt3 = 1.7320508f * (1.0f + 1e-6f);
+    for (int l = level_min; l < nlevels; ++l) {
+      const float cell = ldexpf(g0.cell, l);
+      if (kSqrt3 * cell >= r) break;
+      const GridDesc& gl = levels[l];
+      const int x = c[0] >> l, y = c[1] >> l, z = c[2] >> l;
+      const uint32_t id = gl.cell_base + morton3((uint32_t)x, (uint32_t)y, (uint32_t)z);
+      if (table[id + 1] - table[id] >= k + 1) {
+        R = kSqrt3 * cell;
+        break;
+      }
+      if (2.0f * kSqrt3 * cell >= r) continue;  // the block test could not beat r
+      uint64_t cnt = 0;
+      for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+          for (int dx = -1; dx <= 1; ++dx) {
+            const int xx = x + dx, yy = y + dy, zz = z + dz;
+            if (xx < 0 || yy < 0 || zz < 0 || xx >= gl.dim[0] || yy >= gl.dim[1] ||
+                zz >= gl.dim[2])
+              continue;
+            const uint32_t j = gl.cell_base + morton3((uint32_t)xx, (uint32_t)yy, (uint32_t)zz);
+            cnt += table[j + 1] - table[j];
+          }
+      if (cnt >= (uint64_t)k + 1) {
+        R = 2.0f * kSqrt3 * cell;
+        break;
+      }
+    }
+  }
+  qradius[i] = fminf(R, r);
+}
+
+hipError_t partition_hier(const float4* qproc, uint32_t nq, const GridDesc* levels, int nlevels,
+                          int level_min, const uint32_t* table, uint32_t k, float r,
+                          float* qradius, hipStream_t st) {
+  if (nq == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_partition_hier, dim3((nq + 255) / 256), dim3(256), 0, st, qproc, nq,
+                     levels, nlevels, level_min, table, k, r, qradius);
+  return hipGetLastError();
+}
+
 hipError_t partition_radius(const float4* qproc, uint32_t nq, const GridDesc* grid,
                             const uint32_t* counts, uint32_t k, float r, float* qradius,
                             uint32_t* iter_hist, int hist_bins, hipStream_t st) {

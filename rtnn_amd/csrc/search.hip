@@ -49,8 +49,8 @@ __device__ inline void tile_of(const SearchArgs& a, uint32_t nq, uint32_t t, uin
     Tile tl = a.tiles[t];
     q0 = tl.q0; q1 = tl.q1; gi = tl.grid;
   } else {
-    q0 = t * (uint32_t)kTileQ;
-    q1 = min(q0 + (uint32_t)kTileQ, nq);
+    q0 = t * a.tile_q;
+    q1 = min(q0 + a.tile_q, nq);
     gi = 0;
   }
 }
@@ -65,7 +65,7 @@ __device__ inline uint32_t launch_nq(const SearchArgs& a) {
 struct TileIter {
   uint32_t t, end, step;
   __device__ TileIter(const SearchArgs& a, uint32_t nq) {
-    uint32_t ntiles = a.nq_dev ? (nq + kTileQ - 1) / kTileQ : a.ntiles;
+    uint32_t ntiles = a.nq_dev ? (nq + a.tile_q - 1) / a.tile_q : a.ntiles;
     if (a.persistent) {
       constexpr uint32_t kXcd = 8;
       uint32_t x = blockIdx.x % kXcd, j = blockIdx.x / kXcd;
@@ -97,7 +97,7 @@ __device__ inline void topk_insert(uint64_t (&list)[KC], uint64_t key) {
   list[0] = list[0] > key ? key : list[0];
 }
 
-template <int KC, bool KNN>
+template <int KC, bool KNN, bool HIER>
 __global__ __launch_bounds__(kTileQ) void k_topk_thread(const SearchArgs a) {
   const uint32_t nq = launch_nq(a);
   for (TileIter it(a, nq); it.t < it.end; it.t += it.step) {
@@ -115,60 +115,57 @@ __global__ __launch_bounds__(kTileQ) void k_topk_thread(const SearchArgs a) {
     cell_range(g, qv.x, R, 0, lo0, hi0);
     cell_range(g, qv.y, R, 1, lo1, hi1);
     cell_range(g, qv.z, R, 2, lo2, hi2);
-    const int ny = hi1 - lo1 + 1, nz = hi2 - lo2 + 1;
-    const int nrows = (active && hi0 >= lo0 && ny > 0 && nz > 0) ? ny * nz : 0;
+    uint32_t lvl_base = g.cell_base;
+    if (HIER) {
+      // Finest built level whose cell is >= R * alpha; the level-0 range
+      // shifted right by the level is that level's range (floor division is
+      // monotone and keeps empty ranges empty).
+      int lvl = a.level_min;
+      float c = ldexpf(g.cell, lvl);
+      const float want = R * a.level_alpha;
+      while (lvl < a.nlevels - 1 && c < want) {
+        c *= 2.0f;
+        ++lvl;
+      }
+      lo0 >>= lvl; hi0 >>= lvl; lo1 >>= lvl; hi1 >>= lvl; lo2 >>= lvl; hi2 >>= lvl;
+      lvl_base = a.grids[lvl].cell_base;
+    }
+    const int nx = hi0 - lo0 + 1, ny = hi1 - lo1 + 1, nz = hi2 - lo2 + 1;
+    // raster: one run per (y, z) row; hierarchical: one run per Morton cell
+    const int nrows = (active && nx > 0 && ny > 0 && nz > 0) ? (HIER ? nx : 1) * ny * nz : 0;
 
     uint64_t list[KC];
 #pragma unroll
     for (int i = 0; i < KC; ++i) list[i] = ~0ull;
-    // Hits below the current k-th wait in a 4-entry queue; the insertion
-    // network runs when some lane's queue is full, so its cost is paid once per
-    // four hits instead of on every candidate where any lane hits.
-    uint64_t pq0 = ~0ull, pq1 = ~0ull, pq2 = ~0ull, pq3 = ~0ull;
-    int pn = 0;
 
-    // The rows of the block are walked as one flattened candidate stream; the
-    // bounds of the next row are loaded one row ahead.
+    // One contiguous run of sorted points per raster row (or Morton cell).
     auto row_bounds = [&](int rr, uint32_t& s, uint32_t& e) {
-      const int cy = lo1 + rr % ny, cz = lo2 + rr / ny;
-      const uint32_t rowc =
-          g.cell_base + ((uint32_t)cz * (uint32_t)g.dim[1] + (uint32_t)cy) * (uint32_t)g.dim[0];
-      s = a.cell_start[rowc + lo0];
-      e = a.cell_start[rowc + hi0 + 1];
+      if (HIER) {
+        const int cx = lo0 + rr % nx, t = rr / nx;
+        const int cy = lo1 + t % ny, cz = lo2 + t / ny;
+        const uint32_t id = lvl_base + morton3((uint32_t)cx, (uint32_t)cy, (uint32_t)cz);
+        s = a.cell_start[id];
+        e = a.cell_start[id + 1];
+      } else {
+        const int cy = lo1 + rr % ny, cz = lo2 + rr / ny;
+        const uint32_t rowc = lvl_base + ((uint32_t)cz * (uint32_t)g.dim[1] + (uint32_t)cy) *
+                                             (uint32_t)g.dim[0];
+        s = a.cell_start[rowc + lo0];
+        e = a.cell_start[rowc + hi0 + 1];
+      }
     };
-    uint32_t s = 0, e = 0, ns = 0, ne = 0;
-    int rr = 0;  // row held in (ns, ne)
-    if (nrows > 0) row_bounds(0, ns, ne);
-    for (;;) {
-      if (s == e) {
-        if (rr >= nrows) break;
-        s = ns;
-        e = ne;
-        ++rr;
-        if (rr < nrows) row_bounds(rr, ns, ne);
-        continue;
-      }
-      const float4 p = a.spts[s++];
-      const float d2 = d2_of(qv.x, qv.y, qv.z, p.x, p.y, p.z);
-      const bool ok = KNN ? (d2 > 0.0f && d2 < R2) : (d2 < R2);
-      const uint64_t key = pack_key(d2, __float_as_uint(p.w));
-      if (ok && key < list[KC - 1]) {
-        pq3 = pq2; pq2 = pq1; pq1 = pq0; pq0 = key;
-        ++pn;
-      }
-      if (__any(pn == 4)) {
-#pragma unroll 1
-        for (int f = 0; f < 4; ++f) {
-          topk_insert<KC>(list, pq0);
-          pq0 = pq1; pq1 = pq2; pq2 = pq3; pq3 = ~0ull;
+    for (int rr = 0; rr < nrows; ++rr) {
+      uint32_t s, e;
+      row_bounds(rr, s, e);
+      for (uint32_t j = s; j < e; ++j) {
+        const float4 p = a.spts[j];
+        const float d2 = d2_of(qv.x, qv.y, qv.z, p.x, p.y, p.z);
+        const bool ok = KNN ? (d2 > 0.0f && d2 < R2) : (d2 < R2);
+        if (ok) {
+          const uint64_t key = pack_key(d2, __float_as_uint(p.w));
+          if (key < list[KC - 1]) topk_insert<KC>(list, key);
         }
-        pn = 0;
       }
-    }
-#pragma unroll 1
-    for (int f = 0; f < 4; ++f) {
-      topk_insert<KC>(list, pq0);
-      pq0 = pq1; pq1 = pq2; pq2 = pq3; pq3 = ~0ull;
     }
     if (!active) continue;
 
@@ -231,7 +228,7 @@ __device__ inline void rank_entries(const uint64_t* buf, uint32_t cnt, int l,
   }
 }
 
-template <int G, int KB, bool KNN>
+template <int G, int KB, bool KNN, bool HIER>
 __global__ __launch_bounds__(kTileQ) void k_topk_group(const SearchArgs a) {
   using C = GroupCfg<G, KB>;
   __shared__ uint64_t s_buf[C::NG][C::CAP];
@@ -262,8 +259,21 @@ __global__ __launch_bounds__(kTileQ) void k_topk_group(const SearchArgs a) {
       cell_range(g, qv.x, R, 0, lo0, hi0);
       cell_range(g, qv.y, R, 1, lo1, hi1);
       cell_range(g, qv.z, R, 2, lo2, hi2);
-      const int ny = hi1 - lo1 + 1, nz = hi2 - lo2 + 1;
-      const int nrows = (hi0 >= lo0 && ny > 0 && nz > 0) ? ny * nz : 0;
+      uint32_t lvl_base = g.cell_base;
+      if (HIER) {  // same level rule as k_topk_thread (group-uniform)
+        int lvl = a.level_min;
+        float c = ldexpf(g.cell, lvl);
+        const float want = R * a.level_alpha;
+        while (lvl < a.nlevels - 1 && c < want) {
+          c *= 2.0f;
+          ++lvl;
+        }
+        lo0 >>= lvl; hi0 >>= lvl; lo1 >>= lvl; hi1 >>= lvl; lo2 >>= lvl; hi2 >>= lvl;
+        lvl_base = a.grids[lvl].cell_base;
+      }
+      const int nx = hi0 - lo0 + 1, ny = hi1 - lo1 + 1, nz = hi2 - lo2 + 1;
+      // raster: one run per (y, z) row; hierarchical: one run per Morton cell
+      const int nrows = (nx > 0 && ny > 0 && nz > 0) ? (HIER ? nx : 1) * ny * nz : 0;
 
       uint32_t cnt = 0;
       uint64_t thr = ~0ull;
@@ -271,12 +281,19 @@ __global__ __launch_bounds__(kTileQ) void k_topk_group(const SearchArgs a) {
         const int j = rb + l;
         uint32_t s0 = 0, len = 0;
         if (j < nrows) {
-          const int cy = lo1 + j % ny, cz = lo2 + j / ny;
-          const uint32_t rowc = g.cell_base +
-                                ((uint32_t)cz * (uint32_t)g.dim[1] + (uint32_t)cy) *
-                                    (uint32_t)g.dim[0];
-          s0 = a.cell_start[rowc + lo0];
-          len = a.cell_start[rowc + hi0 + 1] - s0;
+          if (HIER) {
+            const int cx = lo0 + j % nx, t = j / nx;
+            const int cy = lo1 + t % ny, cz = lo2 + t / ny;
+            const uint32_t id = lvl_base + morton3((uint32_t)cx, (uint32_t)cy, (uint32_t)cz);
+            s0 = a.cell_start[id];
+            len = a.cell_start[id + 1] - s0;
+          } else {
+            const int cy = lo1 + j % ny, cz = lo2 + j / ny;
+            const uint32_t rowc = lvl_base + ((uint32_t)cz * (uint32_t)g.dim[1] + (uint32_t)cy) *
+                                                 (uint32_t)g.dim[0];
+            s0 = a.cell_start[rowc + lo0];
+            len = a.cell_start[rowc + hi0 + 1] - s0;
+          }
         }
         const uint32_t incl = group_incl_scan<G>(len, l);
         const uint32_t total = __shfl(incl, G - 1, G);
@@ -348,18 +365,28 @@ __global__ __launch_bounds__(kTileQ) void k_topk_group(const SearchArgs a) {
 
 template <int KC>
 static void launch_thread(const SearchArgs& a, bool knn, dim3 grid, hipStream_t st) {
-  if (knn)
-    hipLaunchKernelGGL((k_topk_thread<KC, true>), grid, dim3(kTileQ), 0, st, a);
+  const bool hier = a.nlevels > 0;
+  if (knn && hier)
+    hipLaunchKernelGGL((k_topk_thread<KC, true, true>), grid, dim3(kTileQ), 0, st, a);
+  else if (knn)
+    hipLaunchKernelGGL((k_topk_thread<KC, true, false>), grid, dim3(kTileQ), 0, st, a);
+  else if (hier)
+    hipLaunchKernelGGL((k_topk_thread<KC, false, true>), grid, dim3(kTileQ), 0, st, a);
   else
-    hipLaunchKernelGGL((k_topk_thread<KC, false>), grid, dim3(kTileQ), 0, st, a);
+    hipLaunchKernelGGL((k_topk_thread<KC, false, false>), grid, dim3(kTileQ), 0, st, a);
 }
 
 template <int G, int KB>
 static void launch_group(const SearchArgs& a, bool knn, dim3 grid, hipStream_t st) {
-  if (knn)
-    hipLaunchKernelGGL((k_topk_group<G, KB, true>), grid, dim3(kTileQ), 0, st, a);
+  const bool hier = a.nlevels > 0;
+  if (knn && hier)
+    hipLaunchKernelGGL((k_topk_group<G, KB, true, true>), grid, dim3(kTileQ), 0, st, a);
+  else if (knn)
+    hipLaunchKernelGGL((k_topk_group<G, KB, true, false>), grid, dim3(kTileQ), 0, st, a);
+  else if (hier)
+    hipLaunchKernelGGL((k_topk_group<G, KB, false, true>), grid, dim3(kTileQ), 0, st, a);
   else
-    hipLaunchKernelGGL((k_topk_group<G, KB, false>), grid, dim3(kTileQ), 0, st, a);
+    hipLaunchKernelGGL((k_topk_group<G, KB, false, false>), grid, dim3(kTileQ), 0, st, a);
 }
 
 uint32_t max_k_for_group(int group_lanes) {
@@ -375,10 +402,17 @@ uint32_t max_k_for_group(int group_lanes) {
 hipError_t launch_search(const SearchArgs& a0, int knn_mode, int group_lanes, hipStream_t st,
                          int num_cus) {
   SearchArgs a = a0;
-  if (a.ntiles == 0) return hipSuccess;
   if (a.nq_dev) a.persistent = 1;  // tile count known only on the device
   const uint32_t k = a.k;
   if (k == 0 || k > max_k_for_group(group_lanes)) return hipErrorInvalidValue;
+  if (!a.tiles) {
+    // Implicit tiles: a block of the thread kernel takes kTileQ queries, a
+    // block of the group kernel one query per group, so small workloads still
+    // spread over many blocks.
+    a.tile_q = group_lanes == 1 ? (uint32_t)kTileQ : (uint32_t)(kTileQ / group_lanes);
+    a.ntiles = (uint32_t)(((uint64_t)a.nq + a.tile_q - 1) / a.tile_q);
+  }
+  if (a.ntiles == 0) return hipSuccess;
   uint32_t blocks = a.ntiles;
   if (a.persistent) {
     uint32_t cap = (uint32_t)(num_cus > 0 ? num_cus : 256) * 8u;

@@ -134,3 +134,40 @@ def test_partitioned_equals_plain(ctx, k):
     u = rng.uniform(0, 30, (40000, 3)).astype(np.float32)
     ctx.set_points(u)
     _plain_vs(ctx, u[:5000], 2.0, k, ctx.search_partitioned)
+
+
+@pytest.mark.parametrize("qpw", [0, 16, 4])
+@pytest.mark.parametrize("levels,alpha", [(1, 0.0), (2, 1.0), (3, 0.5), (5, 1.0), (5, 2.0)])
+def test_hierarchical_grid_exact(ctx, levels, alpha, qpw):
+    # Partitioned and learned KNN on the hierarchical Morton grid (one sorted
+    # point array, per-level tables, per-query level) equal the plain KNN, for
+    # the thread-per-query kernel and the 4- and 16-lane group kernels.
+    pts = rtnn_amd.gen_gaussian_mixture(2, 20, 100000)
+    qs = np.concatenate([rtnn_amd.gen_gaussian_mixture(2, 21, 30000),
+                         np.array([[-50, 0, 0], [1e4, 1e4, 1e4]], np.float32)])
+    ctx.set_points(pts)
+    ctx.set_options(grid_levels=levels, level_alpha=alpha, queries_per_wave=qpw)
+    try:
+        for k in (1, 8, 32):
+            _plain_vs(ctx, qs, 20.0, k, ctx.search_partitioned)
+        ctx.set_mlp_weights(rand_blob(11))
+        _plain_vs(ctx, qs, 20.0, 8, ctx.search_learned)
+    finally:
+        ctx.set_options()
+
+
+def test_autotune_picks_a_valid_config(ctx):
+    rng = np.random.default_rng(21)
+    pts = rng.uniform(0, 40, (50000, 3)).astype(np.float32)
+    ctx.set_points(pts)
+    try:
+        opt, ms = ctx.autotune(None, method="plain", mode="radius", radius=2.0, k=32)
+        assert ms > 0 and opt["cell_factor"] > 0
+        idx, d2, _ = ctx.search(None, mode="radius", radius=2.0, k=32)
+        import oracle
+        wi, wd = oracle.grid_search(pts, pts, oracle.MODE_RADIUS, 2.0, 32, threads=8)
+        assert np.array_equal(idx, wi) and np.array_equal(d2, wd)
+        opt, ms = ctx.autotune(pts[:5000], method="partitioned", mode="knn", radius=3.0, k=8)
+        assert ms > 0
+    finally:
+        ctx.set_options()
