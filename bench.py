@@ -66,12 +66,16 @@ def workload(name):
     if name == "c4":
         xyz, offs = rtnn_amd.gen_lattice_clouds(4, 4096, 1000, 200000, h=0.1, jit=0.02)
         return dict(pts=xyz, qs=None, offsets=offs, mode="radius", r=0.2, k=64,
-                    methods=["percloud", "batched"],
+                    methods=["percloud", "batched", "graph"],
                     cfg={"workload": "batched range r=0.2 K=64: 4096 jittered cubic lattices "
                                      "(h=0.1, jitter 0.02h), sizes log-uniform in [1e3, 2e5], "
                                      "queries = points",
                          "clouds": 4096, "points": int(offs[-1]), "k": 64, "radius": 0.2})
     raise SystemExit(f"unknown workload {name}")
+
+
+# CPU baseline sample per workload: about 10-30 s of 16-thread brute force.
+CPU_SAMPLE = {"c1": 100_000, "c2": 200_000, "c3": 15_000, "c4": 1_000_000, "c5": 6_000}
 
 
 def c27_bytes_clouds(xyz, offs, r, k, sample, seed=5):
@@ -140,7 +144,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="c3")
-    ap.add_argument("--cpu-sample", type=int, default=4000)
+    ap.add_argument("--cpu-sample", type=int, default=None,
+                    help="queries brute-forced on the CPU (default: about 10-30 s of work)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--train-samples", type=int, default=1 << 21)
     ap.add_argument("--train-steps", type=int, default=2000)
@@ -169,6 +174,8 @@ def main():
         return
 
     w = workload(args.workload)
+    if args.cpu_sample is None:
+        args.cpu_sample = CPU_SAMPLE.get(args.workload, 4000)
     pts, qs = w["pts"], w["qs"]
     k, r, mode = w["k"], w["r"], w["mode"]
     nq = len(qs) if qs is not None else len(pts)
@@ -193,6 +200,9 @@ def main():
         elif m == "batched":
             fn = lambda: ctx.search_batched(dp, w["offsets"], mode=mode, radius=r, k=k,
                                             with_d2=False)
+        elif m == "graph":  # the batched pipeline captured once, replayed per step
+            graph = ctx.graph_batched(dp, w["offsets"], mode=mode, radius=r, k=k, with_d2=False)
+            fn = lambda: (graph.idx, graph.d2, graph.launch())
         else:
             t0 = time.perf_counter()
             loss = ctx.mlp_train(r, k, n_samples=args.train_samples, steps=args.train_steps,
@@ -226,8 +236,10 @@ def main():
         log(f"[bench] {m} identical to {w['methods'][0]} on all {nq} rows: {same}")
 
     head = w["methods"][-1]
-    # roofline of the dominant search kernel of the headline method
-    kms = results[head]["ms_search"]
+    # roofline of the dominant search kernel of the headline method (a graph
+    # replay reports only its total: its search kernel is the batched one)
+    kroof = "batched" if head == "graph" else head
+    kms = results[kroof]["ms_search"]
     c27_note = None
     if mode == "knn":
         algo = nq * (12 + 12 * len(pts) / nq + 8 * k)
@@ -240,7 +252,7 @@ def main():
         algo = c27_bytes(pts, pts if qs is None else qs, r, k)
         algo_def = "range bytes per query 12 + 12*C27(q) + 4*K"
     achieved = algo / (kms * 1e-3) / 1e9
-    kname = "k_topk_thread" if results[head]["lanes_per_query"] == 1 else "k_topk_group"
+    kname = "k_topk_thread" if results[kroof]["lanes_per_query"] == 1 else "k_topk_group"
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": kname,
             "kernel_ms": round(kms, 4), "algorithmic_bytes": algo, "bytes_definition": algo_def}
@@ -297,6 +309,9 @@ def main():
                                  "fallback",
                   "batched": "per-cloud bbox + one grid build for all clouds + one persistent "
                              "search launch",
+                  "graph": "one HIP graph replay: per-cloud bbox + device-side grid plan + "
+                           "cell keys + radix sort + cell table + gather + one persistent "
+                           "search launch",
                   "plain": "grid build + query sort + search"}
     cfg = dict(w["cfg"], method=head, step=steps_desc.get(head, head))
     out = {

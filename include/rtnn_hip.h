@@ -144,6 +144,15 @@ int rtnn_search_learned(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_dev
                         int32_t* out_idx, float* out_d2, int out_on_device,
                         rtnn_stats* stats);
 
+/* The first-stage launch radius of each query (input order) under method
+ * RTNN_METHOD_PARTITIONED (the megacell rule, sort.cpp:366-371 with approx
+ * mode 0) or RTNN_METHOD_LEARNED (the MLP's r'), capped at r: what
+ * rtnn_search_partitioned / rtnn_search_learned search first.  For measuring
+ * a proposal (search volume, share of queries it already covers). */
+int rtnn_propose_radius(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_device,
+                        int method, float r, uint32_t k, float* out_radius,
+                        int out_on_device);
+
 /* ---- batched clouds --------------------------------------------------- */
 
 /* nclouds independent clouds packed back to back in xyz; cloud c holds points
@@ -159,6 +168,50 @@ int rtnn_search_percloud(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets
                          uint32_t nclouds, int mode, float r, uint32_t k,
                          int32_t* out_idx, float* out_d2, rtnn_stats* stats);
 
+/* ---- graph-captured batched pipeline ------------------------------------
+ * The batched sort -> grid -> search pipeline captured once in a HIP graph
+ * and replayed with one launch (no host round trip: the per-cloud grids are
+ * planned on the device).  The reference re-runs its host-driven pipeline per
+ * call (src/optixNSearch/main.cpp:91-148); this is the replayable form of
+ * rtnn_search_batched for callers that search the same batch layout
+ * repeatedly.  xyz, out_idx and out_d2 (nullable) are device pointers bound
+ * at capture: a replay reads the current contents of xyz (any values, same
+ * offsets) and rewrites the outputs, equal to rtnn_search_batched on that
+ * data.  The cell table is sized from the data at capture (+25 %); a replay
+ * on data that needs more cells uses a coarser edge (still exact).  Replays
+ * run on the context's stream; one graph per context at a time may run.
+ * Destroy graphs before their context. */
+typedef struct rtnn_graph_s* rtnn_graph;
+int rtnn_graph_batched(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets,
+                       uint32_t nclouds, int mode, float r, uint32_t k,
+                       int32_t* out_idx, float* out_d2, rtnn_graph* out);
+/* Replays the graph.  sync != 0 (or stats != NULL) waits and reports
+ * non-finite input coordinates as RTNN_ERR_INVALID; stats->ms_total is the
+ * replay's device time. */
+int rtnn_graph_launch(rtnn_graph g, int sync, rtnn_stats* stats);
+int rtnn_graph_destroy(rtnn_graph g);
+
+/* ---- boundary-pair report ------------------------------------------------
+ * The reference's inclusion test is d2 < r*r in FP32 (src/optixNSearch/
+ * geometry.cu:61-66,173-187); a pair whose true distance is within a few ulp
+ * of r can flip under another FP32 evaluation order (check.cpp:80-83 admits
+ * such mismatches).  These calls list every (query row, point index) pair with
+ * |d - r| <= tol * r, d evaluated in FP64 from the FP32 coordinates, so such
+ * pairs are reported rather than silently dropped.  Output: pairs[2 * i] =
+ * row, pairs[2 * i + 1] = point, ordered by (row, point); at most `capacity`
+ * pairs are written (out_on_device selects a device or host pointer) and
+ * *count receives the total (call with capacity 0 to size the buffer).
+ * tol in [0, 1e-2]; 1e-6 is the documented default. */
+int rtnn_boundary_pairs(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_device,
+                        float r, double tol, uint32_t* out_pairs, uint64_t capacity,
+                        int out_on_device, uint64_t* count);
+/* Batched form: queries = points of each cloud, point indices cloud-local,
+ * rows global (as rtnn_search_batched). */
+int rtnn_boundary_pairs_batched(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets,
+                                uint32_t nclouds, float r, double tol,
+                                uint32_t* out_pairs, uint64_t capacity,
+                                int out_on_device, uint64_t* count);
+
 /* ---- autotuner --------------------------------------------------------- */
 
 #define RTNN_METHOD_PLAIN       0
@@ -170,7 +223,8 @@ int rtnn_search_percloud(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets
  * options (also returned in *best; *best_ms = its time per search).  Replaces
  * the reference's analytic cell-size and batching models (calcCRRatio,
  * util.cpp:551-676; autoBatchingRange/KNN, sort.cpp:221-320) with
- * measurement.  out_idx/out_d2 are scratch outputs of size nq x k (device). */
+ * measurement.  out_idx/out_d2 are scratch outputs of size nq x k (device;
+ * NULL: the context's own scratch). */
 int rtnn_autotune(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_device, int method,
                   int mode, float r, uint32_t k, int32_t* out_idx, float* out_d2,
                   rtnn_options* best, double* best_ms);

@@ -95,10 +95,19 @@ SIGNATURES = {
     "rtnn_mlp_set_weights": (_I, [_VP, _VP]),
     "rtnn_search_learned": (_I, [_VP, _VP, _U64, _I, _F, _U32, _VP, _VP, _I,
                                  ctypes.POINTER(Stats)]),
+    "rtnn_propose_radius": (_I, [_VP, _VP, _U64, _I, _I, _F, _U32, _VP, _I]),
     "rtnn_search_batched": (_I, [_VP, _VP, _VP, _U32, _I, _F, _U32, _VP, _VP,
                                  ctypes.POINTER(Stats)]),
     "rtnn_search_percloud": (_I, [_VP, _VP, _VP, _U32, _I, _F, _U32, _VP, _VP,
                                   ctypes.POINTER(Stats)]),
+    "rtnn_graph_batched": (_I, [_VP, _VP, _VP, _U32, _I, _F, _U32, _VP, _VP,
+                                ctypes.POINTER(_VP)]),
+    "rtnn_graph_launch": (_I, [_VP, _I, ctypes.POINTER(Stats)]),
+    "rtnn_graph_destroy": (_I, [_VP]),
+    "rtnn_boundary_pairs": (_I, [_VP, _VP, _U64, _I, _F, ctypes.c_double, _VP, _U64, _I,
+                                 ctypes.POINTER(_U64)]),
+    "rtnn_boundary_pairs_batched": (_I, [_VP, _VP, _VP, _U32, _F, ctypes.c_double, _VP, _U64,
+                                         _I, ctypes.POINTER(_U64)]),
     "rtnn_autotune": (_I, [_VP, _VP, _U64, _I, _I, _I, _F, _U32, _VP, _VP,
                            ctypes.POINTER(Options), ctypes.POINTER(ctypes.c_double)]),
     "rtnn_radix_sort_pairs": (_I, [_VP, _VP, _VP, _U64, _U32]),
@@ -266,6 +275,20 @@ class Context:
     def search_learned(self, queries=None, radius=10.0, k=16, with_d2=True, out_device=None):
         return self._run(self._L.rtnn_search_learned, queries, radius, k, with_d2, out_device)
 
+    def propose_radius(self, queries=None, method="learned", radius=10.0, k=16):
+        """First-stage launch radius per query (input order) -> float32 array."""
+        meth = {"partitioned": 1, "learned": 2}[method]
+        if queries is None:
+            nq, qptr, qdev, keep = len(self._points), None, 0, None
+        else:
+            keep = _as_points(queries)
+            qptr, qdev = _ptr(keep)
+            nq = len(keep)
+        out = np.empty(nq, dtype=np.float32)
+        self._check(self._L.rtnn_propose_radius(self._h, qptr, nq, qdev, meth, radius, k,
+                                                ctypes.c_void_p(out.ctypes.data), 0))
+        return out
+
     def mlp_train(self, radius, k, n_samples=1 << 21, steps=2000, batch=4096, lr=1e-3,
                   tau=0.9, seed=1):
         loss = ctypes.c_float(0)
@@ -308,6 +331,60 @@ class Context:
     def search_percloud(self, xyz, offsets, mode="radius", radius=0.2, k=64, with_d2=True):
         return self._batched(self._L.rtnn_search_percloud, xyz, offsets, mode, radius, k,
                              with_d2)
+
+    def graph_batched(self, xyz, offsets, mode="radius", radius=0.2, k=64, with_d2=True):
+        """Captures the batched pipeline in a HIP graph -> BatchedGraph (its
+        .idx / .d2 tensors are rewritten by every .launch())."""
+        import torch
+        if not (_is_torch(xyz) and xyz.is_cuda and xyz.dtype == torch.float32
+                and xyz.is_contiguous()):
+            raise ValueError("graph_batched takes a contiguous float32 CUDA tensor")
+        offs = np.ascontiguousarray(offsets, dtype=np.uint64)
+        n = int(offs[-1])
+        idx = torch.empty((n, k), dtype=torch.int32, device=xyz.device)
+        d2 = torch.empty((n, k), dtype=torch.float32, device=xyz.device) if with_d2 else None
+        h = ctypes.c_void_p()
+        self._check(self._L.rtnn_graph_batched(
+            self._h, ctypes.c_void_p(xyz.data_ptr()), ctypes.c_void_p(offs.ctypes.data),
+            len(offs) - 1, _mode(mode), radius, k, ctypes.c_void_p(idx.data_ptr()),
+            ctypes.c_void_p(d2.data_ptr()) if d2 is not None else None, ctypes.byref(h)))
+        return BatchedGraph(self, h, xyz, idx, d2)
+
+    def boundary_pairs(self, queries=None, radius=2.0, tol=1e-6):
+        """Pairs (query row, point index) with |d - r| <= tol * r, d in FP64,
+        ordered by (row, point) -> uint32 array [count, 2] (host)."""
+        if queries is None:
+            nq, qptr, qdev, keep = len(self._points), None, 0, None
+        else:
+            keep = _as_points(queries)
+            qptr, qdev = _ptr(keep)
+            nq = len(keep)
+        cnt = _U64(0)
+        self._check(self._L.rtnn_boundary_pairs(self._h, qptr, nq, qdev, radius, tol, None, 0, 0,
+                                                ctypes.byref(cnt)))
+        out = np.zeros((cnt.value, 2), dtype=np.uint32)
+        if cnt.value:
+            self._check(self._L.rtnn_boundary_pairs(self._h, qptr, nq, qdev, radius, tol,
+                                                    ctypes.c_void_p(out.ctypes.data),
+                                                    cnt.value, 0, ctypes.byref(cnt)))
+        return out
+
+    def boundary_pairs_batched(self, xyz, offsets, radius=0.2, tol=1e-6):
+        """Batched form: rows global, point indices cloud-local."""
+        import torch
+        if not (_is_torch(xyz) and xyz.is_cuda):
+            raise ValueError("batched boundary report takes a CUDA tensor of points")
+        xyz = xyz.to(torch.float32).contiguous()
+        offs = np.ascontiguousarray(offsets, dtype=np.uint64)
+        cnt = _U64(0)
+        args = (self._h, ctypes.c_void_p(xyz.data_ptr()), ctypes.c_void_p(offs.ctypes.data),
+                len(offs) - 1, radius, tol)
+        self._check(self._L.rtnn_boundary_pairs_batched(*args, None, 0, 0, ctypes.byref(cnt)))
+        out = torch.zeros((cnt.value, 2), dtype=torch.int32, device=xyz.device)
+        if cnt.value:
+            self._check(self._L.rtnn_boundary_pairs_batched(
+                *args, ctypes.c_void_p(out.data_ptr()), cnt.value, 1, ctypes.byref(cnt)))
+        return out.cpu().numpy().view(np.uint32)
 
     def autotune(self, queries=None, method="plain", mode="knn", radius=10.0, k=16):
         """Times the candidate configurations on this workload and keeps the
@@ -429,6 +506,37 @@ def gen_spheres(dist_seed, sample_seed, n, nspheres=2000, box=2000.0, rmin=5.0, 
 
 def gen_levy(seed, n, alpha=1.5, min_step=0.01, box=1000.0):
     return _gen(lib().rtnn_gen_levy, n, seed, n, alpha, min_step, box)
+
+
+class BatchedGraph:
+    """A captured batched pipeline; holds the point and output tensors it is
+    bound to.  launch() replays it (reading xyz's current contents)."""
+
+    def __init__(self, ctx, handle, xyz, idx, d2):
+        self._ctx, self._h, self.xyz, self.idx, self.d2 = ctx, handle, xyz, idx, d2
+
+    def launch(self, sync=True):
+        st = Stats()
+        self._ctx._check(self._ctx._L.rtnn_graph_launch(self._h, 1 if sync else 0,
+                                                        ctypes.byref(st) if sync else None))
+        return st
+
+    def close(self):
+        if self._h:
+            self._ctx._L.rtnn_graph_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def gen_lattice_clouds(seed, nclouds, nmin=1000, nmax=200000, h=0.1, jit=0.02):

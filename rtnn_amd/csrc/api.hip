@@ -13,6 +13,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -90,11 +91,15 @@ struct rtnn_ctx_s {
   DevBuf tr_q, tr_idx, tr_d2, tr_t, tr_f, tr_out;
   MlpState mlp;
 
+  // boundary-pair report
+  DevBuf bcnt, boff, bpairs;
+  uint32_t batch_tiles = 0;  // tiles of the last batched grid
+
   ~rtnn_ctx_s() {
     DevBuf* all[] = {&pts, &keys, &vals, &counts, &cell_start, &spts, &tmp, &grids,
                      &qin, &qkeys, &qvals, &qsorted, &oidx, &od2, &shortrows, &misc,
                      &tiles, &offs, &bboxes, &pgrid, &pcounts, &qradius, &tr_q, &tr_idx,
-                     &tr_d2, &tr_t, &tr_f, &tr_out};
+                     &tr_d2, &tr_t, &tr_f, &tr_out, &bcnt, &boff, &bpairs};
     for (DevBuf* b : all) b->release();
     mlp_release(mlp);
     for (auto& e : ev)
@@ -148,7 +153,10 @@ void plan_grid(const float* bb, float cell, GridDesc& g) {
   }
 }
 
+// Cell count, saturated at 1e15 (dims up to 2e9 per axis would wrap 64 bits).
 uint64_t grid_cells(const GridDesc& g) {
+  const double c = (double)g.dim[0] * (double)g.dim[1] * (double)g.dim[2];
+  if (c > 1e15) return (uint64_t)1e15;
   return (uint64_t)g.dim[0] * (uint64_t)g.dim[1] * (uint64_t)g.dim[2];
 }
 
@@ -604,30 +612,17 @@ int build_count_grid(rtnn_ctx ctx, float cell, bool grow) {
 
 enum class Proposal { kLearned, kPartition };
 
-// KNN with a proposed radius r' <= r per query, then an exact fallback: the
-// rows with fewer than k hits within r' are searched again at r.  If a row
-// has >= k hits with d2 < r'^2 <= r^2, any other point has d2 >= r'^2, so
-// those k are the k nearest within r.
-int proposal_search(rtnn_ctx ctx, Proposal kind, const float* qxyz, uint64_t nq, int on_device,
-                    float r, uint32_t k, int32_t* out_idx, float* out_d2, int out_on_device,
-                    rtnn_stats* stats) {
-  int rc = check_search_args(ctx, RTNN_MODE_KNN, r, k, out_idx);
-  if (rc) return rc;
-  if (ctx->n == 0) return fail(ctx, RTNN_ERR_STATE, "rtnn_set_points has not been called");
-  if (kind == Proposal::kLearned && !ctx->mlp.ready)
-    return fail(ctx, RTNN_ERR_STATE, "the radius MLP has no weights (train or load it)");
-  if (k > 64) return fail(ctx, RTNN_ERR_UNSUPPORTED, "proposal search supports k <= 64");
-  if (!qxyz) nq = ctx->n;
-  if (nq == 0) return RTNN_OK;
-  if (nq >= 0xffffffffull) return fail(ctx, RTNN_ERR_UNSUPPORTED, "more than 2^32-2 queries");
-  CK(hipSetDevice(ctx->device));
-  CK(hipEventRecord(ctx->ev[0], ctx->st));
+// Grid for the proposal method (hierarchical Morton grid or one raster grid),
+// queries in processing order (*qproc) and their launch radius in
+// ctx->qradius; ctx->ev[1] is recorded after the grid and query sort.
+int compute_proposals(rtnn_ctx ctx, Proposal kind, const float* qxyz, uint64_t nq, int on_device,
+                      float r, uint32_t k, const float4** qproc_out) {
   const float f = ctx->opt.cell_factor > 0 ? ctx->opt.cell_factor
                                            : (kind == Proposal::kLearned ? 2.0f : 4.0f);
   const int levels = ctx->opt.grid_levels > 0 ? ctx->opt.grid_levels
                                               : (kind == Proposal::kLearned ? 1 : 5);
   const bool hier = levels >= 2;
-  rc = hier ? build_hgrid(ctx, r, levels) : build_grid(ctx, r / f);
+  int rc = hier ? build_hgrid(ctx, r, levels) : build_grid(ctx, r / f);
   if (rc) return rc;
   const float4* qproc = nullptr;
   rc = prepare_queries(ctx, qxyz, nq, on_device, &qproc);
@@ -661,6 +656,31 @@ int proposal_search(rtnn_ctx ctx, Proposal kind, const float* qxyz, uint64_t nq,
                         ctx->pcounts.as<uint32_t>(), k, r, ctx->qradius.as<float>(), nullptr, 0,
                         ctx->st));
   }
+  *qproc_out = qproc;
+  return RTNN_OK;
+}
+
+// KNN with a proposed radius r' <= r per query, then an exact fallback: the
+// rows with fewer than k hits within r' are searched again at r.  If a row
+// has >= k hits with d2 < r'^2 <= r^2, any other point has d2 >= r'^2, so
+// those k are the k nearest within r.
+int proposal_search(rtnn_ctx ctx, Proposal kind, const float* qxyz, uint64_t nq, int on_device,
+                    float r, uint32_t k, int32_t* out_idx, float* out_d2, int out_on_device,
+                    rtnn_stats* stats) {
+  int rc = check_search_args(ctx, RTNN_MODE_KNN, r, k, out_idx);
+  if (rc) return rc;
+  if (ctx->n == 0) return fail(ctx, RTNN_ERR_STATE, "rtnn_set_points has not been called");
+  if (kind == Proposal::kLearned && !ctx->mlp.ready)
+    return fail(ctx, RTNN_ERR_STATE, "the radius MLP has no weights (train or load it)");
+  if (k > 64) return fail(ctx, RTNN_ERR_UNSUPPORTED, "proposal search supports k <= 64");
+  if (!qxyz) nq = ctx->n;
+  if (nq == 0) return RTNN_OK;
+  if (nq >= 0xffffffffull) return fail(ctx, RTNN_ERR_UNSUPPORTED, "more than 2^32-2 queries");
+  CK(hipSetDevice(ctx->device));
+  CK(hipEventRecord(ctx->ev[0], ctx->st));
+  const float4* qproc = nullptr;
+  rc = compute_proposals(ctx, kind, qxyz, nq, on_device, r, k, &qproc);
+  if (rc) return rc;
   CK(hipEventRecord(ctx->ev[2], ctx->st));
   OutBufs o;
   rc = output_targets(ctx, nq, k, out_idx, out_d2, out_on_device, o);
@@ -732,6 +752,36 @@ int rtnn_search_learned(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_dev
                         rtnn_stats* stats) {
   return proposal_search(ctx, Proposal::kLearned, qxyz, nq, on_device, r, k, out_idx, out_d2,
                          out_on_device, stats);
+}
+
+int rtnn_propose_radius(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_device, int method,
+                        float r, uint32_t k, float* out_radius, int out_on_device) {
+  if (!ctx) return RTNN_ERR_INVALID;
+  if (method != RTNN_METHOD_PARTITIONED && method != RTNN_METHOD_LEARNED)
+    return fail(ctx, RTNN_ERR_INVALID, "method must be partitioned or learned");
+  if (!(r > 0.0f) || !std::isfinite(r) || k == 0 || !out_radius)
+    return fail(ctx, RTNN_ERR_INVALID, "invalid radius, k or output");
+  if (ctx->n == 0) return fail(ctx, RTNN_ERR_STATE, "rtnn_set_points has not been called");
+  const Proposal kind = method == RTNN_METHOD_LEARNED ? Proposal::kLearned : Proposal::kPartition;
+  if (kind == Proposal::kLearned && !ctx->mlp.ready)
+    return fail(ctx, RTNN_ERR_STATE, "the radius MLP has no weights (train or load it)");
+  if (!qxyz) nq = ctx->n;
+  if (nq == 0) return RTNN_OK;
+  if (nq >= 0xffffffffull) return fail(ctx, RTNN_ERR_UNSUPPORTED, "more than 2^32-2 queries");
+  CK(hipSetDevice(ctx->device));
+  const float4* qproc = nullptr;
+  int rc = compute_proposals(ctx, kind, qxyz, nq, on_device, r, k, &qproc);
+  if (rc) return rc;
+  float* dst = out_radius;
+  if (!out_on_device) {
+    CK(ctx->od2.ensure(4 * nq));
+    dst = ctx->od2.as<float>();
+  }
+  CK(scatter_rows(qproc, ctx->qradius.as<float>(), nq, r, dst, ctx->st));
+  if (!out_on_device)
+    CK(hipMemcpyAsync(out_radius, dst, 4 * nq, hipMemcpyDeviceToHost, ctx->st));
+  CK(hipStreamSynchronize(ctx->st));
+  return RTNN_OK;
 }
 
 // ------------------------------------------------------------------ MLP --
@@ -861,28 +911,30 @@ int rtnn_mlp_features(rtnn_ctx ctx, const float* qxyz, uint64_t nq, float r, flo
   return RTNN_OK;
 }
 
+}  // extern "C"
+
 // ------------------------------------------------------------- batched ----
 
-int rtnn_search_batched(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets,
-                        uint32_t nclouds, int mode, float r, uint32_t k, int32_t* out_idx,
-                        float* out_d2, rtnn_stats* stats) {
-  int rc = check_search_args(ctx, mode, r, k, out_idx);
-  if (rc) return rc;
+namespace {
+
+int check_offsets(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets, uint32_t nclouds) {
   if (!xyz || !offsets || nclouds == 0)
     return fail(ctx, RTNN_ERR_INVALID, "xyz/offsets must be set and nclouds >= 1");
-  const uint64_t n = offsets[nclouds];
   if (offsets[0] != 0) return fail(ctx, RTNN_ERR_INVALID, "offsets[0] must be 0");
   for (uint32_t c = 0; c < nclouds; ++c)
-    if (offsets[c + 1] < offsets[c]) return fail(ctx, RTNN_ERR_INVALID, "offsets must be non-decreasing");
-  if (n == 0) return RTNN_OK;
-  if (n >= 0xffffffffull) return fail(ctx, RTNN_ERR_UNSUPPORTED, "more than 2^32-2 points");
-  const int gl = group_lanes_from_opt(ctx, mode, k);
-  if (gl < 0 || k > max_k_for_group(gl))
-    return fail(ctx, RTNN_ERR_UNSUPPORTED, "k=%u not supported by the selected kernel", k);
-  CK(hipSetDevice(ctx->device));
-  const bool timing = ctx->opt.time_stages != 0 || stats != nullptr;
-  if (timing) CK(hipEventRecord(ctx->ev[0], ctx->st));
+    if (offsets[c + 1] < offsets[c])
+      return fail(ctx, RTNN_ERR_INVALID, "offsets must be non-decreasing");
+  if (offsets[nclouds] >= 0xffffffffull)
+    return fail(ctx, RTNN_ERR_UNSUPPORTED, "more than 2^32-2 points");
+  return RTNN_OK;
+}
 
+// Builds one grid per cloud (edge `cell`, grown until the total cell count
+// fits the budget) over the concatenated clouds, the sorted points and the
+// query tiles (runs of <= kTileQ points of one cloud); ctx->batch_tiles is set.
+int build_batched(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets, uint32_t nclouds,
+                  float cell) {
+  const uint64_t n = offsets[nclouds];
   // Per-cloud bbox on the device, plan on the host.
   CK(ctx->offs.ensure(8 * (nclouds + 1)));
   CK(hipMemcpyAsync(ctx->offs.p, offsets, 8 * (nclouds + 1), hipMemcpyHostToDevice, ctx->st));
@@ -899,8 +951,6 @@ int rtnn_search_batched(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets,
   std::memcpy(&nbad, &hb[6 * nclouds], 4);
   if (nbad) return fail(ctx, RTNN_ERR_INVALID, "%u non-finite point coordinates", nbad);
 
-  float f = ctx->opt.cell_factor > 0 ? ctx->opt.cell_factor : default_cell_factor(mode);
-  float cell = r / f;
   std::vector<GridDesc> gd(nclouds);
   uint64_t total = 0;
   const uint64_t cap = max_cells_for(ctx, n);
@@ -950,14 +1000,87 @@ int rtnn_search_batched(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets,
   CK(ctx->tiles.ensure(sizeof(Tile) * tl.size()));
   CK(hipMemcpyAsync(ctx->tiles.p, tl.data(), sizeof(Tile) * tl.size(), hipMemcpyHostToDevice,
                     ctx->st));
+  ctx->batch_tiles = (uint32_t)tl.size();
+  return RTNN_OK;
+}
+
+// SearchArgs over the batched grid: queries = the sorted points, one tile
+// list, cloud-local indices, persistent launch.
+SearchArgs batched_args(rtnn_ctx ctx, uint64_t n, float r, uint32_t k, const OutBufs& o) {
+  SearchArgs a = base_args(ctx, ctx->spts.as<float4>(), n, r, k, o);
+  a.tiles = ctx->tiles.as<Tile>();
+  a.ntiles = ctx->batch_tiles;
+  a.local_index = 1;
+  a.persistent = 1;
+  return a;
+}
+
+// Boundary-pair report over the grid and queries described by `a` (rows =
+// output rows): count pass, scan, total to the host, write pass.
+int boundary_run(rtnn_ctx ctx, const SearchArgs& a, uint64_t rows, double tol,
+                 uint32_t* out_pairs, uint64_t capacity, int out_on_device, uint64_t* count) {
+  CK(ctx->bcnt.ensure(4 * (rows + 1)));
+  CK(ctx->boff.ensure(4 * (rows + 1)));
+  CK(ctx->tmp.ensure(4 * (uint64_t)scan_tmp_words(rows + 1)));
+  uint32_t* cnt = ctx->bcnt.as<uint32_t>();
+  uint32_t* off = ctx->boff.as<uint32_t>();
+  CK(fill_u32(cnt, rows + 1, 0, ctx->st));
+  CK(launch_boundary(a, tol, cnt, nullptr, 0, nullptr, ctx->st, ctx->num_cus));
+  CK(exclusive_scan(cnt, off, rows + 1, ctx->tmp.as<uint32_t>(), ctx->st));
+  uint32_t total = 0;
+  CK(hipMemcpyAsync(&total, off + rows, 4, hipMemcpyDeviceToHost, ctx->st));
+  CK(hipStreamSynchronize(ctx->st));
+  *count = total;
+  const uint64_t w = std::min<uint64_t>(total, capacity);
+  if (w == 0) return RTNN_OK;
+  uint32_t* dst = out_pairs;
+  if (!out_on_device) {
+    CK(ctx->bpairs.ensure(8 * w));
+    dst = ctx->bpairs.as<uint32_t>();
+  }
+  CK(launch_boundary(a, tol, cnt, off, w, dst, ctx->st, ctx->num_cus));
+  if (!out_on_device)
+    CK(hipMemcpyAsync(out_pairs, dst, 8 * w, hipMemcpyDeviceToHost, ctx->st));
+  CK(hipStreamSynchronize(ctx->st));
+  return RTNN_OK;
+}
+
+int check_boundary_args(rtnn_ctx ctx, float r, double tol, const void* out_pairs,
+                        uint64_t capacity, const uint64_t* count) {
+  if (!ctx) return RTNN_ERR_INVALID;
+  if (!(r > 0.0f) || !std::isfinite(r)) return fail(ctx, RTNN_ERR_INVALID, "radius must be > 0");
+  if (!(tol >= 0.0 && tol <= 1e-2)) return fail(ctx, RTNN_ERR_INVALID, "tol must be in [0, 1e-2]");
+  if (!count) return fail(ctx, RTNN_ERR_INVALID, "count is NULL");
+  if (capacity > 0 && !out_pairs) return fail(ctx, RTNN_ERR_INVALID, "out_pairs is NULL");
+  return RTNN_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rtnn_search_batched(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets,
+                        uint32_t nclouds, int mode, float r, uint32_t k, int32_t* out_idx,
+                        float* out_d2, rtnn_stats* stats) {
+  int rc = check_search_args(ctx, mode, r, k, out_idx);
+  if (rc) return rc;
+  rc = check_offsets(ctx, xyz, offsets, nclouds);
+  if (rc) return rc;
+  const uint64_t n = offsets[nclouds];
+  if (n == 0) return RTNN_OK;
+  const int gl = group_lanes_from_opt(ctx, mode, k);
+  if (gl < 0 || k > max_k_for_group(gl))
+    return fail(ctx, RTNN_ERR_UNSUPPORTED, "k=%u not supported by the selected kernel", k);
+  CK(hipSetDevice(ctx->device));
+  const bool timing = ctx->opt.time_stages != 0 || stats != nullptr;
+  if (timing) CK(hipEventRecord(ctx->ev[0], ctx->st));
+  const float f = ctx->opt.cell_factor > 0 ? ctx->opt.cell_factor : default_cell_factor(mode);
+  rc = build_batched(ctx, xyz, offsets, nclouds, r / f);
+  if (rc) return rc;
   if (timing) CK(hipEventRecord(ctx->ev[1], ctx->st));
 
   OutBufs o{out_idx, out_d2};
-  SearchArgs a = base_args(ctx, ctx->spts.as<float4>(), n, r, k, o);
-  a.tiles = ctx->tiles.as<Tile>();
-  a.ntiles = (uint32_t)tl.size();
-  a.local_index = 1;
-  a.persistent = 1;
+  SearchArgs a = batched_args(ctx, n, r, k, o);
   CK(launch_search(a, mode == RTNN_MODE_KNN, gl, ctx->st, ctx->num_cus));
   if (timing) CK(hipEventRecord(ctx->ev[2], ctx->st));
   CK(hipStreamSynchronize(ctx->st));
@@ -1003,14 +1126,63 @@ int rtnn_search_percloud(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets
   return RTNN_OK;
 }
 
+// ------------------------------------------------------ boundary pairs --
+
+int rtnn_boundary_pairs(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_device, float r,
+                        double tol, uint32_t* out_pairs, uint64_t capacity, int out_on_device,
+                        uint64_t* count) {
+  int rc = check_boundary_args(ctx, r, tol, out_pairs, capacity, count);
+  if (rc) return rc;
+  *count = 0;
+  if (ctx->n == 0) return fail(ctx, RTNN_ERR_STATE, "rtnn_set_points has not been called");
+  if (!qxyz) nq = ctx->n;
+  if (nq == 0) return RTNN_OK;
+  if (nq >= 0xffffffffull) return fail(ctx, RTNN_ERR_UNSUPPORTED, "more than 2^32-2 queries");
+  CK(hipSetDevice(ctx->device));
+  rc = build_grid(ctx, r);
+  if (rc) return rc;
+  const float4* qproc = nullptr;
+  rc = prepare_queries(ctx, qxyz, nq, on_device, &qproc);
+  if (rc) return rc;
+  OutBufs o{nullptr, nullptr};
+  SearchArgs a = base_args(ctx, qproc, nq, r, 1, o);
+  return boundary_run(ctx, a, nq, tol, out_pairs, capacity, out_on_device, count);
+}
+
+int rtnn_boundary_pairs_batched(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets,
+                                uint32_t nclouds, float r, double tol, uint32_t* out_pairs,
+                                uint64_t capacity, int out_on_device, uint64_t* count) {
+  int rc = check_boundary_args(ctx, r, tol, out_pairs, capacity, count);
+  if (rc) return rc;
+  *count = 0;
+  rc = check_offsets(ctx, xyz, offsets, nclouds);
+  if (rc) return rc;
+  const uint64_t n = offsets[nclouds];
+  if (n == 0) return RTNN_OK;
+  CK(hipSetDevice(ctx->device));
+  rc = build_batched(ctx, xyz, offsets, nclouds, r);
+  if (rc) return rc;
+  OutBufs o{nullptr, nullptr};
+  SearchArgs a = batched_args(ctx, n, r, 1, o);
+  return boundary_run(ctx, a, n, tol, out_pairs, capacity, out_on_device, count);
+}
+
 // ------------------------------------------------------------ autotune --
 
 int rtnn_autotune(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_device, int method,
                   int mode, float r, uint32_t k, int32_t* out_idx, float* out_d2,
                   rtnn_options* best, double* best_ms) {
-  if (!ctx || !out_idx) return RTNN_ERR_INVALID;
+  if (!ctx) return RTNN_ERR_INVALID;
   if (method < RTNN_METHOD_PLAIN || method > RTNN_METHOD_LEARNED)
     return fail(ctx, RTNN_ERR_INVALID, "unknown method %d", method);
+  if (!out_idx) {  // scratch outputs owned by the context
+    const uint64_t rows = qxyz ? nq : ctx->n;
+    CK(hipSetDevice(ctx->device));
+    CK(ctx->oidx.ensure(4 * rows * std::max<uint32_t>(k, 1)));
+    CK(ctx->od2.ensure(4 * rows * std::max<uint32_t>(k, 1)));
+    out_idx = ctx->oidx.as<int32_t>();
+    out_d2 = ctx->od2.as<float>();
+  }
   const rtnn_options base = ctx->opt;
   static const float kPlainCf[] = {1.0f, 2.0f, 3.0f};
   static const float kPropCf[] = {1.5f, 2.0f, 2.5f, 3.0f};
@@ -1100,6 +1272,171 @@ int rtnn_morton3(rtnn_ctx ctx, const uint32_t* xyz, uint32_t* codes, uint64_t n)
   CK(hipSetDevice(ctx->device));
   CK(morton3_codes(xyz, n, codes, ctx->st));
   CK(hipStreamSynchronize(ctx->st));
+  return RTNN_OK;
+}
+
+}  // extern "C"
+
+// ================================================ graph-captured batches ==
+
+struct rtnn_graph_s {
+  rtnn_ctx ctx = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  // Buffers owned by the graph, so calls on the context between replays
+  // cannot move memory the graph's nodes point at.
+  DevBuf offs, bboxes, status, grids, keys, vals, cell_start, spts, tmp, tiles;
+  uint64_t n = 0;
+  uint32_t nclouds = 0, ntiles = 0;
+  uint64_t cap_cells = 0;
+  ~rtnn_graph_s() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    DevBuf* all[] = {&offs, &bboxes, &status, &grids, &keys, &vals, &cell_start, &spts, &tmp,
+                     &tiles};
+    for (DevBuf* b : all) b->release();
+  }
+};
+
+namespace {
+
+// The whole batched pipeline on ctx->st with the graph's buffers: status
+// clear -> per-cloud bbox -> device-side grid plan -> cell keys -> radix sort
+// -> cell table -> gather -> one persistent search.  Kernels only: no host
+// synchronisation, allocation, memset or host copy, so it can be captured.
+hipError_t enqueue_batched(rtnn_graph g, const float* xyz, float cell, int mode, int gl, float r,
+                           uint32_t k, int32_t* out_idx, float* out_d2) {
+  hipStream_t st = g->ctx->st;
+  uint32_t* status = g->status.as<uint32_t>();
+  const uint64_t* off = g->offs.as<uint64_t>();
+  uint32_t* keys = g->keys.as<uint32_t>();
+  uint32_t* vals = g->vals.as<uint32_t>();
+  hipError_t e;
+  if ((e = fill_u32(status, 4, 0, st)) != hipSuccess) return e;
+  if ((e = bbox_segmented(xyz, off, g->nclouds, g->bboxes.as<float>(), status, st)) != hipSuccess)
+    return e;
+  if ((e = plan_batched(g->bboxes.as<float>(), off, g->nclouds, cell, g->cap_cells,
+                        g->grids.as<GridDesc>(), status, st)) != hipSuccess)
+    return e;
+  if ((e = cell_keys(xyz, g->n, g->grids.as<GridDesc>(), off, g->nclouds, keys, vals, nullptr,
+                     st)) != hipSuccess)
+    return e;
+  if ((e = radix_sort_pairs(keys, vals, g->n, bits_for(g->cap_cells - 1), g->tmp.as<uint32_t>(),
+                            st, nullptr)) != hipSuccess)
+    return e;
+  if ((e = cell_bounds(keys, g->n, (uint32_t)g->cap_cells, g->cell_start.as<uint32_t>(),
+                       g->tmp.as<uint32_t>(), st)) != hipSuccess)
+    return e;
+  if ((e = gather_points(xyz, vals, g->n, g->spts.as<float4>(), st)) != hipSuccess) return e;
+  SearchArgs a;
+  std::memset(&a, 0, sizeof a);
+  a.spts = g->spts.as<float4>();
+  a.cell_start = g->cell_start.as<uint32_t>();
+  a.grids = g->grids.as<GridDesc>();
+  a.queries = g->spts.as<float4>();
+  a.tiles = g->tiles.as<Tile>();
+  a.ntiles = g->ntiles;
+  a.nq = (uint32_t)g->n;
+  a.r = r;
+  a.qradius_scale = 1.0f;
+  a.k = k;
+  a.out_idx = out_idx;
+  a.out_d2 = out_d2;
+  a.local_index = 1;
+  a.persistent = 1;
+  return launch_search(a, mode == RTNN_MODE_KNN, gl, st, g->ctx->num_cus);
+}
+
+}  // namespace
+
+extern "C" {
+
+int rtnn_graph_batched(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets,
+                       uint32_t nclouds, int mode, float r, uint32_t k, int32_t* out_idx,
+                       float* out_d2, rtnn_graph* out) {
+  if (!ctx) return RTNN_ERR_INVALID;
+  if (!out) return fail(ctx, RTNN_ERR_INVALID, "out is NULL");
+  *out = nullptr;
+  int rc = check_search_args(ctx, mode, r, k, out_idx);
+  if (rc) return rc;
+  rc = check_offsets(ctx, xyz, offsets, nclouds);
+  if (rc) return rc;
+  const uint64_t n = offsets[nclouds];
+  if (n == 0) return fail(ctx, RTNN_ERR_INVALID, "the batch holds no points");
+  const int gl = group_lanes_from_opt(ctx, mode, k);
+  if (gl < 0 || k > max_k_for_group(gl))
+    return fail(ctx, RTNN_ERR_UNSUPPORTED, "k=%u not supported by the selected kernel", k);
+  CK(hipSetDevice(ctx->device));
+  const float f = ctx->opt.cell_factor > 0 ? ctx->opt.cell_factor : default_cell_factor(mode);
+  const float cell = r / f;
+  // One host-planned build sizes the cell table and the tiles; replays plan
+  // on the device within that table (growing the edge if new data needs it).
+  rc = build_batched(ctx, xyz, offsets, nclouds, cell);
+  if (rc) return rc;
+  std::unique_ptr<rtnn_graph_s> g(new rtnn_graph_s);
+  g->ctx = ctx;
+  g->n = n;
+  g->nclouds = nclouds;
+  g->ntiles = ctx->batch_tiles;
+  g->cap_cells = std::min<uint64_t>(max_cells_for(ctx, n), ctx->ncells + ctx->ncells / 4 + 4096);
+  g->cap_cells = std::max<uint64_t>(g->cap_cells, ctx->ncells);
+  CK(g->offs.ensure(8 * (nclouds + 1)));
+  CK(g->bboxes.ensure(24 * (uint64_t)nclouds));
+  CK(g->status.ensure(16));
+  CK(g->grids.ensure(sizeof(GridDesc) * nclouds));
+  CK(g->keys.ensure(4 * n));
+  CK(g->vals.ensure(4 * n));
+  CK(g->cell_start.ensure(4 * (g->cap_cells + 1)));
+  CK(g->spts.ensure(16 * n));
+  CK(g->tmp.ensure(4 * std::max(radix_tmp_words(n), cell_bounds_tmp_words(g->cap_cells))));
+  CK(g->tiles.ensure(sizeof(Tile) * g->ntiles));
+  CK(hipMemcpyAsync(g->offs.p, offsets, 8 * (nclouds + 1), hipMemcpyHostToDevice, ctx->st));
+  CK(hipMemcpyAsync(g->tiles.p, ctx->tiles.p, sizeof(Tile) * g->ntiles, hipMemcpyDeviceToDevice,
+                    ctx->st));
+  // Eager run first: loads every kernel and checks the inputs.
+  CK(enqueue_batched(g.get(), xyz, cell, mode, gl, r, k, out_idx, out_d2));
+  uint32_t hs[4];
+  CK(hipMemcpyAsync(hs, g->status.p, 16, hipMemcpyDeviceToHost, ctx->st));
+  CK(hipStreamSynchronize(ctx->st));
+  if (hs[0]) return fail(ctx, RTNN_ERR_INVALID, "%u non-finite point coordinates", hs[0]);
+  CK(hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
+  const hipError_t e = enqueue_batched(g.get(), xyz, cell, mode, gl, r, k, out_idx, out_d2);
+  hipGraph_t graph = nullptr;
+  const hipError_t e2 = hipStreamEndCapture(ctx->st, &graph);
+  g->graph = graph;
+  if (e != hipSuccess) return hip_fail(ctx, e, "capturing the batched pipeline");
+  CK(e2);
+  CK(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+  *out = g.release();
+  return RTNN_OK;
+}
+
+int rtnn_graph_launch(rtnn_graph g, int sync, rtnn_stats* stats) {
+  if (!g) return RTNN_ERR_INVALID;
+  rtnn_ctx ctx = g->ctx;
+  CK(hipSetDevice(ctx->device));
+  if (stats) CK(hipEventRecord(ctx->ev[6], ctx->st));
+  CK(hipGraphLaunch(g->exec, ctx->st));
+  if (stats) CK(hipEventRecord(ctx->ev[7], ctx->st));
+  if (!sync && !stats) return RTNN_OK;
+  uint32_t hs[4];
+  CK(hipMemcpyAsync(hs, g->status.p, 16, hipMemcpyDeviceToHost, ctx->st));
+  CK(hipStreamSynchronize(ctx->st));
+  if (stats) {
+    std::memset(stats, 0, sizeof *stats);
+    stats->ms_total = ev_ms(ctx->ev[6], ctx->ev[7]);
+    stats->search_launches = 1;
+  }
+  if (hs[0])
+    return fail(ctx, RTNN_ERR_INVALID, "%u non-finite point coordinates (rows undefined)", hs[0]);
+  return RTNN_OK;
+}
+
+int rtnn_graph_destroy(rtnn_graph g) {
+  if (!g) return RTNN_OK;
+  (void)hipSetDevice(g->ctx->device);
+  (void)hipStreamSynchronize(g->ctx->st);
+  delete g;
   return RTNN_OK;
 }
 

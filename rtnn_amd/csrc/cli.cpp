@@ -10,7 +10,11 @@
 //   reproduced: -a takes 0/1/2, and --crStep has no trailing space.
 // Added: -o <file> writes the result (format below), --learned 0|1 selects the
 // learned-radius KNN, --mlp-in/--mlp-out load/save its weights, --cell-factor
-// and --qpw set the grid cell factor and queries per wave.
+// and --qpw set the grid cell factor and queries per wave, --boundary <file>
+// lists the pairs whose distance lies within 1e-6 r of r.  Of the reference's
+// scheduling flags, -cr sets the cell factor (cell edge = r / crRatio, as
+// sort.cpp:80) and -ac 1 runs the autotuner (measured instead of modelled,
+// util.cpp:551-676); the others are accepted and do not change results.
 //
 // Output file (little endian): "RTNNRES1", u64 nq, u32 k, u32 flags (bit 0:
 // d2 present), then int32 idx[nq][k] (empty slot = -1), then float d2[nq][k].
@@ -42,7 +46,10 @@ struct Args {
   bool check = false, partition = true, learned = false;
   int approx = 2;
   float cell_factor = 0.0f;
+  float cr_ratio = 0.0f;  // -cr: cell edge = r / crRatio (sort.cpp:80)
+  bool autotune = false;  // -ac 1: measure the grid configurations first
   int qpw = 0;
+  std::string boundary;  // --boundary: boundary-pair report file
 };
 
 struct Timer {
@@ -81,7 +88,9 @@ struct Timer {
                "  -o <file>          Write results (RTNNRES1 binary).\n"
                "  --learned 0|1      Learned-radius KNN (trains the MLP unless --mlp-in).\n"
                "  --mlp-in <file>    Load MLP weights;  --mlp-out <file>  save them.\n"
-               "  --cell-factor <f>  Grid cell edge = r / f;  --qpw <n>  queries per wave.\n",
+               "  --cell-factor <f>  Grid cell edge = r / f;  --qpw <n>  queries per wave.\n"
+               "  --boundary <file>  Write pairs with |d - r| <= 1e-6 r (query point d).\n"
+               "  (-cr sets the cell factor, -ac 1 runs the autotuner first.)\n",
                argv0, RTNN_KNN);
   std::exit(code);
 }
@@ -120,13 +129,18 @@ Args parse(int argc, char** argv) {
       }
     } else if (s == "--msr" || s == "-m" || s == "--deferFree" || s == "-df" ||
                s == "--filterQueries" || s == "-fq" || s == "--numbatch" || s == "-nb" ||
-               s == "--autobatch" || s == "-ab" || s == "--autocrratio" || s == "-ac" ||
+               s == "--autobatch" || s == "-ab" ||
                s == "--interleave" || s == "-i" || s == "--pointsort" || s == "-ps" ||
-               s == "--querysort" || s == "-qs" || s == "--crratio" || s == "-cr" ||
+               s == "--querysort" || s == "-qs" ||
                s == "--gpumemused" || s == "-gmu" || s == "--gather" || s == "-g" ||
                s == "--metacellScale" || s == "-mc") {
       need(i);  // scheduling knobs of the BVH pipeline: no effect on results here
-    } else if (s == "-o" || s == "--out") a.out = need(i);
+    } else if (s == "--crratio" || s == "-cr") {
+      a.cr_ratio = std::stof(need(i));
+      if (!(a.cr_ratio > 0)) usage(argv[0], 1);
+    } else if (s == "--autocrratio" || s == "-ac") a.autotune = std::atoi(need(i)) != 0;
+    else if (s == "--boundary") a.boundary = need(i);
+    else if (s == "-o" || s == "--out") a.out = need(i);
     else if (s == "--learned") a.learned = std::atoi(need(i)) != 0;
     else if (s == "--mlp-in") a.mlp_in = need(i);
     else if (s == "--mlp-out") a.mlp_out = need(i);
@@ -253,7 +267,7 @@ int main(int argc, char** argv) {
   rtnn_ctx ctx = nullptr;
   check_rc(nullptr, rtnn_create(a.device, &ctx), "rtnn_create");
   rtnn_options opt{};
-  opt.cell_factor = a.cell_factor;
+  opt.cell_factor = a.cell_factor > 0 ? a.cell_factor : a.cr_ratio;
   opt.queries_per_wave = a.qpw;
   check_rc(ctx, rtnn_set_options(ctx, &opt), "rtnn_set_options");
   {
@@ -287,6 +301,21 @@ int main(int argc, char** argv) {
       std::fwrite(blob.data(), 4, blob.size(), f);
       std::fclose(f);
     }
+  }
+  const int method = knn && a.learned ? RTNN_METHOD_LEARNED
+                     : knn && a.partition ? RTNN_METHOD_PARTITIONED : RTNN_METHOD_PLAIN;
+  if (a.autotune) {
+    Timer t;
+    rtnn_options best{};
+    double ms = 0;
+    check_rc(ctx, rtnn_autotune(ctx, samepq ? nullptr : Q.data(), nq, 0, method,
+                                knn ? RTNN_MODE_KNN : RTNN_MODE_RADIUS, a.radius, k, nullptr,
+                                nullptr, &best, &ms),
+             "rtnn_autotune");
+    t.stop("autotune");
+    std::printf("autotune: cell_factor %g, queries_per_wave %d, grid_levels %d, level_alpha %g"
+                " (%g ms)\n", best.cell_factor, best.queries_per_wave, best.grid_levels,
+                best.level_alpha, ms);
   }
   std::vector<int32_t> idx(nq * (uint64_t)k);
   std::vector<float> d2(a.out.empty() ? 0 : nq * (uint64_t)k);
@@ -326,6 +355,30 @@ int main(int argc, char** argv) {
     std::fwrite(idx.data(), 4, idx.size(), f);
     std::fwrite(d2.data(), 4, d2.size(), f);
     std::fclose(f);
+  }
+  if (!a.boundary.empty()) {
+    // One "query point distance" line per pair with |d - r| <= 1e-6 r.
+    uint64_t cnt = 0;
+    check_rc(ctx, rtnn_boundary_pairs(ctx, samepq ? nullptr : Q.data(), nq, 0, a.radius, 1e-6,
+                                      nullptr, 0, 0, &cnt), "rtnn_boundary_pairs");
+    std::vector<uint32_t> pr(2 * cnt);
+    if (cnt)
+      check_rc(ctx, rtnn_boundary_pairs(ctx, samepq ? nullptr : Q.data(), nq, 0, a.radius, 1e-6,
+                                        pr.data(), cnt, 0, &cnt), "rtnn_boundary_pairs");
+    FILE* f = std::fopen(a.boundary.c_str(), "w");
+    if (!f) {
+      std::fprintf(stderr, "cannot write %s\n", a.boundary.c_str());
+      return 1;
+    }
+    for (uint64_t i = 0; i < cnt; ++i) {
+      const float* q = &Q[3 * (uint64_t)pr[2 * i]];
+      const float* p = &P[3 * (uint64_t)pr[2 * i + 1]];
+      const double dx = (double)q[0] - p[0], dy = (double)q[1] - p[1], dz = (double)q[2] - p[2];
+      std::fprintf(f, "%u %u %.9g\n", pr[2 * i], pr[2 * i + 1],
+                   std::sqrt(dx * dx + dy * dy + dz * dz));
+    }
+    std::fclose(f);
+    std::printf("boundary pairs (|d - r| <= 1e-6 r): %llu\n", (unsigned long long)cnt);
   }
   int code = 0;
   if (a.check && !sanity_check(a, P, Q, idx, k)) code = 1;

@@ -5,9 +5,14 @@
 // cell ids, atomicAdd local indices (non-deterministic within a cell),
 // Thrust scan and sort.  Here:
 //   bbox        float min/max, block reduction (no atomics on 6 addresses);
-//   cell_keys   raster cell id per point (x fastest) + per-cell count;
-//   scan        exclusive scan of the counts -> cell_start (sort.hip);
+//   cell_keys   raster cell id per point (x fastest), or the level-0 Morton
+//               code of a hierarchical grid (morton_keys);
 //   radix sort  stable sort of (cell, index) -> points ordered by (cell, index);
+//   cell_bounds cell_start[c] = first sorted position with key >= c: one
+//               binary search per 4096-cell tile, then each tile's cells
+//               searched inside its own key range staged in LDS, so the work
+//               per thread is bounded however many cells are empty; a Morton
+//               level l uses key >> 3l and the same sorted keys;
 //   gather      sorted points as float4 (x, y, z, bits(original index)) so one
 //               16-byte load yields a candidate and its output index.
 // Query order: Morton code of the query's cell (helper_mortonCode.h:19-34
@@ -120,6 +125,112 @@ hipError_t bbox_segmented(const float* xyz, const uint64_t* offsets, uint32_t nc
   if (nclouds == 0) return hipSuccess;
   hipLaunchKernelGGL(k_bbox_seg, dim3(nclouds), dim3(kBboxThreads), 0, st, xyz, offsets, out,
                      nonfinite);
+  return hipGetLastError();
+}
+
+// ---- device-side planning of the batched grids (no host round trip, so the
+// whole batched pipeline can be captured in a graph).
+
+__device__ inline uint64_t cells_sat(int d0, int d1, int d2) {
+  const double c = (double)d0 * (double)d1 * (double)d2;
+  return c > 1e15 ? (uint64_t)1e15 : (uint64_t)d0 * (uint64_t)d1 * (uint64_t)d2;
+}
+
+// Dims of one cloud's grid; the host planner's arithmetic (-ffp-contract=off).
+__device__ inline void plan_dims(const float* bb, float inv, bool empty, int d[3]) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float e = (bb[a + 3] - bb[a]) * inv;
+    const float f = floorf(e) + 1.0f;
+    d[a] = empty ? 1 : (int)fminf(fmaxf(f, 1.0f), 2.0e9f);  // NaN, -inf -> 1
+  }
+}
+
+constexpr int kPlanThreads = 1024;
+
+__device__ inline uint64_t block_sum_u64(uint64_t v, uint64_t* sh) {
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  __syncthreads();
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  uint64_t t = 0;
+  for (int i = 0; i < kPlanThreads / kWave; ++i) t += sh[i];
+  return t;
+}
+
+// One block.  Grows the edge by 1.25x until the total cell count fits cap
+// (the host planner's loop), then writes every cloud's GridDesc with its
+// cell_base.  If no edge fits within 200 steps, every cloud gets a single
+// cell (inv = 0): slow but exact, and status[1] is set.
+__global__ __launch_bounds__(kPlanThreads) void k_plan_batched(
+    const float* __restrict__ bboxes, const uint64_t* __restrict__ off, uint32_t nclouds,
+    float cell, uint64_t cap, GridDesc* __restrict__ grids, uint32_t* __restrict__ status) {
+  __shared__ uint64_t sh[kPlanThreads / kWave];
+  __shared__ uint64_t s_scan[kPlanThreads];
+  float c = cell;
+  bool ok = false;
+  for (int tries = 0; tries <= 200 && !ok; ++tries) {
+    const float inv = 1.0f / c;
+    uint64_t tot = 0;
+    for (uint32_t i = threadIdx.x; i < nclouds; i += kPlanThreads) {
+      int d[3];
+      plan_dims(bboxes + 6 * i, inv, off[i + 1] == off[i], d);
+      tot = min(tot + cells_sat(d[0], d[1], d[2]), (uint64_t)1e15);  // no wrap
+    }
+    tot = block_sum_u64(tot, sh);
+    ok = tot <= cap && tot < 0xfffffff0ull;
+    if (!ok) c *= 1.25f;
+  }
+  const float inv = ok ? 1.0f / c : 0.0f;
+  if (!ok && threadIdx.x == 0) status[1] = 1u;
+  uint64_t carry = 0;
+  for (uint32_t i0 = 0; i0 < nclouds; i0 += kPlanThreads) {
+    const uint32_t i = i0 + threadIdx.x;
+    int d[3] = {1, 1, 1};
+    bool empty = true;
+    if (i < nclouds) {
+      empty = off[i + 1] == off[i];
+      if (ok) plan_dims(bboxes + 6 * i, inv, empty, d);
+    }
+    const uint64_t mine = i < nclouds ? cells_sat(d[0], d[1], d[2]) : 0;
+    // inclusive scan of this chunk (Hillis-Steele in LDS)
+    __syncthreads();
+    s_scan[threadIdx.x] = mine;
+    __syncthreads();
+    for (int o = 1; o < kPlanThreads; o <<= 1) {
+      const uint64_t t = threadIdx.x >= (uint32_t)o ? s_scan[threadIdx.x - o] : 0;
+      __syncthreads();
+      s_scan[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (i < nclouds) {
+      GridDesc g;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        g.mn[a] = empty ? 0.0f : bboxes[6 * i + a];
+        g.dim[a] = d[a];
+      }
+      g.inv = inv;
+      g.cell = ok ? c : FLT_MAX;
+      g.cell_base = (uint32_t)(carry + s_scan[threadIdx.x] - mine);
+      g.point_base = (uint32_t)off[i];
+      g.npoints = (uint32_t)(off[i + 1] - off[i]);
+      g.morton = 0;
+      grids[i] = g;
+    }
+    carry += s_scan[kPlanThreads - 1];
+  }
+  if (threadIdx.x == 0) status[2] = (uint32_t)carry;
+}
+
+hipError_t plan_batched(const float* bboxes, const uint64_t* offsets, uint32_t nclouds,
+                        float cell, uint64_t cap, GridDesc* grids, uint32_t* status,
+                        hipStream_t st) {
+  if (nclouds == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_plan_batched, dim3(1), dim3(kPlanThreads), 0, st, bboxes, offsets,
+                     nclouds, cell, cap, grids, status);
   return hipGetLastError();
 }
 
@@ -412,6 +523,21 @@ hipError_t fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t st) {
 hipError_t iota_u32(uint32_t* p, uint64_t n, hipStream_t st) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_iota, fill_grid(n), dim3(256), 0, st, p, n);
+  return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_scatter_rows(const float4* __restrict__ qproc,
+                                                      const float* __restrict__ val, uint64_t n,
+                                                      float r, float* __restrict__ out) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * 256)
+    out[__float_as_uint(qproc[i].w)] = fminf(val[i], r);
+}
+
+hipError_t scatter_rows(const float4* qproc, const float* val, uint64_t n, float r, float* out,
+                        hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_scatter_rows, fill_grid(n), dim3(256), 0, st, qproc, val, n, r, out);
   return hipGetLastError();
 }
 

@@ -26,6 +26,12 @@ hipError_t bbox(const float* xyz, uint64_t n, float* part, float* out, uint32_t*
 // Per-cloud bbox: out[6 * c .. 6 * c + 6) for cloud c (offsets: nclouds + 1).
 hipError_t bbox_segmented(const float* xyz, const uint64_t* offsets, uint32_t nclouds,
                           float* out, uint32_t* nonfinite, hipStream_t st);
+// Per-cloud grids planned on the device from the per-cloud bboxes: edge
+// `cell`, grown by 1.25x until the total cell count is <= cap (one cell per
+// cloud if none fits, status[1] = 1); status[2] = total cells.
+hipError_t plan_batched(const float* bboxes, const uint64_t* offsets, uint32_t nclouds,
+                        float cell, uint64_t cap, GridDesc* grids, uint32_t* status,
+                        hipStream_t st);
 // keys[i] = grid cell of point i, vals[i] = i, counts[key] += 1 (counts nullable).
 hipError_t cell_keys(const float* xyz, uint64_t n, const GridDesc* grids,
                      const uint64_t* offsets, uint32_t nclouds, uint32_t* keys,
@@ -51,6 +57,9 @@ hipError_t query_morton_keys(const float* q, uint64_t nq, const GridDesc* g, int
                              uint32_t* keys, uint32_t* vals, hipStream_t st);
 hipError_t morton3_codes(const uint32_t* xyz, uint64_t n, uint32_t* codes, hipStream_t st);
 hipError_t fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t st);
+// out[row of qproc[i]] = min(val[i], r): per-query values back in input order.
+hipError_t scatter_rows(const float4* qproc, const float* val, uint64_t n, float r, float* out,
+                        hipStream_t st);
 hipError_t iota_u32(uint32_t* p, uint64_t n, hipStream_t st);
 // Coarse count grid: counts[cell of point] += 1 (grid g, single cloud).
 hipError_t count_cells(const float* xyz, uint64_t n, const GridDesc* g, uint32_t* counts,
@@ -78,5 +87,11 @@ hipError_t launch_search(const SearchArgs& a, int knn_mode, int group_lanes, hip
                          int num_cus);
 // Largest k supported for a group width (0 = unsupported width).
 uint32_t max_k_for_group(int group_lanes);
+// Boundary-pair report over a raster grid (a.r = r): count pass when offs is
+// null (counts[row]), else write pass into pairs[2 * (offs[row] + i)] (first
+// cap pairs), each row sorted by point index.
+hipError_t launch_boundary(const SearchArgs& a, double tol, uint32_t* counts,
+                           const uint32_t* offs, uint64_t cap, uint32_t* pairs, hipStream_t st,
+                           int num_cus);
 
 }  // namespace rtnn

@@ -2,7 +2,7 @@
 //
 // Inference (k_mlp_propose, k_mlp_forward): MFMA v_mfma_f32_16x16x16_bf16.
 // The layers are computed transposed, H^T = W * X^T: the weights are the A
-// operand (held in VGPRs for the whole persistent loop) and the activations
+// operand (staged once per block in LDS, in fragment order) and the activations
 // of 16 queries are the B operand.  For this instruction lane l holds
 //   A[l % 16][4 * (l / 16) + j],  B[4 * (l / 16) + j][l % 16],
 //   D[4 * (l / 16) + j][l % 16]      (j = 0..3),
@@ -18,7 +18,9 @@
 //
 // GELU is the tanh form of Hendrycks & Gimpel (2016),
 //   gelu(x) = 0.5 x (1 + tanh(sqrt(2/pi) (x + 0.044715 x^3))),
-// with tanh(u) = 1 - 2 / (1 + exp(2u)) evaluated through the hardware exp.
+// with tanh(u) = 1 - 2 / (1 + exp(2u)) evaluated through the hardware exp and
+// reciprocal (v_exp_f32, v_rcp_f32; about 1 ulp each, far below the BF16
+// rounding of the activations).
 #include <math.h>
 
 #include "../../include/rtnn_hip.h"
@@ -49,7 +51,7 @@ constexpr float kGeluC = 0.7978845608028654f;  // sqrt(2 / pi)
 constexpr float kGeluA = 0.044715f;
 
 __device__ inline float tanh_fast(float u) {
-  return 1.0f - 2.0f / (1.0f + __expf(2.0f * u));
+  return 1.0f - 2.0f * __builtin_amdgcn_rcpf(1.0f + __expf(2.0f * u));
 }
 __device__ inline float gelu(float x) {
   float u = kGeluC * (x + kGeluA * x * x * x);
@@ -126,79 +128,84 @@ __global__ __launch_bounds__(256) void k_mlp_features(const float* __restrict__ 
 
 // ------------------------------------------------------- MFMA inference --
 
-struct MfmaWeights {
-  bf16x4 a1[4][2];  // layer 1: M tile s (16 outputs), K step ks (16 inputs)
-  bf16x4 a2[4][4];  // layer 2
-  float b1[4][4];   // bias of output 16s + 4*(l/16) + j
-  float b2[4][4];
-  float w3[4][4];   // BF16-rounded output weights
+// Weights of the two MFMA layers in LDS, one copy per block, in A-fragment
+// order: lane l reads its bf16x4 of tile (s, ks) at [s][ks][l], 8 consecutive
+// bytes per lane (conflict-free ds_read_b64).  Biases and the last layer by
+// output index (each read is a 4-address broadcast within the wave).  Keeping
+// them out of VGPRs leaves room for more waves to hide the feature gathers.
+struct MfmaLds {
+  bf16x4 a1[4][2][64];
+  bf16x4 a2[4][4][64];
+  float b1[kMlpH], b2[kMlpH], w3[kMlpH];  // w3 BF16-rounded
   float b3;
 };
 
-__device__ inline void load_weights(const float* __restrict__ P, MfmaWeights& w) {
-  const int l = threadIdx.x & 63;
-  const int row = l & 15, grp = l >> 4;
-#pragma unroll
-  for (int s = 0; s < 4; ++s) {
-#pragma unroll
-    for (int ks = 0; ks < 2; ++ks) {
-      bf16x4 v;
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        int col = 16 * ks + 4 * grp + j;
-        float x = col < kMlpF ? P[kOW1 + (16 * s + row) * kMlpF + col] : 0.0f;
-        v[j] = (short)f2bf(x);
-      }
-      w.a1[s][ks] = v;
-    }
-#pragma unroll
-    for (int ks = 0; ks < 4; ++ks) {
-      bf16x4 v;
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-        v[j] = (short)f2bf(P[kOW2 + (16 * s + row) * kMlpH + 16 * ks + 4 * grp + j]);
-      w.a2[s][ks] = v;
-    }
+// Whole block; ends with a barrier.
+__device__ inline void stage_weights(const float* __restrict__ P, MfmaLds& w) {
+  for (int e = threadIdx.x; e < 4 * 2 * 64; e += blockDim.x) {
+    const int s = e / 128, ks = (e / 64) % 2, l = e % 64, row = l & 15, grp = l >> 4;
+    bf16x4 v;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      int o = 16 * s + 4 * grp + j;
-      w.b1[s][j] = P[kOB1 + o];
-      w.b2[s][j] = P[kOB2 + o];
-      w.w3[s][j] = round_bf(P[kOW3 + o]);
+      const int col = 16 * ks + 4 * grp + j;
+      v[j] = (short)f2bf(col < kMlpF ? P[kOW1 + (16 * s + row) * kMlpF + col] : 0.0f);
     }
+    w.a1[s][ks][l] = v;
   }
-  w.b3 = P[kOB3];
+  for (int e = threadIdx.x; e < 4 * 4 * 64; e += blockDim.x) {
+    const int s = e / 256, ks = (e / 64) % 4, l = e % 64, row = l & 15, grp = l >> 4;
+    bf16x4 v;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      v[j] = (short)f2bf(P[kOW2 + (16 * s + row) * kMlpH + 16 * ks + 4 * grp + j]);
+    w.a2[s][ks][l] = v;
+  }
+  for (int e = threadIdx.x; e < kMlpH; e += blockDim.x) {
+    w.b1[e] = P[kOB1 + e];
+    w.b2[e] = P[kOB2 + e];
+    w.w3[e] = round_bf(P[kOW3 + e]);
+  }
+  if (threadIdx.x == 0) w.b3 = P[kOB3];
+  __syncthreads();
 }
 
 // Forward of one 16-query tile given the B operands of layer 1 (x[ks] holds
 // features 16ks + 4*(l/16) + j of query l%16).  Returns o for query l%16 in
 // every lane of the column.
-__device__ inline float mlp_tile(const MfmaWeights& w, const bf16x4 (&x)[2]) {
+__device__ inline float mlp_tile(const MfmaLds& w, const bf16x4 (&x)[2]) {
+  const int l = threadIdx.x & 63, grp = l >> 4;
+  // Compiler-only barrier: keeps the weight reads inside the tile loop (hoisted
+  // out of it they would occupy ~100 VGPRs and halve the waves per SIMD).
+  __asm__ __volatile__("" ::: "memory");
   f32x4 acc[4];
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
     acc[s] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int ks = 0; ks < 2; ++ks)
-      acc[s] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w.a1[s][ks], x[ks], acc[s], 0, 0, 0);
+      acc[s] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w.a1[s][ks][l], x[ks], acc[s], 0, 0, 0);
   }
   bf16x4 h1[4];
 #pragma unroll
   for (int s = 0; s < 4; ++s)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) h1[s][j] = (short)f2bf(gelu(acc[s][j] + w.b1[s][j]));
+    for (int j = 0; j < 4; ++j)
+      h1[s][j] = (short)f2bf(gelu(acc[s][j] + w.b1[16 * s + 4 * grp + j]));
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
     acc[s] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks)
-      acc[s] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w.a2[s][ks], h1[ks], acc[s], 0, 0, 0);
+      acc[s] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w.a2[s][ks][l], h1[ks], acc[s], 0, 0, 0);
   }
   float part = 0.0f;
 #pragma unroll
   for (int s = 0; s < 4; ++s)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) part += w.w3[s][j] * round_bf(gelu(acc[s][j] + w.b2[s][j]));
+    for (int j = 0; j < 4; ++j) {
+      const int o = 16 * s + 4 * grp + j;
+      part += w.w3[o] * round_bf(gelu(acc[s][j] + w.b2[o]));
+    }
   part += __shfl_xor(part, 16, 64);
   part += __shfl_xor(part, 32, 64);
   return part + w.b3;
@@ -208,8 +215,8 @@ __device__ inline float mlp_tile(const MfmaWeights& w, const bf16x4 (&x)[2]) {
 __global__ __launch_bounds__(256) void k_mlp_forward(const float* __restrict__ P,
                                                      const float* __restrict__ feats, uint64_t n,
                                                      float* __restrict__ out) {
-  MfmaWeights w;
-  load_weights(P, w);
+  __shared__ MfmaLds w;
+  stage_weights(P, w);
   const int l = threadIdx.x & 63, row = l & 15, grp = l >> 4;
   const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64;
   const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / 64;
@@ -234,8 +241,8 @@ __global__ __launch_bounds__(256) void k_mlp_propose(const float* __restrict__ P
                                                      const uint32_t* __restrict__ counts,
                                                      float count_scale, float margin, float r,
                                                      float* __restrict__ qradius) {
-  MfmaWeights w;
-  load_weights(P, w);
+  __shared__ MfmaLds w;
+  stage_weights(P, w);
   const GridDesc g = *coarse;
   const int l = threadIdx.x & 63, row = l & 15, grp = l >> 4;
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / 64;

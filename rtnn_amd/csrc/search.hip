@@ -83,6 +83,22 @@ struct TileIter {
   }
 };
 
+// Appends the processing positions of short rows (fewer than k hits) with
+// one atomic per wave; a wave's entries stay together and in lane order, so
+// the re-run over the list keeps the Morton locality of the queries.
+__device__ inline void append_short(const SearchArgs& a, bool is_short, uint32_t qs) {
+  const uint64_t m = __ballot(is_short);
+  if (m == 0) return;
+  const int leader = __ffsll((unsigned long long)m) - 1;
+  uint32_t base = 0;
+  if ((int)(threadIdx.x & (kWave - 1)) == leader) base = atomicAdd(a.short_count, (uint32_t)__popcll(m));
+  base = __shfl(base, leader, kWave);
+  if (is_short) a.short_rows[base + (uint32_t)__popcll(m & lanemask_lt64())] = qs;
+}
+
+// Candidates loaded per lane before the first is tested.
+constexpr int kUnroll = 4;
+
 // --------------------------------------------------------------------------
 // thread per query
 
@@ -157,13 +173,22 @@ __global__ __launch_bounds__(kTileQ) void k_topk_thread(const SearchArgs a) {
     for (int rr = 0; rr < nrows; ++rr) {
       uint32_t s, e;
       row_bounds(rr, s, e);
-      for (uint32_t j = s; j < e; ++j) {
-        const float4 p = a.spts[j];
-        const float d2 = d2_of(qv.x, qv.y, qv.z, p.x, p.y, p.z);
-        const bool ok = KNN ? (d2 > 0.0f && d2 < R2) : (d2 < R2);
-        if (ok) {
-          const uint64_t key = pack_key(d2, __float_as_uint(p.w));
-          if (key < list[KC - 1]) topk_insert<KC>(list, key);
+      for (uint32_t j0 = s; j0 < e; j0 += kUnroll) {
+        // kUnroll loads in flight before the first test
+        float4 pv[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u)
+          pv[u] = j0 + u < e ? a.spts[j0 + u] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+          if (j0 + u >= e) break;
+          const float4 p = pv[u];
+          const float d2 = d2_of(qv.x, qv.y, qv.z, p.x, p.y, p.z);
+          const bool ok = KNN ? (d2 > 0.0f && d2 < R2) : (d2 < R2);
+          if (ok) {
+            const uint64_t key = pack_key(d2, __float_as_uint(p.w));
+            if (key < list[KC - 1]) topk_insert<KC>(list, key);
+          }
         }
       }
     }
@@ -186,7 +211,7 @@ __global__ __launch_bounds__(kTileQ) void k_topk_thread(const SearchArgs a) {
 #pragma unroll
       for (int i = 0; i < KC; ++i)
         if ((uint32_t)i == k - 1) is_short = list[i] == ~0ull;
-      if (is_short) a.short_rows[atomicAdd(a.short_count, 1u)] = qs;
+      append_short(a, is_short, qs);
     }
   }
 }
@@ -206,26 +231,18 @@ __device__ inline uint32_t group_incl_scan(uint32_t v, int l) {
 
 template <int G, int KB>
 struct GroupCfg {
-  static constexpr int NG = kTileQ / G;                  // groups per block
-  static constexpr int CAP = ((2 * KB + G) + G - 1) / G * G;  // LDS keys per group
-  static constexpr int EPL = CAP / G;                    // buffer entries per lane
+  static constexpr int NG = kTileQ / G;  // groups per block
+  // LDS keys per group: k kept + room for later hits before the next select.
+  // Odd, so the groups of a wave start on different LDS banks.
+  static constexpr int CAP = (2 * KB + G) | 1;
 };
 
-// Ranks of this lane's buffer entries among the first cnt entries.
-template <int G, int EPL>
-__device__ inline void rank_entries(const uint64_t* buf, uint32_t cnt, int l,
-                                    uint64_t (&kv)[EPL], uint32_t (&rk)[EPL]) {
-#pragma unroll
-  for (int i = 0; i < EPL; ++i) {
-    const uint32_t e = (uint32_t)(l + i * G);
-    kv[i] = e < cnt ? buf[e] : ~0ull;
-    rk[i] = 0;
-  }
-  for (uint32_t j = 0; j < cnt; ++j) {
-    const uint64_t b = buf[j];
-#pragma unroll
-    for (int i = 0; i < EPL; ++i) rk[i] += b < kv[i] ? 1u : 0u;
-  }
+// Rank of key among buf[0 .. cnt) (keys are distinct: the index is part of
+// the key).
+__device__ inline uint32_t rank_of(const uint64_t* buf, uint32_t cnt, uint64_t key) {
+  uint32_t rk = 0;
+  for (uint32_t j = 0; j < cnt; ++j) rk += buf[j] < key ? 1u : 0u;
+  return rk;
 }
 
 template <int G, int KB, bool KNN, bool HIER>
@@ -234,6 +251,7 @@ __global__ __launch_bounds__(kTileQ) void k_topk_group(const SearchArgs a) {
   __shared__ uint64_t s_buf[C::NG][C::CAP];
   __shared__ uint32_t s_rs[C::NG][G];
   __shared__ uint32_t s_ro[C::NG][G + 1];
+  __shared__ uint64_t s_thr[C::NG];
 
   const int gq = threadIdx.x / G;
   const int l = threadIdx.x % G;
@@ -302,13 +320,26 @@ __global__ __launch_bounds__(kTileQ) void k_topk_group(const SearchArgs a) {
         if (l == 0) ro[0] = 0;
         wave_sync();
         int rp = 0;
-        for (uint32_t v0 = 0; v0 < total; v0 += G) {
-          const uint32_t v = v0 + (uint32_t)l;
+        for (uint32_t v0 = 0; v0 < total; v0 += G * kUnroll) {
+          // Issue the loads of kUnroll candidates per lane before testing any,
+          // so each wave keeps several L2/HBM requests in flight.
+          float4 pv[kUnroll];
+#pragma unroll
+          for (int u = 0; u < kUnroll; ++u) {
+            const uint32_t v = v0 + (uint32_t)(u * G + l);
+            pv[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (v < total) {
+              while (ro[rp + 1] <= v) ++rp;
+              pv[u] = a.spts[rs[rp] + (v - ro[rp])];
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < kUnroll; ++u) {
+          const uint32_t v = v0 + (uint32_t)(u * G + l);
           bool hit = false;
           uint64_t key = 0;
           if (v < total) {
-            while (ro[rp + 1] <= v) ++rp;
-            const float4 p = a.spts[rs[rp] + (v - ro[rp])];
+            const float4 p = pv[u];
             const float d2 = d2_of(qv.x, qv.y, qv.z, p.x, p.y, p.z);
             const bool ok = KNN ? (d2 > 0.0f && d2 < R2) : (d2 < R2);
             key = pack_key(d2, __float_as_uint(p.w));
@@ -318,18 +349,30 @@ __global__ __launch_bounds__(kTileQ) void k_topk_group(const SearchArgs a) {
           if (hit) buf[cnt + (uint32_t)__popcll(m & lanemask_lt64())] = key;
           cnt += (uint32_t)__popcll(m);
           if (cnt > (uint32_t)(C::CAP - G)) {
-            // Keep the k smallest; later hits must beat the k-th.
+            // Keep the k smallest: find the k-th smallest key T, then compact
+            // the entries <= T to the front in place (a round writes below
+            // the entries it has read), and later hits must beat T.
             wave_sync();
-            uint64_t kv[C::EPL];
-            uint32_t rk[C::EPL];
-            rank_entries<G, C::EPL>(buf, cnt, l, kv, rk);
+            for (uint32_t e = (uint32_t)l; e < cnt; e += G) {
+              const uint64_t key2 = buf[e];
+              if (rank_of(buf, cnt, key2) == k - 1) s_thr[gq] = key2;
+            }
             wave_sync();
-#pragma unroll
-            for (int i = 0; i < C::EPL; ++i)
-              if ((uint32_t)(l + i * G) < cnt && rk[i] < k) buf[rk[i]] = kv[i];
-            wave_sync();
+            const uint64_t T = s_thr[gq];
+            uint32_t kept = 0;
+            for (uint32_t e0 = 0; e0 < cnt; e0 += G) {
+              const uint32_t e = e0 + (uint32_t)l;
+              const uint64_t key2 = e < cnt ? buf[e] : ~0ull;
+              const bool keep = e < cnt && key2 <= T;
+              const uint64_t km = __ballot(keep) & gmask;
+              wave_sync();
+              if (keep) buf[kept + (uint32_t)__popcll(km & lanemask_lt64())] = key2;
+              kept += (uint32_t)__popcll(km);
+              wave_sync();
+            }
             cnt = k;
-            thr = buf[k - 1];
+            thr = T;
+          }
           }
         }
         wave_sync();
@@ -337,24 +380,22 @@ __global__ __launch_bounds__(kTileQ) void k_topk_group(const SearchArgs a) {
 
       // Final: rank-sort the buffer and write the row.
       wave_sync();
-      uint64_t kv[C::EPL];
-      uint32_t rk[C::EPL];
-      rank_entries<G, C::EPL>(buf, cnt, l, kv, rk);
       const uint32_t base = a.local_index ? g.point_base : 0u;
       int32_t* oi = a.out_idx + (uint64_t)row * k;
       float* od = a.out_d2 ? a.out_d2 + (uint64_t)row * k : nullptr;
-#pragma unroll
-      for (int i = 0; i < C::EPL; ++i) {
-        if ((uint32_t)(l + i * G) < cnt && rk[i] < k) {
-          oi[rk[i]] = (int32_t)(key_idx(kv[i]) - base);
-          if (od) od[rk[i]] = key_d2(kv[i]);
+      for (uint32_t e = (uint32_t)l; e < cnt; e += G) {
+        const uint64_t key2 = buf[e];
+        const uint32_t rk = rank_of(buf, cnt, key2);
+        if (rk < k) {
+          oi[rk] = (int32_t)(key_idx(key2) - base);
+          if (od) od[rk] = key_d2(key2);
         }
       }
       for (uint32_t s = cnt + (uint32_t)l; s < k; s += G) {
         oi[s] = -1;
         if (od) od[s] = __builtin_huge_valf();
       }
-      if (a.short_rows && l == 0 && cnt < k) a.short_rows[atomicAdd(a.short_count, 1u)] = qs;
+      if (a.short_rows) append_short(a, l == 0 && cnt < k, qs);
       wave_sync();
     }
   }
@@ -441,6 +482,99 @@ hipError_t launch_search(const SearchArgs& a0, int knn_mode, int group_lanes, hi
     else if (k <= 256) launch_group<64, 256>(a, knn, grid, st);
     else launch_group<64, 1024>(a, knn, grid, st);
   }
+  return hipGetLastError();
+}
+
+}  // namespace rtnn
+
+namespace rtnn {
+
+// --------------------------------------------------------------------------
+// boundary-pair report
+
+// Pairs (query row, point index) whose distance, evaluated in FP64 from the
+// FP32 coordinates, lies within tol * r of r; these are the pairs whose
+// membership a 1-ulp difference in the FP32 d2 could flip.  Count pass
+// (offs == nullptr): counts[row] = pairs of that row.  Write pass: the pairs
+// of a row go to pairs[2 * (offs[row] + i)] in (row, point) order, positions
+// at or past cap are dropped.  Thread per query; any tile layout of
+// SearchArgs (single grid or batched clouds) works.
+__global__ __launch_bounds__(kTileQ) void k_boundary(const SearchArgs a, double tol,
+                                                     uint32_t* __restrict__ counts,
+                                                     const uint32_t* __restrict__ offs,
+                                                     uint64_t cap, uint32_t* __restrict__ pairs) {
+  const uint32_t nq = launch_nq(a);
+  for (TileIter it(a, nq); it.t < it.end; it.t += it.step) {
+    uint32_t q0, q1, gi;
+    tile_of(a, nq, it.t, q0, q1, gi);
+    const uint32_t qi = q0 + threadIdx.x;
+    if (qi >= q1) continue;
+    const uint32_t qs = a.qsel ? a.qsel[qi] : qi;
+    const GridDesc g = a.grids[gi];
+    const float4 qv = a.queries[qs];
+    const uint32_t row = __float_as_uint(qv.w);
+    const double rr = (double)a.r, lim = tol * rr;
+    const float R = (float)(rr + lim);  // cell_range's padding covers the rounding
+    int lo0, hi0, lo1, hi1, lo2, hi2;
+    cell_range(g, qv.x, R, 0, lo0, hi0);
+    cell_range(g, qv.y, R, 1, lo1, hi1);
+    cell_range(g, qv.z, R, 2, lo2, hi2);
+    const uint32_t base = a.local_index ? g.point_base : 0u;
+    const uint64_t o = offs ? offs[row] : 0;
+    uint32_t c = 0;
+    for (int cz = lo2; cz <= hi2; ++cz)
+      for (int cy = lo1; cy <= hi1; ++cy) {
+        if (lo0 > hi0) continue;
+        const uint32_t rowc = g.cell_base + ((uint32_t)cz * (uint32_t)g.dim[1] + (uint32_t)cy) *
+                                                (uint32_t)g.dim[0];
+        const uint32_t s = a.cell_start[rowc + lo0], e = a.cell_start[rowc + hi0 + 1];
+        for (uint32_t j = s; j < e; ++j) {
+          const float4 p = a.spts[j];
+          const double dx = (double)qv.x - (double)p.x;
+          const double dy = (double)qv.y - (double)p.y;
+          const double dz = (double)qv.z - (double)p.z;
+          const double d = sqrt(dx * dx + dy * dy + dz * dz);
+          if (fabs(d - rr) <= lim) {
+            if (offs && o + c < cap) {
+              pairs[2 * (o + c)] = row;
+              pairs[2 * (o + c) + 1] = __float_as_uint(p.w) - base;
+            }
+            ++c;
+          }
+        }
+      }
+    if (!offs) {
+      counts[row] = c;
+      continue;
+    }
+    // The row's pairs arrive in cell order; sort them by point index.
+    const uint64_t end = o + c < cap ? o + c : cap;
+    for (uint64_t i = o + 1; i < end; ++i) {
+      const uint32_t v = pairs[2 * i + 1];
+      uint64_t j = i;
+      for (; j > o && pairs[2 * (j - 1) + 1] > v; --j) pairs[2 * j + 1] = pairs[2 * (j - 1) + 1];
+      pairs[2 * j + 1] = v;
+    }
+  }
+}
+
+hipError_t launch_boundary(const SearchArgs& a0, double tol, uint32_t* counts,
+                           const uint32_t* offs, uint64_t cap, uint32_t* pairs, hipStream_t st,
+                           int num_cus) {
+  SearchArgs a = a0;
+  if (a.nlevels > 0) return hipErrorInvalidValue;  // raster grids only
+  if (!a.tiles) {
+    a.tile_q = kTileQ;
+    a.ntiles = (uint32_t)(((uint64_t)a.nq + kTileQ - 1) / kTileQ);
+  }
+  if (a.ntiles == 0) return hipSuccess;
+  uint32_t blocks = a.ntiles;
+  if (a.persistent) {
+    const uint32_t lim = (uint32_t)(num_cus > 0 ? num_cus : 256) * 8u;
+    if (blocks > lim) blocks = lim;
+  }
+  hipLaunchKernelGGL(k_boundary, dim3(blocks), dim3(kTileQ), 0, st, a, tol, counts, offs, cap,
+                     pairs);
   return hipGetLastError();
 }
 

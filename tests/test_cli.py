@@ -76,3 +76,26 @@ def test_cli_knn_separate_queries(tmp_path, extra):
     k = idx.shape[1]
     wi, wd = oracle.bruteforce(p, q, oracle.MODE_KNN, 3.0, k, threads=4)
     assert np.array_equal(idx, wi) and np.array_equal(d2, wd)
+
+
+@pytest.mark.gpu
+def test_cli_crratio_autotune_and_boundary(tmp_path):
+    # -cr sets the cell factor, -ac 1 runs the autotuner, --boundary lists the
+    # pairs within 1e-6 r of r; the result stays the exact range search.
+    import oracle
+    g = np.stack(np.meshgrid(*[np.arange(12, dtype=np.float32) * np.float32(0.1)] * 3),
+                 -1).reshape(-1, 3)
+    pf, out, bf = tmp_path / "p.txt", str(tmp_path / "o.bin"), str(tmp_path / "b.txt")
+    np.savetxt(pf, g, fmt="%.9g", delimiter=",")
+    p = np.loadtxt(pf, delimiter=",", dtype=np.float64).astype(np.float32)
+    for extra in (["-cr", "2"], ["-ac", "1"]):
+        r = run("-f", str(pf), "-r", "0.2", "-k", "40", "-o", out, "--boundary", bf, *extra)
+        assert r.returncode == 0, r.stderr
+        idx, d2 = read_result(out)
+        wi, wd = oracle.bruteforce(p, p, oracle.MODE_RADIUS, 0.2, 40, threads=4)
+        assert np.array_equal(idx, wi) and np.array_equal(d2, wd)
+        cnt, want = oracle.boundary_pairs(p, p, 0.2, max_out=1 << 20)
+        got = np.loadtxt(bf, dtype=np.float64, ndmin=2)
+        assert cnt > 0 and len(got) == cnt
+        assert np.array_equal(got[:, :2].astype(np.uint64), want)
+    assert "autotune: cell_factor" in r.stdout

@@ -158,8 +158,34 @@ def test_boundary_pairs_listed_and_exact(ctx):
     xyz, offs = rtnn_amd.gen_lattice_clouds(4, 1, 3000, 3000, jit=0.0)
     ctx.set_points(xyz)
     check(ctx, xyz, None, "radius", 0.2, 64)
-    cnt, _ = oracle.boundary_pairs(xyz, xyz, 0.2, qsel=np.arange(200))
+    cnt, want = oracle.boundary_pairs(xyz, xyz, 0.2, max_out=1 << 20)
     assert cnt > 0  # the boundary is exercised (and the check above was exact)
+    got = ctx.boundary_pairs(None, radius=0.2)
+    assert np.array_equal(got.astype(np.uint64), want)
+    # separate queries, a wider tolerance
+    rng = np.random.default_rng(8)
+    qs = xyz[rng.integers(0, len(xyz), 500)] + rng.uniform(-1e-3, 1e-3, (500, 3)).astype(np.float32)
+    cnt, want = oracle.boundary_pairs(xyz, qs, 0.2, tol=1e-2, max_out=1 << 20)
+    got = ctx.boundary_pairs(qs, radius=0.2, tol=1e-2)
+    assert cnt > 0 and np.array_equal(got.astype(np.uint64), want)
+
+
+def test_boundary_pairs_batched(ctx):
+    import torch
+    xyz, offs = rtnn_amd.gen_lattice_clouds(5, 3, 500, 3000)
+    offs = np.insert(offs, 1, offs[1])  # an empty cloud
+    got = ctx.boundary_pairs_batched(torch.from_numpy(xyz).cuda(), offs, radius=0.2, tol=1e-4)
+    want = []
+    for c in range(len(offs) - 1):
+        a, b = int(offs[c]), int(offs[c + 1])
+        if a == b:
+            continue
+        n, pr = oracle.boundary_pairs(xyz[a:b], xyz[a:b], 0.2, tol=1e-4, max_out=1 << 20)
+        pr = pr.copy()
+        pr[:, 0] += a
+        want.append(pr)
+    want = np.concatenate(want)
+    assert len(want) > 0 and np.array_equal(got.astype(np.uint64), want)
 
 
 def test_errors(ctx, samplepc):
@@ -208,6 +234,52 @@ def test_batched_clouds(ctx, qpw):
     a, b = int(offs[5]), int(offs[6])
     wi, wd = oracle.grid_search(xyz[a:b], xyz[a:b], oracle.MODE_KNN, 0.25, 8, threads=8)
     assert np.array_equal(ki.cpu().numpy()[a:b], wi)
+
+
+@pytest.mark.parametrize("mode,k", [("radius", 64), ("knn", 8)])
+def test_graph_batched_replays(ctx, mode, k):
+    # The captured pipeline (device-side grid plan, sort, cell table, gather,
+    # persistent search) equals the eager batched search on every replay, with
+    # unrelated eager work and synchronisations between replays, and follows
+    # new point values written into the bound tensor.
+    import torch
+    xyz, offs = rtnn_amd.gen_lattice_clouds(11, 30, 500, 6000)
+    offs = np.insert(offs, 3, offs[3])  # an empty cloud
+    t = torch.from_numpy(xyz).cuda()
+    r = 0.2 if mode == "radius" else 0.25
+    want_i, want_d, _ = ctx.search_batched(t, offs, mode=mode, radius=r, k=k)
+    g = ctx.graph_batched(t, offs, mode=mode, radius=r, k=k)
+    try:
+        other = rtnn_amd.gen_uniform(3, 20000, 0.0, 10.0)
+        for rep in range(4):
+            g.idx.fill_(-7)
+            g.d2.fill_(-7.0)
+            st = g.launch()
+            assert st.ms_total > 0
+            assert torch.equal(g.idx, want_i), rep
+            assert torch.equal(g.d2, want_d), rep
+            ctx.set_points(other)  # eager work on the same context and stream
+            ctx.search(None, mode="radius", radius=0.5, k=16)
+            torch.cuda.synchronize()
+        # new values in the bound tensor: one cloud stretched (needs a coarser
+        # table), one translated far away
+        a, b = int(offs[1]), int(offs[2])
+        t[a:b] *= 3.0
+        a, b = int(offs[5]), int(offs[6])
+        t[a:b] += 1000.0
+        g.launch()
+        want_i, want_d, _ = ctx.search_batched(t, offs, mode=mode, radius=r, k=k)
+        assert torch.equal(g.idx, want_i) and torch.equal(g.d2, want_d)
+        h = t.cpu().numpy()
+        wi, wd = oracle.grid_search(h[a:b], h[a:b], MODES[mode], r, k, threads=8)
+        assert np.array_equal(g.idx[a:b].cpu().numpy(), wi)
+        assert np.array_equal(g.d2[a:b].cpu().numpy(), wd)
+        # a non-finite coordinate is reported by a synchronous replay
+        t[0, 0] = float("nan")
+        with pytest.raises(rtnn_amd.RtnnError, match="non-finite"):
+            g.launch()
+    finally:
+        g.close()
 
 
 @pytest.mark.slow
