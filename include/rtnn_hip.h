@@ -60,6 +60,9 @@ typedef struct {
                              that many levels (edges r, r/2, ...); 0 = auto       */
   float level_alpha;      /* hierarchical: a query of radius R walks the finest
                              level whose cell edge is >= R * level_alpha (0: 1.0) */
+  int   group_buffer;     /* group kernel's LDS hit buffer per query: 0 auto / 2
+                             wide (2k + G keys), 1 compact (k + 2G keys: less LDS,
+                             more waves per CU, more selects on dense rows)     */
 } rtnn_options;
 
 /* Filled by search entry points when a non-NULL pointer is passed. */
@@ -179,8 +182,8 @@ int rtnn_search_percloud(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets
  * offsets) and rewrites the outputs, equal to rtnn_search_batched on that
  * data.  The cell table is sized from the data at capture (+25 %); a replay
  * on data that needs more cells uses a coarser edge (still exact).  Replays
- * run on the context's stream; one graph per context at a time may run.
- * Destroy graphs before their context. */
+ * run on the context's stream.  rtnn_destroy also destroys the context's
+ * remaining graphs (their handles must not be used afterwards). */
 typedef struct rtnn_graph_s* rtnn_graph;
 int rtnn_graph_batched(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets,
                        uint32_t nclouds, int mode, float r, uint32_t k,

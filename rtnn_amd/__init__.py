@@ -13,6 +13,7 @@ order, empty slots -1 (the reference's UINT_MAX sentinel, search.cpp:14-18).
 """
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
@@ -47,7 +48,8 @@ class Options(ctypes.Structure):
                 ("time_stages", ctypes.c_int),
                 ("max_cells", ctypes.c_uint32),
                 ("grid_levels", ctypes.c_int),
-                ("level_alpha", ctypes.c_float)]
+                ("level_alpha", ctypes.c_float),
+                ("group_buffer", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -192,9 +194,12 @@ class Context:
         self._h = h
         self._L = L
         self._points = None
+        self._graphs = weakref.WeakSet()
 
     def close(self):
         if getattr(self, "_h", None):
+            for g in list(self._graphs):
+                g.close()
             self._L.rtnn_destroy(self._h)
             self._h = None
 
@@ -215,9 +220,10 @@ class Context:
             raise RtnnError(rc, self._L.rtnn_last_error(self._h).decode())
 
     def set_options(self, cell_factor=0.0, queries_per_wave=0, query_order=0,
-                    time_stages=0, max_cells=0, grid_levels=0, level_alpha=0.0):
+                    time_stages=0, max_cells=0, grid_levels=0, level_alpha=0.0,
+                    group_buffer=0):
         o = Options(cell_factor, queries_per_wave, query_order, time_stages, max_cells,
-                    grid_levels, level_alpha)
+                    grid_levels, level_alpha, group_buffer)
         self._check(self._L.rtnn_set_options(self._h, ctypes.byref(o)))
 
     def synchronize(self):
@@ -348,7 +354,9 @@ class Context:
             self._h, ctypes.c_void_p(xyz.data_ptr()), ctypes.c_void_p(offs.ctypes.data),
             len(offs) - 1, _mode(mode), radius, k, ctypes.c_void_p(idx.data_ptr()),
             ctypes.c_void_p(d2.data_ptr()) if d2 is not None else None, ctypes.byref(h)))
-        return BatchedGraph(self, h, xyz, idx, d2)
+        g = BatchedGraph(self, h, xyz, idx, d2)
+        self._graphs.add(g)
+        return g
 
     def boundary_pairs(self, queries=None, radius=2.0, tol=1e-6):
         """Pairs (query row, point index) with |d - r| <= tol * r, d in FP64,
@@ -406,7 +414,8 @@ class Context:
                                           ctypes.c_void_p(d2.data_ptr()), ctypes.byref(best),
                                           ctypes.byref(ms)))
         return ({"cell_factor": best.cell_factor, "queries_per_wave": best.queries_per_wave,
-                 "grid_levels": best.grid_levels, "level_alpha": best.level_alpha}, ms.value)
+                 "grid_levels": best.grid_levels, "level_alpha": best.level_alpha,
+                 "group_buffer": best.group_buffer}, ms.value)
 
     # MLP building blocks (device tensors)
     def mlp_features(self, queries, radius):
@@ -522,9 +531,9 @@ class BatchedGraph:
         return st
 
     def close(self):
-        if self._h:
+        if self._h and getattr(self._ctx, "_h", None):
             self._ctx._L.rtnn_graph_destroy(self._h)
-            self._h = None
+        self._h = None
 
     def __enter__(self):
         return self

@@ -91,6 +91,9 @@ struct rtnn_ctx_s {
   DevBuf tr_q, tr_idx, tr_d2, tr_t, tr_f, tr_out;
   MlpState mlp;
 
+  // graphs captured on this context (destroyed with it)
+  std::vector<rtnn_graph_s*> graphs;
+
   // boundary-pair report
   DevBuf bcnt, boff, bpairs;
   uint32_t batch_tiles = 0;  // tiles of the last batched grid
@@ -107,6 +110,9 @@ struct rtnn_ctx_s {
     if (st) (void)hipStreamDestroy(st);
   }
 };
+
+// Destroys the graphs still alive on ctx (defined with rtnn_graph_s).
+static void destroy_graphs(rtnn_ctx ctx);
 
 namespace {
 
@@ -174,6 +180,9 @@ int default_group_lanes(int mode, uint32_t k) {
   if (k <= 256) return 16;
   return 64;
 }
+
+// Group kernel LDS buffer: compact (k + 2G keys per query) only on request.
+bool wide_buffers(rtnn_ctx ctx) { return ctx->opt.group_buffer != 1; }
 
 int group_lanes_from_opt(rtnn_ctx ctx, int mode, uint32_t k) {
   int qpw = ctx->opt.queries_per_wave;
@@ -461,6 +470,7 @@ int rtnn_destroy(rtnn_ctx ctx) {
   if (!ctx) return RTNN_ERR_INVALID;
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->st);
+  destroy_graphs(ctx);
   delete ctx;
   return RTNN_OK;
 }
@@ -478,6 +488,8 @@ int rtnn_set_options(rtnn_ctx ctx, const rtnn_options* opt) {
     return fail(ctx, RTNN_ERR_INVALID, "query_order must be 0, 1 or 2");
   if (opt->grid_levels < 0 || opt->grid_levels > 8)
     return fail(ctx, RTNN_ERR_INVALID, "grid_levels must be in [0, 8]");
+  if (opt->group_buffer < 0 || opt->group_buffer > 2)
+    return fail(ctx, RTNN_ERR_INVALID, "group_buffer must be 0, 1 or 2");
   if (!(opt->level_alpha >= 0.0f) || !std::isfinite(opt->level_alpha))
     return fail(ctx, RTNN_ERR_INVALID, "level_alpha must be >= 0");
   ctx->opt = *opt;
@@ -551,7 +563,7 @@ static int search_impl(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_devi
   rc = output_targets(ctx, nq, k, out_idx, out_d2, out_on_device, o);
   if (rc) return rc;
   SearchArgs a = base_args(ctx, qproc, nq, r, k, o);
-  CK(launch_search(a, mode == RTNN_MODE_KNN, gl, ctx->st, ctx->num_cus));
+  CK(launch_search(a, mode == RTNN_MODE_KNN, gl, ctx->st, ctx->num_cus, wide_buffers(ctx)));
   if (timing) CK(hipEventRecord(ctx->ev[3], ctx->st));
   rc = copy_out(ctx, nq, k, out_idx, out_d2, out_on_device, o);
   if (rc) return rc;
@@ -699,7 +711,7 @@ int proposal_search(rtnn_ctx ctx, Proposal kind, const float* qxyz, uint64_t nq,
   a.qradius = ctx->qradius.as<float>();
   a.short_rows = short1;
   a.short_count = &counters[0];
-  CK(launch_search(a, 1, gl, ctx->st, ctx->num_cus));
+  CK(launch_search(a, 1, gl, ctx->st, ctx->num_cus, wide_buffers(ctx)));
   CK(hipEventRecord(ctx->ev[3], ctx->st));
   // Stage 2: short rows at min(r, 2 r'); stage 3: what is still short, at r.
   SearchArgs s2 = base_args(ctx, qproc, nq, r, k, o);
@@ -709,11 +721,11 @@ int proposal_search(rtnn_ctx ctx, Proposal kind, const float* qxyz, uint64_t nq,
   s2.qradius_scale = 2.0f;
   s2.short_rows = short2;
   s2.short_count = &counters[1];
-  CK(launch_search(s2, 1, gl, ctx->st, ctx->num_cus));
+  CK(launch_search(s2, 1, gl, ctx->st, ctx->num_cus, wide_buffers(ctx)));
   SearchArgs s3 = base_args(ctx, qproc, nq, r, k, o);
   s3.qsel = short2;
   s3.nq_dev = &counters[1];
-  CK(launch_search(s3, 1, gl, ctx->st, ctx->num_cus));
+  CK(launch_search(s3, 1, gl, ctx->st, ctx->num_cus, wide_buffers(ctx)));
   CK(hipEventRecord(ctx->ev[4], ctx->st));
   rc = copy_out(ctx, nq, k, out_idx, out_d2, out_on_device, o);
   if (rc) return rc;
@@ -1081,7 +1093,7 @@ int rtnn_search_batched(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets,
 
   OutBufs o{out_idx, out_d2};
   SearchArgs a = batched_args(ctx, n, r, k, o);
-  CK(launch_search(a, mode == RTNN_MODE_KNN, gl, ctx->st, ctx->num_cus));
+  CK(launch_search(a, mode == RTNN_MODE_KNN, gl, ctx->st, ctx->num_cus, wide_buffers(ctx)));
   if (timing) CK(hipEventRecord(ctx->ev[2], ctx->st));
   CK(hipStreamSynchronize(ctx->st));
   if (stats) {
@@ -1198,18 +1210,24 @@ int rtnn_autotune(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_device, i
   rtnn_options best_opt = base;
   // Candidates: raster grids (cell factor x lanes per query); for the radius
   // proposal methods also hierarchical Morton grids (levels x level alpha).
+  // Group kernels on raster grids are also tried with the compact LDS buffer.
   struct Cand {
     float cf;
     int qpw, levels;
     float alpha;
+    int buffer;
   };
   std::vector<Cand> cands;
   for (int i = 0; i < ncf; ++i)
-    for (int j = 0; j < nqpw; ++j) cands.push_back({cfs[i], qpws[j], plain ? 0 : 1, 0.0f});
+    for (int j = 0; j < nqpw; ++j) {
+      const bool group = qpws[j] > 0 && qpws[j] < 64;
+      cands.push_back({cfs[i], qpws[j], plain ? 0 : 1, 0.0f, group ? 2 : 0});
+      if (group) cands.push_back({cfs[i], qpws[j], plain ? 0 : 1, 0.0f, 1});
+    }
   if (!plain)
     for (int lv : {3, 5})
       for (float al : {1.0f, 0.5f})
-        for (int qpw : {0, 16}) cands.push_back({0.0f, qpw, lv, al});
+        for (int qpw : {0, 16}) cands.push_back({0.0f, qpw, lv, al, 0});
   for (const Cand& cand : cands) {
     {
       const int qpw = cand.qpw;
@@ -1220,6 +1238,7 @@ int rtnn_autotune(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_device, i
       ctx->opt.queries_per_wave = qpw;
       ctx->opt.grid_levels = cand.levels;
       ctx->opt.level_alpha = cand.alpha;
+      ctx->opt.group_buffer = cand.buffer;
       rtnn_stats st{};
       int rc = RTNN_OK;
       for (int rep = 0; rep < 2 && rc == RTNN_OK; ++rep) {  // first run sizes the buffers
@@ -1298,6 +1317,11 @@ struct rtnn_graph_s {
   }
 };
 
+static void destroy_graphs(rtnn_ctx ctx) {
+  for (rtnn_graph_s* g : ctx->graphs) delete g;
+  ctx->graphs.clear();
+}
+
 namespace {
 
 // The whole batched pipeline on ctx->st with the graph's buffers: status
@@ -1344,7 +1368,7 @@ hipError_t enqueue_batched(rtnn_graph g, const float* xyz, float cell, int mode,
   a.out_d2 = out_d2;
   a.local_index = 1;
   a.persistent = 1;
-  return launch_search(a, mode == RTNN_MODE_KNN, gl, st, g->ctx->num_cus);
+  return launch_search(a, mode == RTNN_MODE_KNN, gl, st, g->ctx->num_cus, wide_buffers(g->ctx));
 }
 
 }  // namespace
@@ -1408,6 +1432,7 @@ int rtnn_graph_batched(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets,
   CK(e2);
   CK(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
   *out = g.release();
+  ctx->graphs.push_back(*out);
   return RTNN_OK;
 }
 
@@ -1434,8 +1459,11 @@ int rtnn_graph_launch(rtnn_graph g, int sync, rtnn_stats* stats) {
 
 int rtnn_graph_destroy(rtnn_graph g) {
   if (!g) return RTNN_OK;
-  (void)hipSetDevice(g->ctx->device);
-  (void)hipStreamSynchronize(g->ctx->st);
+  rtnn_ctx ctx = g->ctx;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->st);
+  auto it = std::find(ctx->graphs.begin(), ctx->graphs.end(), g);
+  if (it != ctx->graphs.end()) ctx->graphs.erase(it);
   delete g;
   return RTNN_OK;
 }

@@ -114,6 +114,46 @@ __device__ inline float feature(const GridDesc& g, const uint32_t* __restrict__ 
   return 0.0f;
 }
 
+// The 8 features 16ks + 4*grp + j (ks < 2, j < 4) of one query, as the two
+// BF16 B-operand fragments of layer 1: every count address is formed first
+// and all loads are issued before the first is used (branch-free; an out of
+// range cell loads entry 0 and is masked), so a lane waits on one memory
+// round trip per tile instead of one per feature.  Same values as feature().
+__device__ inline void features_frag(const GridDesc& g, const uint32_t* __restrict__ counts,
+                                     const QueryCell& qc, int grp, float count_scale,
+                                     bf16x4 (&x)[2]) {
+  uint32_t id[8];
+  bool valid[8];
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    const int f = 16 * (t / 4) + 4 * grp + (t % 4);
+    const int cx = qc.cx + f % 3 - 1, cy = qc.cy + (f / 3) % 3 - 1, cz = qc.cz + f / 9 - 1;
+    valid[t] = f < 27 && cx >= 0 && cy >= 0 && cz >= 0 && cx < g.dim[0] && cy < g.dim[1] &&
+               cz < g.dim[2];
+    uint32_t cell;
+    if (g.morton)
+      cell = g.cell_base + morton3((uint32_t)cx, (uint32_t)cy, (uint32_t)cz);
+    else
+      cell = ((uint32_t)cz * (uint32_t)g.dim[1] + (uint32_t)cy) * (uint32_t)g.dim[0] +
+             (uint32_t)cx;
+    id[t] = valid[t] ? cell : 0u;
+  }
+  uint32_t c0[8], c1[8];
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    c0[t] = counts[id[t]];
+    c1[t] = g.morton ? counts[id[t] + 1] : 0u;
+  }
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    const int f = 16 * (t / 4) + 4 * grp + (t % 4);
+    const uint32_t c = g.morton ? c1[t] - c0[t] : c0[t];
+    float v = valid[t] ? __logf(1.0f + (float)c) * count_scale : 0.0f;
+    v = f == 27 ? qc.ox : (f == 28 ? qc.oy : (f == 29 ? qc.oz : v));
+    x[t / 4][t % 4] = (short)f2bf(v);
+  }
+}
+
 __global__ __launch_bounds__(256) void k_mlp_features(const float* __restrict__ q, uint64_t n,
                                                       const GridDesc* __restrict__ coarse,
                                                       const uint32_t* __restrict__ counts,
@@ -254,11 +294,7 @@ __global__ __launch_bounds__(256) void k_mlp_propose(const float* __restrict__ P
     if (qi < nq) {
       const float4 q = qproc[qi];
       QueryCell qc = query_cell(g, q.x, q.y, q.z);
-#pragma unroll
-      for (int ks = 0; ks < 2; ++ks)
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-          x[ks][j] = (short)f2bf(feature(g, counts, qc, 16 * ks + 4 * grp + j, count_scale));
+      features_frag(g, counts, qc, grp, count_scale, x);
     } else {
 #pragma unroll
       for (int ks = 0; ks < 2; ++ks) x[ks] = bf16x4{0, 0, 0, 0};

@@ -229,25 +229,39 @@ __device__ inline uint32_t group_incl_scan(uint32_t v, int l) {
   return v;
 }
 
-template <int G, int KB>
+template <int G, int KB, bool WIDE>
 struct GroupCfg {
   static constexpr int NG = kTileQ / G;  // groups per block
-  // LDS keys per group: k kept + room for later hits before the next select.
-  // Odd, so the groups of a wave start on different LDS banks.
-  static constexpr int CAP = (2 * KB + G) | 1;
+  // LDS keys per group: k kept + room for later hits before the next select
+  // (wide: k more; compact: 2G more, a smaller LDS footprint for more waves
+  // per CU when rows rarely overflow).  Odd, so the groups of a wave start on
+  // different LDS banks.
+  static constexpr int CAP = (WIDE ? 2 * KB + G : KB + 2 * G) | 1;
 };
 
-// Rank of key among buf[0 .. cnt) (keys are distinct: the index is part of
-// the key).
-__device__ inline uint32_t rank_of(const uint64_t* buf, uint32_t cnt, uint64_t key) {
-  uint32_t rk = 0;
-  for (uint32_t j = 0; j < cnt; ++j) rk += buf[j] < key ? 1u : 0u;
-  return rk;
+// Ranks among buf[0 .. cnt) of this lane's entries e = base + l + G*i,
+// i < CH (rank ~0 marks no entry), in one pass over the buffer.  Keys are
+// distinct (the index is part of the key).
+template <int G, int CH>
+__device__ inline void rank_chunk(const uint64_t* buf, uint32_t cnt, uint32_t base, int l,
+                                  uint64_t (&kv)[CH], uint32_t (&rk)[CH]) {
+#pragma unroll
+  for (int i = 0; i < CH; ++i) {
+    const uint32_t e = base + (uint32_t)(l + G * i);
+    kv[i] = e < cnt ? buf[e] : ~0ull;
+    rk[i] = e < cnt ? 0u : ~0u;
+  }
+  for (uint32_t j = 0; j < cnt; ++j) {
+    const uint64_t b = buf[j];
+#pragma unroll
+    for (int i = 0; i < CH; ++i) rk[i] += b < kv[i] ? 1u : 0u;
+  }
 }
+constexpr int kRankChunk = 8;
 
-template <int G, int KB, bool KNN, bool HIER>
+template <int G, int KB, bool KNN, bool HIER, bool WIDE>
 __global__ __launch_bounds__(kTileQ) void k_topk_group(const SearchArgs a) {
-  using C = GroupCfg<G, KB>;
+  using C = GroupCfg<G, KB, WIDE>;
   __shared__ uint64_t s_buf[C::NG][C::CAP];
   __shared__ uint32_t s_rs[C::NG][G];
   __shared__ uint32_t s_ro[C::NG][G + 1];
@@ -353,9 +367,13 @@ __global__ __launch_bounds__(kTileQ) void k_topk_group(const SearchArgs a) {
             // the entries <= T to the front in place (a round writes below
             // the entries it has read), and later hits must beat T.
             wave_sync();
-            for (uint32_t e = (uint32_t)l; e < cnt; e += G) {
-              const uint64_t key2 = buf[e];
-              if (rank_of(buf, cnt, key2) == k - 1) s_thr[gq] = key2;
+            for (uint32_t b0 = 0; b0 < cnt; b0 += G * kRankChunk) {
+              uint64_t kv[kRankChunk];
+              uint32_t rk[kRankChunk];
+              rank_chunk<G, kRankChunk>(buf, cnt, b0, l, kv, rk);
+#pragma unroll
+              for (int i = 0; i < kRankChunk; ++i)
+                if (rk[i] == k - 1) s_thr[gq] = kv[i];
             }
             wave_sync();
             const uint64_t T = s_thr[gq];
@@ -383,13 +401,16 @@ __global__ __launch_bounds__(kTileQ) void k_topk_group(const SearchArgs a) {
       const uint32_t base = a.local_index ? g.point_base : 0u;
       int32_t* oi = a.out_idx + (uint64_t)row * k;
       float* od = a.out_d2 ? a.out_d2 + (uint64_t)row * k : nullptr;
-      for (uint32_t e = (uint32_t)l; e < cnt; e += G) {
-        const uint64_t key2 = buf[e];
-        const uint32_t rk = rank_of(buf, cnt, key2);
-        if (rk < k) {
-          oi[rk] = (int32_t)(key_idx(key2) - base);
-          if (od) od[rk] = key_d2(key2);
-        }
+      for (uint32_t b0 = 0; b0 < cnt; b0 += G * kRankChunk) {
+        uint64_t kv[kRankChunk];
+        uint32_t rk[kRankChunk];
+        rank_chunk<G, kRankChunk>(buf, cnt, b0, l, kv, rk);
+#pragma unroll
+        for (int i = 0; i < kRankChunk; ++i)
+          if (rk[i] < k) {
+            oi[rk[i]] = (int32_t)(key_idx(kv[i]) - base);
+            if (od) od[rk[i]] = key_d2(kv[i]);
+          }
       }
       for (uint32_t s = cnt + (uint32_t)l; s < k; s += G) {
         oi[s] = -1;
@@ -418,16 +439,20 @@ static void launch_thread(const SearchArgs& a, bool knn, dim3 grid, hipStream_t 
 }
 
 template <int G, int KB>
-static void launch_group(const SearchArgs& a, bool knn, dim3 grid, hipStream_t st) {
+static void launch_group(const SearchArgs& a, bool knn, bool wide, dim3 grid, hipStream_t st) {
   const bool hier = a.nlevels > 0;
   if (knn && hier)
-    hipLaunchKernelGGL((k_topk_group<G, KB, true, true>), grid, dim3(kTileQ), 0, st, a);
+    hipLaunchKernelGGL((k_topk_group<G, KB, true, true, true>), grid, dim3(kTileQ), 0, st, a);
+  else if (knn && wide)
+    hipLaunchKernelGGL((k_topk_group<G, KB, true, false, true>), grid, dim3(kTileQ), 0, st, a);
   else if (knn)
-    hipLaunchKernelGGL((k_topk_group<G, KB, true, false>), grid, dim3(kTileQ), 0, st, a);
+    hipLaunchKernelGGL((k_topk_group<G, KB, true, false, false>), grid, dim3(kTileQ), 0, st, a);
   else if (hier)
-    hipLaunchKernelGGL((k_topk_group<G, KB, false, true>), grid, dim3(kTileQ), 0, st, a);
+    hipLaunchKernelGGL((k_topk_group<G, KB, false, true, true>), grid, dim3(kTileQ), 0, st, a);
+  else if (wide)
+    hipLaunchKernelGGL((k_topk_group<G, KB, false, false, true>), grid, dim3(kTileQ), 0, st, a);
   else
-    hipLaunchKernelGGL((k_topk_group<G, KB, false, false>), grid, dim3(kTileQ), 0, st, a);
+    hipLaunchKernelGGL((k_topk_group<G, KB, false, false, false>), grid, dim3(kTileQ), 0, st, a);
 }
 
 uint32_t max_k_for_group(int group_lanes) {
@@ -441,7 +466,7 @@ uint32_t max_k_for_group(int group_lanes) {
 }
 
 hipError_t launch_search(const SearchArgs& a0, int knn_mode, int group_lanes, hipStream_t st,
-                         int num_cus) {
+                         int num_cus, bool wide) {
   SearchArgs a = a0;
   if (a.nq_dev) a.persistent = 1;  // tile count known only on the device
   const uint32_t k = a.k;
@@ -470,17 +495,17 @@ hipError_t launch_search(const SearchArgs& a0, int knn_mode, int group_lanes, hi
     else if (k <= 32) launch_thread<32>(a, knn, grid, st);
     else launch_thread<64>(a, knn, grid, st);
   } else if (group_lanes == 4) {
-    if (k <= 16) launch_group<4, 16>(a, knn, grid, st);
-    else launch_group<4, 64>(a, knn, grid, st);
+    if (k <= 16) launch_group<4, 16>(a, knn, wide, grid, st);
+    else launch_group<4, 64>(a, knn, wide, grid, st);
   } else if (group_lanes == 16) {
-    if (k <= 16) launch_group<16, 16>(a, knn, grid, st);
-    else if (k <= 64) launch_group<16, 64>(a, knn, grid, st);
-    else launch_group<16, 256>(a, knn, grid, st);
+    if (k <= 16) launch_group<16, 16>(a, knn, wide, grid, st);
+    else if (k <= 64) launch_group<16, 64>(a, knn, wide, grid, st);
+    else launch_group<16, 256>(a, knn, wide, grid, st);
   } else {
-    if (k <= 16) launch_group<64, 16>(a, knn, grid, st);
-    else if (k <= 64) launch_group<64, 64>(a, knn, grid, st);
-    else if (k <= 256) launch_group<64, 256>(a, knn, grid, st);
-    else launch_group<64, 1024>(a, knn, grid, st);
+    if (k <= 16) launch_group<64, 16>(a, knn, wide, grid, st);
+    else if (k <= 64) launch_group<64, 64>(a, knn, wide, grid, st);
+    else if (k <= 256) launch_group<64, 256>(a, knn, wide, grid, st);
+    else launch_group<64, 1024>(a, knn, wide, grid, st);
   }
   return hipGetLastError();
 }

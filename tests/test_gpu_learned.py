@@ -171,3 +171,34 @@ def test_autotune_picks_a_valid_config(ctx):
         assert ms > 0
     finally:
         ctx.set_options()
+
+
+def test_trained_network_gain(ctx):
+    # What the trained network buys over the method without it.  Without the
+    # MLP every query is searched at r (relative search volume 1).  With it, a
+    # query is first searched at r' and re-run at r only if r' holds fewer
+    # than k neighbours, so its expected work is about (r'/r)^3 plus 1 for a
+    # miss.  Trained, that is well under half of the plain search and better
+    # than an untrained network; the exact result is checked elsewhere.
+    pts = rtnn_amd.gen_spheres(3, 30, 300000, nspheres=60, box=600.0)
+    qs = rtnn_amd.gen_spheres(3, 31, 50000, nspheres=60, box=600.0)
+    r, k = 10.0, 16
+    ctx.set_points(pts)
+    _, d2, _ = ctx.search(qs, mode="knn", radius=r, k=k)
+    dk = np.sqrt(d2[:, k - 1].astype(np.float64))  # inf where fewer than k within r
+
+    def work(rp):
+        covered = dk < rp
+        return float(np.mean((rp / r) ** 3) + np.mean(~covered)), float(np.mean(covered))
+
+    ctx.set_mlp_weights(rand_blob(12))
+    w0, c0 = work(ctx.propose_radius(qs, "learned", r, k))
+    ctx.mlp_train(r, k, n_samples=1 << 16, steps=600, batch=4096, seed=3)
+    rp = ctx.propose_radius(qs, "learned", r, k)
+    assert (rp > 0).all() and (rp <= r).all()
+    w1, c1 = work(rp)
+    wp, cp = work(ctx.propose_radius(qs, "partitioned", r, k))
+    print(f"relative work: trained {w1:.3f} (covers {c1:.2%}), untrained {w0:.3f} "
+          f"(covers {c0:.2%}), density partition {wp:.3f} (covers {cp:.2%}), plain 1.0")
+    assert w1 < 0.5 and c1 > 0.7
+    assert w1 < w0

@@ -80,10 +80,11 @@ def test_kernel_variants_and_cell_factors(ctx, qpw, cf):
     pts = rng.uniform(0, 20, (20000, 3)).astype(np.float32)
     qs = np.concatenate([pts[:3000], rng.uniform(-3, 23, (2000, 3)).astype(np.float32)])
     ctx.set_points(pts)
-    ctx.set_options(cell_factor=cf, queries_per_wave=qpw)
-    for mode in ("radius", "knn"):
-        for k in (1, 5, 16, 33, 64):
-            check(ctx, pts, qs, mode, 1.5, k)
+    for buf in ((0, 1) if qpw in (4, 16) else (0,)):
+        ctx.set_options(cell_factor=cf, queries_per_wave=qpw, group_buffer=buf)
+        for mode in ("radius", "knn"):
+            for k in (1, 5, 16, 33, 64):
+                check(ctx, pts, qs, mode, 1.5, k)
 
 
 def test_large_k_group_kernel(ctx):
@@ -103,10 +104,12 @@ def test_group_buffer_overflow_path(ctx):
     rng = np.random.default_rng(8)
     pts = rng.normal(0, 1.0, (60000, 3)).astype(np.float32)
     ctx.set_points(pts)
-    for qpw in (1, 4, 16):  # 64, 16 and 4 lanes per query
-        ctx.set_options(queries_per_wave=qpw)
-        check(ctx, pts, pts[:1500], "radius", 2.0, 16)
-        check(ctx, pts, pts[:1500], "knn", 2.0, 64)
+    for buf in (2, 1):  # wide (2k + G keys) and compact (k + 2G keys) buffers
+        for qpw in (1, 4, 16):  # 64, 16 and 4 lanes per query
+            ctx.set_options(queries_per_wave=qpw, group_buffer=buf)
+            check(ctx, pts, pts[:1500], "radius", 2.0, 16)
+            check(ctx, pts, pts[:1500], "knn", 2.0, 64)
+            check(ctx, pts, pts[:300], "radius", 2.0, 5)
 
 
 def test_knn_separate_queries_gaussian_mixture(ctx):
