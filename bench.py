@@ -189,6 +189,7 @@ def main():
     results = {}
     outputs = {}
     train = None
+    batched_opts = None
     for m in w["methods"]:
         if m == "plain":
             fn = lambda: ctx.search(dq, mode=mode, radius=r, k=k, out_device=True)
@@ -200,8 +201,7 @@ def main():
         elif m == "batched":
             fn = lambda: ctx.search_batched(dp, w["offsets"], mode=mode, radius=r, k=k,
                                             with_d2=False)
-        elif m == "graph":  # the batched pipeline captured once, replayed per step
-            graph = ctx.graph_batched(dp, w["offsets"], mode=mode, radius=r, k=k, with_d2=False)
+        elif m == "graph":  # the batched pipeline captured once (below), replayed per step
             fn = lambda: (graph.idx, graph.d2, graph.launch())
         else:
             t0 = time.perf_counter()
@@ -214,10 +214,22 @@ def main():
             fn = lambda: ctx.search_learned(dq, radius=r, k=k, out_device=True)
         ctx.set_options()
         tuned = None
+        if args.autotune and m in ("batched", "graph"):
+            if batched_opts is None:
+                t0 = time.perf_counter()
+                opt, tms = ctx.autotune_batched(dp, w["offsets"], mode=mode, radius=r, k=k)
+                batched_opts = (opt, round(tms, 4), round(time.perf_counter() - t0, 3))
+                log(f"[bench] batched: autotuned {batched_opts}")
+            ctx.set_options(**batched_opts[0])
+            tuned = dict(batched_opts[0], search_ms=batched_opts[1], seconds=batched_opts[2])
+        if m == "graph":  # capture with the options in force (tuned or default)
+            graph = ctx.graph_batched(dp, w["offsets"], mode=mode, radius=r, k=k, with_d2=False)
         if args.autotune and m in ("plain", "partitioned", "learned"):
             t0 = time.perf_counter()
             opt, tms = ctx.autotune(dq, method=m, mode=mode, radius=r, k=k)
             tuned = dict(opt, search_ms=round(tms, 4), seconds=round(time.perf_counter() - t0, 3))
+            if m == "learned":
+                tuned["radius_margin"] = round(float(ctx.mlp_weights()[-1]), 4)
             log(f"[bench] {m}: autotuned {tuned}")
         ms, agg, out = timed(fn, args.steps, args.warmup, m)
         results[m] = dict(ms_per_step=round(ms, 4), queries_per_s=round(nq / (ms * 1e-3), 1),

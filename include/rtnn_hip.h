@@ -221,9 +221,12 @@ int rtnn_boundary_pairs_batched(rtnn_ctx ctx, const float* xyz, const uint64_t* 
 #define RTNN_METHOD_PARTITIONED 1
 #define RTNN_METHOD_LEARNED     2
 
-/* Times every (cell_factor, queries_per_wave) candidate of the method on this
- * exact workload (same queries, r, k) and keeps the fastest in the context's
- * options (also returned in *best; *best_ms = its time per search).  Replaces
+/* Times every candidate configuration of the method on this exact workload
+ * (same queries, r, k): cell factor x queries per wave x group buffer, plus
+ * hierarchical grid levels x level alpha for the proposal methods, and for
+ * the learned method then the MLP's radius margin.  Keeps the fastest in the
+ * context's options (also returned in *best; *best_ms = its time per search)
+ * and, for the learned method, its margin in the context's weights.  Replaces
  * the reference's analytic cell-size and batching models (calcCRRatio,
  * util.cpp:551-676; autoBatchingRange/KNN, sort.cpp:221-320) with
  * measurement.  out_idx/out_d2 are scratch outputs of size nq x k (device;
@@ -231,6 +234,14 @@ int rtnn_boundary_pairs_batched(rtnn_ctx ctx, const float* xyz, const uint64_t* 
 int rtnn_autotune(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_device, int method,
                   int mode, float r, uint32_t k, int32_t* out_idx, float* out_d2,
                   rtnn_options* best, double* best_ms);
+/* The same for rtnn_search_batched (and so for graphs captured afterwards):
+ * cell edge r / {0.5, 1, 2} x queries per wave {1, 4, 16, 64} x the group
+ * kernel's LDS buffer (wide / compact), timed on this batch.  out_idx
+ * (required) and out_d2 (nullable) are device scratch of n x k. */
+int rtnn_autotune_batched(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets,
+                          uint32_t nclouds, int mode, float r, uint32_t k,
+                          int32_t* out_idx, float* out_d2,
+                          rtnn_options* best, double* best_ms);
 
 /* ---- building blocks (exported for parity tests; device pointers) ---- */
 

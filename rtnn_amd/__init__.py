@@ -112,6 +112,8 @@ SIGNATURES = {
                                          _I, ctypes.POINTER(_U64)]),
     "rtnn_autotune": (_I, [_VP, _VP, _U64, _I, _I, _I, _F, _U32, _VP, _VP,
                            ctypes.POINTER(Options), ctypes.POINTER(ctypes.c_double)]),
+    "rtnn_autotune_batched": (_I, [_VP, _VP, _VP, _U32, _I, _F, _U32, _VP, _VP,
+                                   ctypes.POINTER(Options), ctypes.POINTER(ctypes.c_double)]),
     "rtnn_radix_sort_pairs": (_I, [_VP, _VP, _VP, _U64, _U32]),
     "rtnn_exclusive_scan": (_I, [_VP, _VP, _VP, _U64]),
     "rtnn_morton3": (_I, [_VP, _VP, _VP, _U64]),
@@ -415,6 +417,25 @@ class Context:
                                           ctypes.byref(ms)))
         return ({"cell_factor": best.cell_factor, "queries_per_wave": best.queries_per_wave,
                  "grid_levels": best.grid_levels, "level_alpha": best.level_alpha,
+                 "group_buffer": best.group_buffer}, ms.value)
+
+    def autotune_batched(self, xyz, offsets, mode="radius", radius=0.2, k=64):
+        """Times the batched-search configurations on this batch and keeps the
+        fastest; returns (options dict, ms per search)."""
+        import torch
+        if not (_is_torch(xyz) and xyz.is_cuda):
+            raise ValueError("batched autotune takes a CUDA tensor of points")
+        xyz = xyz.to(torch.float32).contiguous()
+        offs = np.ascontiguousarray(offsets, dtype=np.uint64)
+        idx = torch.empty((int(offs[-1]), k), dtype=torch.int32, device=xyz.device)
+        best = Options()
+        ms = ctypes.c_double(0)
+        self._check(self._L.rtnn_autotune_batched(
+            self._h, ctypes.c_void_p(xyz.data_ptr()), ctypes.c_void_p(offs.ctypes.data),
+            len(offs) - 1, _mode(mode), radius, k, ctypes.c_void_p(idx.data_ptr()), None,
+            ctypes.byref(best), ctypes.byref(ms)))
+        del idx
+        return ({"cell_factor": best.cell_factor, "queries_per_wave": best.queries_per_wave,
                  "group_buffer": best.group_buffer}, ms.value)
 
     # MLP building blocks (device tensors)

@@ -1260,6 +1260,67 @@ int rtnn_autotune(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_device, i
     }
   }
   ctx->opt = best_opt;
+  if (method == RTNN_METHOD_LEARNED) {
+    // The MLP's radius margin trades first-stage volume against fallbacks;
+    // measured here with the chosen configuration (results are exact for any).
+    const float m0 = ctx->mlp.margin;
+    float best_m = m0;
+    for (float f : {0.85f, 1.15f, 1.3f, 1.5f}) {
+      ctx->mlp.margin = m0 * f;
+      rtnn_stats st{};
+      for (int rep = 0; rep < 2; ++rep) {
+        const int rc = proposal_search(ctx, Proposal::kLearned, qxyz, nq, on_device, r, k,
+                                       out_idx, out_d2, 1, &st);
+        if (rc != RTNN_OK) {
+          ctx->mlp.margin = m0;
+          return rc;
+        }
+      }
+      if (st.ms_total < best_t) {
+        best_t = st.ms_total;
+        best_m = ctx->mlp.margin;
+      }
+    }
+    ctx->mlp.margin = best_m;
+  }
+  if (best) *best = best_opt;
+  if (best_ms) *best_ms = best_t;
+  return RTNN_OK;
+}
+
+int rtnn_autotune_batched(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets,
+                          uint32_t nclouds, int mode, float r, uint32_t k, int32_t* out_idx,
+                          float* out_d2, rtnn_options* best, double* best_ms) {
+  if (!ctx) return RTNN_ERR_INVALID;
+  int rc = check_search_args(ctx, mode, r, k, out_idx);
+  if (rc) return rc;
+  const rtnn_options base = ctx->opt;
+  double best_t = 1e300;
+  rtnn_options best_opt = base;
+  // Cell edge r / {0.5, 1, 2} x lanes per query {64, 16, 4, 1} (queries per
+  // wave 1, 4, 16, 64) x the group kernel's LDS buffer (wide, compact).
+  for (float cf : {0.5f, 1.0f, 2.0f})
+    for (int qpw : {1, 4, 16, 64})
+      for (int buf : {2, 1}) {
+        const int gl = 64 / qpw;
+        if (k > max_k_for_group(gl) || (gl == 1 && buf == 1)) continue;
+        ctx->opt = base;
+        ctx->opt.cell_factor = cf;
+        ctx->opt.queries_per_wave = qpw;
+        ctx->opt.group_buffer = gl == 1 ? 0 : buf;
+        rtnn_stats st{};
+        for (int rep = 0; rep < 2 && rc == RTNN_OK; ++rep)
+          rc = rtnn_search_batched(ctx, xyz, offsets, nclouds, mode, r, k, out_idx, out_d2, &st);
+        if (rc != RTNN_OK) {
+          ctx->opt = base;
+          return rc;
+        }
+        if (st.ms_total < best_t) {
+          best_t = st.ms_total;
+          best_opt = ctx->opt;
+        }
+      }
+  ctx->opt = best_opt;
   if (best) *best = best_opt;
   if (best_ms) *best_ms = best_t;
   return RTNN_OK;

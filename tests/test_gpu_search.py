@@ -302,3 +302,15 @@ def test_config3_full_size_sampled(ctx):
     valid = idx >= 0
     assert (valid[:, 1:] <= valid[:, :-1]).all()  # sentinels only at the tail
     assert (d2[valid] > 0).all() and (d2[valid] < np.float32(100)).all()
+
+
+def test_autotune_batched_keeps_result(ctx):
+    import torch
+    xyz, offs = rtnn_amd.gen_lattice_clouds(13, 12, 500, 4000)
+    t = torch.from_numpy(xyz).cuda()
+    want_i, want_d, _ = ctx.search_batched(t, offs, mode="radius", radius=0.2, k=64)
+    opt, ms = ctx.autotune_batched(t, offs, mode="radius", radius=0.2, k=64)
+    assert ms > 0 and opt["cell_factor"] in (0.5, 1.0, 2.0)
+    assert opt["queries_per_wave"] in (1, 4, 16, 64)
+    got_i, got_d, _ = ctx.search_batched(t, offs, mode="radius", radius=0.2, k=64)
+    assert torch.equal(got_i, want_i) and torch.equal(got_d, want_d)
