@@ -34,6 +34,25 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def pmc_traffic(workload_name, kernel):
+    """HBM bytes per launch of `kernel` on this workload, from the committed
+    rocprofv3 counter summary profiles/hbm_traffic.md (FETCH_SIZE and
+    WRITE_SIZE per dispatch, each scaled by what the counter reads for a copy
+    kernel of known size).  Rows: | workload | kernel | dispatches | FETCH KiB |
+    WRITE KiB | bytes per launch |.  None if the summary has no such row."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cols = [c.strip().strip("`") for c in line.strip().strip("|").split("|")]
+        if len(cols) == 6 and cols[0] == workload_name and cols[1] == kernel:
+            try:
+                return float(cols[5])
+            except ValueError:
+                return None
+    return None
+
+
 def workload(name):
     import rtnn_amd
     if name == "c3":
@@ -59,7 +78,7 @@ def workload(name):
     if name == "c5":
         return dict(pts=rtnn_amd.gen_levy(5, 30_000_000, alpha=1.5, min_step=0.01, box=1000.0),
                     qs=rtnn_amd.gen_uniform(51, 30_000_000, 0.0, 1000.0), mode="knn", r=100.0,
-                    k=32, methods=["partitioned", "learned"],
+                    k=32, methods=["partitioned", "learned"], train_on_queries=True,
                     cfg={"workload": "KNN K=32 r=100: 30M Levy-flight points (Pareto alpha 1.5, "
                                      "min step 0.01, periodic [0,1000]^3), 30M uniform queries",
                          "points": 30_000_000, "queries": 30_000_000, "k": 32, "radius": 100.0})
@@ -205,8 +224,10 @@ def main():
             fn = lambda: (graph.idx, graph.d2, graph.launch())
         else:
             t0 = time.perf_counter()
+            # uniform queries over clustered points: train on a sample of the queries
+            tq = dq[:args.train_samples] if w.get("train_on_queries") else None
             loss = ctx.mlp_train(r, k, n_samples=args.train_samples, steps=args.train_steps,
-                                 batch=4096, lr=1e-3, tau=0.9, seed=1)
+                                 batch=4096, lr=1e-3, tau=0.9, seed=1, queries=tq)
             train = {"seconds": round(time.perf_counter() - t0, 3), "final_pinball_loss": loss,
                      "samples": args.train_samples, "steps": args.train_steps, "batch": 4096,
                      "lr": 1e-3, "tau": 0.9}
@@ -265,8 +286,9 @@ def main():
         algo_def = "range bytes per query 12 + 12*C27(q) + 4*K"
     achieved = algo / (kms * 1e-3) / 1e9
     kname = "k_topk_thread" if results[kroof]["lanes_per_query"] == 1 else "k_topk_group"
+    traffic = pmc_traffic(args.workload, kname)
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": kname,
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "kernel": kname,
             "kernel_ms": round(kms, 4), "algorithmic_bytes": algo, "bytes_definition": algo_def}
     if c27_note:
         roof["c27"] = c27_note

@@ -111,6 +111,17 @@ int rtnn_search(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_device,
                 int32_t* out_idx, float* out_d2, int out_on_device,
                 rtnn_stats* stats);
 
+/* Stream-ordered form of rtnn_search for device-resident queries (qxyz, or
+ * NULL for the points) and outputs: the grid build, query ordering and search
+ * are enqueued on `stream` (a hipStream_t) and the call returns without
+ * waiting.  The work is ordered after the context's earlier calls, and the
+ * context's later calls are ordered after it (event waits between the two
+ * streams), so the context may be used again at once; the caller synchronises
+ * `stream` before reading the outputs.  Replaces the reference's per-batch
+ * stream launches (search.cpp:39, util.cpp:708-709). */
+int rtnn_search_async(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int mode, float r,
+                      uint32_t k, int32_t* out_idx, float* out_d2, void* stream);
+
 /* KNN with the reference's exact density partition (sort.cpp:140-219,346-391
  * with approx mode 0): per query cell the smallest megacell holding >= k+1
  * points gives a launch radius min(r, sqrt(3)*(iter+1)*cell); queries whose
@@ -124,12 +135,20 @@ int rtnn_search_partitioned(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on
 /* ---- learned radius (KNN) --------------------------------------------- */
 
 /* Trains the radius-proposal MLP (30->64->64->1, GELU, BF16 weights, FP32
- * accumulation) on exact k-th neighbor distances of n_samples queries drawn
- * from the context's points, computed by the plain search at radius r.
+ * accumulation) on exact k-th neighbor distances (within r) of n_samples
+ * queries drawn from the context's points.
  * Pinball loss with quantile tau; Adam.  Runs entirely on the GPU. */
 int rtnn_mlp_train(rtnn_ctx ctx, float r, uint32_t k, uint32_t n_samples,
                    uint32_t steps, uint32_t batch, float lr, float tau,
                    uint64_t seed, float* final_loss);
+/* The same, trained on the caller's sample of queries (nq of them; device
+ * pointer if on_device), for query distributions unlike the points (e.g.
+ * uniform queries over clustered points).  The exact k-th distances of the
+ * training queries come from the partitioned search (exact, and cheap when r
+ * is large). */
+int rtnn_mlp_train_queries(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_device,
+                           float r, uint32_t k, uint32_t steps, uint32_t batch,
+                           float lr, float tau, uint64_t seed, float* final_loss);
 /* Weights as FP32, row-major [out][in]:
  *   [w1 64x30 | b1 64 | w2 64x64 | b2 64 | w3 64 | b3 1]  (6209 parameters)
  * followed by [count_scale, radius_margin] (feature scale of the log1p counts,

@@ -89,10 +89,13 @@ SIGNATURES = {
     "rtnn_set_points": (_I, [_VP, _VP, _U64, _I]),
     "rtnn_search": (_I, [_VP, _VP, _U64, _I, _I, _F, _U32, _VP, _VP, _I,
                          ctypes.POINTER(Stats)]),
+    "rtnn_search_async": (_I, [_VP, _VP, _U64, _I, _F, _U32, _VP, _VP, _VP]),
     "rtnn_search_partitioned": (_I, [_VP, _VP, _U64, _I, _F, _U32, _VP, _VP, _I,
                                      ctypes.POINTER(Stats)]),
     "rtnn_mlp_train": (_I, [_VP, _F, _U32, _U32, _U32, _U32, _F, _F, _U64,
                             ctypes.POINTER(_F)]),
+    "rtnn_mlp_train_queries": (_I, [_VP, _VP, _U64, _I, _F, _U32, _U32, _U32, _F, _F, _U64,
+                                    ctypes.POINTER(_F)]),
     "rtnn_mlp_get_weights": (_I, [_VP, _VP]),
     "rtnn_mlp_set_weights": (_I, [_VP, _VP]),
     "rtnn_search_learned": (_I, [_VP, _VP, _U64, _I, _F, _U32, _VP, _VP, _I,
@@ -275,6 +278,26 @@ class Context:
         return self._run(self._L.rtnn_search, queries, radius, k, with_d2, out_device,
                          extra=(_mode(mode),))
 
+    def search_async(self, queries, idx, d2=None, mode="radius", radius=2.0, stream=None):
+        """Enqueues a plain search of device tensor `queries` (None: the points)
+        into device tensors idx [nq, k] (and d2) on `stream` (a torch CUDA
+        stream; default: torch's current stream) and returns at once."""
+        import torch
+        st = stream if stream is not None else torch.cuda.current_stream()
+        if queries is None:
+            qptr, nq = None, len(self._points)
+        else:
+            q = _as_points(queries)
+            if not (_is_torch(q) and q.is_cuda):
+                raise ValueError("search_async takes device tensors")
+            qptr, nq = ctypes.c_void_p(q.data_ptr()), len(q)
+            self._async_keep = q
+        k = idx.shape[1]
+        self._check(self._L.rtnn_search_async(
+            self._h, qptr, nq, _mode(mode), radius, k, ctypes.c_void_p(idx.data_ptr()),
+            ctypes.c_void_p(d2.data_ptr()) if d2 is not None else None,
+            ctypes.c_void_p(st.cuda_stream)))
+
     def search_partitioned(self, queries=None, radius=2.0, k=8, with_d2=True,
                            out_device=None):
         return self._run(self._L.rtnn_search_partitioned, queries, radius, k, with_d2,
@@ -298,10 +321,19 @@ class Context:
         return out
 
     def mlp_train(self, radius, k, n_samples=1 << 21, steps=2000, batch=4096, lr=1e-3,
-                  tau=0.9, seed=1):
+                  tau=0.9, seed=1, queries=None):
+        """Trains the radius MLP on n_samples queries drawn from the points, or
+        on `queries` when given; returns the final pinball loss."""
         loss = ctypes.c_float(0)
-        self._check(self._L.rtnn_mlp_train(self._h, radius, k, n_samples, steps, batch, lr, tau,
-                                           seed, ctypes.byref(loss)))
+        if queries is None:
+            self._check(self._L.rtnn_mlp_train(self._h, radius, k, n_samples, steps, batch, lr,
+                                               tau, seed, ctypes.byref(loss)))
+        else:
+            q = _as_points(queries)
+            qptr, qdev = _ptr(q)
+            self._check(self._L.rtnn_mlp_train_queries(self._h, qptr, len(q), qdev, radius, k,
+                                                       steps, batch, lr, tau, seed,
+                                                       ctypes.byref(loss)))
         return float(loss.value)
 
     def mlp_weights(self):

@@ -84,6 +84,7 @@ struct rtnn_ctx_s {
   // queries / outputs
   DevBuf qin, qkeys, qvals, qsorted, oidx, od2, shortrows, misc, tiles, offs, bboxes;
   hipEvent_t ev[8] = {};
+  hipEvent_t ev_join = nullptr;  // orders rtnn_search_async work with the context stream
 
   // radius proposals (learned / partitioned)
   DevBuf pgrid, pcounts, qradius;
@@ -107,6 +108,7 @@ struct rtnn_ctx_s {
     mlp_release(mlp);
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e);
+    if (ev_join) (void)hipEventDestroy(ev_join);
     if (st) (void)hipStreamDestroy(st);
   }
 };
@@ -458,6 +460,7 @@ int rtnn_create(int device, rtnn_ctx* out) {
     return RTNN_ERR_HIP;
   }
   for (auto& ev : c->ev) (void)hipEventCreate(&ev);
+  (void)hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming);
   int cus = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
       cus > 0)
@@ -534,7 +537,7 @@ int rtnn_set_points(rtnn_ctx ctx, const float* xyz, uint64_t n, int on_device) {
 
 static int search_impl(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_device, int mode,
                        float r, uint32_t k, int32_t* out_idx, float* out_d2, int out_on_device,
-                       rtnn_stats* stats) {
+                       rtnn_stats* stats, bool wait = true) {
   int rc = check_search_args(ctx, mode, r, k, out_idx);
   if (rc) return rc;
   if (ctx->n == 0) return fail(ctx, RTNN_ERR_STATE, "rtnn_set_points has not been called");
@@ -549,7 +552,7 @@ static int search_impl(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_devi
     return fail(ctx, RTNN_ERR_UNSUPPORTED, "k=%u exceeds the supported maximum (1024)", k);
   }
   CK(hipSetDevice(ctx->device));
-  const bool timing = ctx->opt.time_stages != 0 || stats != nullptr;
+  const bool timing = wait && (ctx->opt.time_stages != 0 || stats != nullptr);
   if (timing) CK(hipEventRecord(ctx->ev[0], ctx->st));
   float f = ctx->opt.cell_factor > 0 ? ctx->opt.cell_factor : default_cell_factor(mode);
   rc = build_grid(ctx, r / f);
@@ -567,6 +570,7 @@ static int search_impl(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_devi
   if (timing) CK(hipEventRecord(ctx->ev[3], ctx->st));
   rc = copy_out(ctx, nq, k, out_idx, out_d2, out_on_device, o);
   if (rc) return rc;
+  if (!wait) return RTNN_OK;
   if (timing) CK(hipEventRecord(ctx->ev[4], ctx->st));
   CK(hipStreamSynchronize(ctx->st));
   if (stats) {
@@ -587,6 +591,27 @@ int rtnn_search(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_device, int
                 rtnn_stats* stats) {
   return search_impl(ctx, qxyz, nq, on_device, mode, r, k, out_idx, out_d2, out_on_device,
                      stats);
+}
+
+int rtnn_search_async(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int mode, float r,
+                      uint32_t k, int32_t* out_idx, float* out_d2, void* stream) {
+  if (!ctx) return RTNN_ERR_INVALID;
+  CK(hipSetDevice(ctx->device));
+  hipStream_t us = static_cast<hipStream_t>(stream);
+  hipStream_t own = ctx->st;
+  if (us == own) return search_impl(ctx, qxyz, nq, 1, mode, r, k, out_idx, out_d2, 1, nullptr,
+                                    false);
+  // The caller's stream waits for the context's earlier work; the context's
+  // stream waits for this work, so later calls on the context are ordered
+  // after it without a host synchronisation.
+  CK(hipEventRecord(ctx->ev_join, own));
+  CK(hipStreamWaitEvent(us, ctx->ev_join, 0));
+  ctx->st = us;
+  const int rc = search_impl(ctx, qxyz, nq, 1, mode, r, k, out_idx, out_d2, 1, nullptr, false);
+  ctx->st = own;
+  CK(hipEventRecord(ctx->ev_join, us));
+  CK(hipStreamWaitEvent(own, ctx->ev_join, 0));
+  return rc;
 }
 
 }  // extern "C"
@@ -798,36 +823,34 @@ int rtnn_propose_radius(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_dev
 
 // ------------------------------------------------------------------ MLP --
 
-int rtnn_mlp_train(rtnn_ctx ctx, float r, uint32_t k, uint32_t n_samples, uint32_t steps,
-                   uint32_t batch, float lr, float tau, uint64_t seed, float* final_loss) {
-  if (!ctx) return RTNN_ERR_INVALID;
-  if (ctx->n == 0) return fail(ctx, RTNN_ERR_STATE, "rtnn_set_points has not been called");
-  if (!(r > 0.0f) || k == 0 || k > 64 || n_samples == 0 || batch == 0 || batch > (1u << 20) ||
-      !(lr > 0.0f) || !(tau > 0.0f && tau < 1.0f))
-    return fail(ctx, RTNN_ERR_INVALID, "invalid training arguments");
-  CK(hipSetDevice(ctx->device));
-  const uint64_t ns = n_samples;
-  // 1. training queries drawn from the points, their exact k-th distances
-  CK(ctx->tr_q.ensure(12 * ns));
-  CK(mlp_sample_points(ctx->pts.as<float>(), ctx->n, ns, seed, ctx->tr_q.as<float>(), ctx->st));
+}  // extern "C"
+
+namespace {
+
+// Trains on the ns training queries already in ctx->tr_q: their exact k-th
+// distances (the partitioned search: exact, and far cheaper than the plain
+// search when r is large), features on the edge-r count grid, Adam on the
+// pinball loss, loss on the first 65536 samples.
+int train_impl(rtnn_ctx ctx, uint64_t ns, float r, uint32_t k, uint32_t steps, uint32_t batch,
+               float lr, float tau, uint64_t seed, float* final_loss) {
   CK(ctx->tr_idx.ensure(4 * ns * k));
   CK(ctx->tr_d2.ensure(4 * ns * k));
   rtnn_options saved = ctx->opt;
   ctx->opt.query_order = 0;
-  int rc = search_impl(ctx, ctx->tr_q.as<float>(), ns, 1, RTNN_MODE_KNN, r, k,
-                       ctx->tr_idx.as<int32_t>(), ctx->tr_d2.as<float>(), 1, nullptr);
+  int rc = proposal_search(ctx, Proposal::kPartition, ctx->tr_q.as<float>(), ns, 1, r, k,
+                           ctx->tr_idx.as<int32_t>(), ctx->tr_d2.as<float>(), 1, nullptr);
   ctx->opt = saved;
   if (rc) return rc;
   CK(ctx->tr_t.ensure(4 * ns));
   CK(mlp_targets(ctx->tr_d2.as<float>(), k, ns, r, ctx->tr_t.as<float>(), ctx->st));
-  // 2. features on the edge-r count grid
+  // features on the edge-r count grid
   rc = build_count_grid(ctx, r, false);
   if (rc) return rc;
   CK(ctx->tr_f.ensure(4ull * kMlpFP * ns));
   CK(mlp_features(ctx->tr_q.as<float>(), ns, ctx->pgrid.as<GridDesc>(),
                   ctx->pcounts.as<uint32_t>(), ctx->mlp.count_scale, ctx->tr_f.as<float>(),
                   ctx->st));
-  // 3. Adam on the pinball loss
+  // Adam on the pinball loss
   const uint32_t nb = (batch + kGradBlock - 1) / kGradBlock;
   const uint32_t eval_n = (uint32_t)std::min<uint64_t>(ns, 1u << 16);
   const uint32_t eval_nb = (eval_n + kGradBlock - 1) / kGradBlock;
@@ -843,7 +866,7 @@ int rtnn_mlp_train(rtnn_ctx ctx, float r, uint32_t k, uint32_t n_samples, uint32
                 1.0f / (float)batch, ctx->mlp.partial, ctx->st));
     CK(mlp_adam(ctx->mlp, nb, lr, (int)s, ctx->st));
   }
-  // 4. loss on the first eval_n samples
+  // loss on the first eval_n samples
   CK(ctx->tr_out.ensure(4 * (kMlpNP + 8)));
   CK(mlp_grad(ctx->mlp, ctx->tr_f.as<float>(), ctx->tr_t.as<float>(), ns, eval_n, 0, tau,
               1.0f / (float)eval_n, ctx->mlp.partial, ctx->st));
@@ -855,6 +878,46 @@ int rtnn_mlp_train(rtnn_ctx ctx, float r, uint32_t k, uint32_t n_samples, uint32
   if (final_loss) *final_loss = loss;
   ctx->mlp.ready = true;
   return RTNN_OK;
+}
+
+int check_train_args(rtnn_ctx ctx, float r, uint32_t k, uint64_t n_samples, uint32_t batch,
+                     float lr, float tau) {
+  if (ctx->n == 0) return fail(ctx, RTNN_ERR_STATE, "rtnn_set_points has not been called");
+  if (!(r > 0.0f) || k == 0 || k > 64 || n_samples == 0 || n_samples >= 0xffffffffull ||
+      batch == 0 || batch > (1u << 20) || !(lr > 0.0f) || !(tau > 0.0f && tau < 1.0f))
+    return fail(ctx, RTNN_ERR_INVALID, "invalid training arguments");
+  return RTNN_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rtnn_mlp_train(rtnn_ctx ctx, float r, uint32_t k, uint32_t n_samples, uint32_t steps,
+                   uint32_t batch, float lr, float tau, uint64_t seed, float* final_loss) {
+  if (!ctx) return RTNN_ERR_INVALID;
+  int rc = check_train_args(ctx, r, k, n_samples, batch, lr, tau);
+  if (rc) return rc;
+  CK(hipSetDevice(ctx->device));
+  const uint64_t ns = n_samples;
+  // training queries drawn from the points
+  CK(ctx->tr_q.ensure(12 * ns));
+  CK(mlp_sample_points(ctx->pts.as<float>(), ctx->n, ns, seed, ctx->tr_q.as<float>(), ctx->st));
+  return train_impl(ctx, ns, r, k, steps, batch, lr, tau, seed, final_loss);
+}
+
+int rtnn_mlp_train_queries(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_device, float r,
+                           uint32_t k, uint32_t steps, uint32_t batch, float lr, float tau,
+                           uint64_t seed, float* final_loss) {
+  if (!ctx) return RTNN_ERR_INVALID;
+  if (!qxyz) return fail(ctx, RTNN_ERR_INVALID, "qxyz is NULL");
+  int rc = check_train_args(ctx, r, k, nq, batch, lr, tau);
+  if (rc) return rc;
+  CK(hipSetDevice(ctx->device));
+  CK(ctx->tr_q.ensure(12 * nq));
+  CK(hipMemcpyAsync(ctx->tr_q.p, qxyz, 12 * nq,
+                    on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, ctx->st));
+  return train_impl(ctx, nq, r, k, steps, batch, lr, tau, seed, final_loss);
 }
 
 int rtnn_mlp_get_weights(rtnn_ctx ctx, float* blob) {

@@ -170,9 +170,12 @@ __global__ __launch_bounds__(kTileQ) void k_topk_thread(const SearchArgs a) {
         e = a.cell_start[rowc + hi0 + 1];
       }
     };
+    // The next run's bounds are loaded while the current run is scanned.
+    uint32_t s_next = 0, e_next = 0;
+    if (nrows > 0) row_bounds(0, s_next, e_next);
     for (int rr = 0; rr < nrows; ++rr) {
-      uint32_t s, e;
-      row_bounds(rr, s, e);
+      const uint32_t s = s_next, e = e_next;
+      if (rr + 1 < nrows) row_bounds(rr + 1, s_next, e_next);
       for (uint32_t j0 = s; j0 < e; j0 += kUnroll) {
         // kUnroll loads in flight before the first test
         float4 pv[kUnroll];

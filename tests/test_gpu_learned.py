@@ -202,3 +202,15 @@ def test_trained_network_gain(ctx):
           f"(covers {c0:.2%}), density partition {wp:.3f} (covers {cp:.2%}), plain 1.0")
     assert w1 < 0.5 and c1 > 0.7
     assert w1 < w0
+
+
+def test_train_on_query_sample_exact(ctx):
+    # Uniform queries over clustered points: the network is trained on a
+    # sample of the queries themselves; the learned search stays exact.
+    pts = rtnn_amd.gen_levy(5, 200000, alpha=1.5, min_step=0.01, box=100.0)
+    qs = rtnn_amd.gen_uniform(51, 40000, 0.0, 100.0)
+    ctx.set_points(pts)
+    loss = ctx.mlp_train(15.0, 8, steps=300, batch=2048, seed=5, queries=qs[:20000])
+    assert np.isfinite(loss)
+    _plain_vs(ctx, qs, 15.0, 8, ctx.search_learned)
+    _plain_vs(ctx, qs, 15.0, 8, ctx.search_partitioned)

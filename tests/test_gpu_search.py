@@ -314,3 +314,24 @@ def test_autotune_batched_keeps_result(ctx):
     assert opt["queries_per_wave"] in (1, 4, 16, 64)
     got_i, got_d, _ = ctx.search_batched(t, offs, mode="radius", radius=0.2, k=64)
     assert torch.equal(got_i, want_i) and torch.equal(got_d, want_d)
+
+
+def test_search_async_on_a_torch_stream(ctx):
+    # Enqueued on a caller's stream; the context's next (synchronous) call is
+    # ordered after it without any host synchronisation in between.
+    import torch
+    pts = rtnn_amd.gen_uniform(17, 60000, 0.0, 30.0)
+    qs = rtnn_amd.gen_uniform(18, 20000, -2.0, 32.0)
+    tp, tq = torch.from_numpy(pts).cuda(), torch.from_numpy(qs).cuda()
+    ctx.set_points(tp)
+    want_i, want_d, _ = ctx.search(tq, mode="knn", radius=1.5, k=12)
+    s = torch.cuda.Stream()
+    idx = torch.full((len(qs), 12), -7, dtype=torch.int32, device="cuda")
+    d2 = torch.zeros((len(qs), 12), dtype=torch.float32, device="cuda")
+    ctx.search_async(tq, idx, d2, mode="knn", radius=1.5, stream=s)
+    # the context's next call runs after the async search on the other stream
+    ri, rd, _ = ctx.search(None, mode="radius", radius=1.0, k=8)
+    s.synchronize()
+    assert torch.equal(idx, want_i) and torch.equal(d2, want_d)
+    wi, wd = oracle.grid_search(pts, pts, oracle.MODE_RADIUS, 1.0, 8, threads=8)
+    assert np.array_equal(ri.cpu().numpy(), wi) and np.array_equal(rd.cpu().numpy(), wd)
