@@ -22,6 +22,19 @@ extern "C" {
  * blank lines are skipped, and an unreadable file is an error (-2) instead of
  * assert(0). */
 int rtnn_read_points(const char* path, float* xyz, uint64_t* n);
+/* Reads a file of point sets: every line holds the same count of
+ * comma-separated numbers, d (set by the first line, rounded up to a multiple
+ * of 3); triple b of line i is point i of set b (util.cpp:18-99, tokenize and
+ * read_pc_data(..., int* d): each number parsed with stof, decimal -> float
+ * directly).  Two calls: with xyz == NULL it counts lines into *n and sets
+ * into *nsets; then, with *n as returned and xyz of 3 * nsets * n floats, it
+ * fills xyz set-major ([set][line][3]).  Differences, on purpose: a line with
+ * another count of numbers than the first, or a token that is not a number,
+ * is an error (-1, *n = its 1-based line number) where the reference reads
+ * past its vector or throws; a first line whose count is not a multiple of 3
+ * is padded with zeros; blank lines are skipped; -2 unreadable file, -4 the
+ * file changed between the calls. */
+int rtnn_read_point_sets(const char* path, float* xyz, uint64_t* n, uint32_t* nsets);
 /* Writes "x,y,z" lines with 9 significant digits (round-trips FP32). */
 int rtnn_write_points(const char* path, const float* xyz, uint64_t n);
 

@@ -126,6 +126,8 @@ SIGNATURES = {
     "rtnn_version": (ctypes.c_char_p, []),
     "rtnn_read_points": (_I, [ctypes.c_char_p, _VP, ctypes.POINTER(_U64)]),
     "rtnn_write_points": (_I, [ctypes.c_char_p, _VP, _U64]),
+    "rtnn_read_point_sets": (_I, [ctypes.c_char_p, _VP, ctypes.POINTER(_U64),
+                                  ctypes.POINTER(_U32)]),
     "rtnn_gen_uniform": (_I, [_U64, _U64, _F, _F, _VP]),
     "rtnn_gen_gaussian_mixture": (_I, [_U64, _U64, _U64, _U32, _F, _F, _F, _VP]),
     "rtnn_gen_spheres": (_I, [_U64, _U64, _U64, _U32, _F, _F, _F, _F, _VP]),
@@ -530,6 +532,26 @@ def read_points(path):
     rc = L.rtnn_read_points(path.encode(), ctypes.c_void_p(out.ctypes.data), ctypes.byref(n))
     if rc == -1:
         raise ValueError(f"{path}: line {n.value} does not hold three numbers")
+    if rc != 0:
+        raise ValueError(f"cannot read {path} (code {rc})")
+    return out
+
+
+def read_point_sets(path):
+    """File of point sets (each line: d numbers = d/3 points, one per set) ->
+    float32 [nsets, n, 3] (reference tokenize/read_pc_data(..., int* d))."""
+    L = lib()
+    n, ns = ctypes.c_uint64(0), ctypes.c_uint32(0)
+    rc = L.rtnn_read_point_sets(path.encode(), None, ctypes.byref(n), ctypes.byref(ns))
+    if rc == -2:
+        raise FileNotFoundError(path)
+    if rc == -1:
+        raise ValueError(f"{path}: line {n.value} is malformed or has another count of numbers")
+    if rc != 0:
+        raise ValueError(f"cannot read {path} (code {rc})")
+    out = np.empty((ns.value, n.value, 3), dtype=np.float32)
+    rc = L.rtnn_read_point_sets(path.encode(), ctypes.c_void_p(out.ctypes.data),
+                                ctypes.byref(n), ctypes.byref(ns))
     if rc != 0:
         raise ValueError(f"cannot read {path} (code {rc})")
     return out

@@ -96,6 +96,61 @@ int rtnn_read_points(const char* path, float* xyz, uint64_t* n) {
   return rc;
 }
 
+int rtnn_read_point_sets(const char* path, float* xyz, uint64_t* n, uint32_t* nsets) {
+  if (!path || !n || !nsets) return -3;
+  const uint64_t n_in = *n;  // second call: the line count of the first
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return -2;
+  std::vector<char> line(1 << 16);
+  std::vector<float> vals;
+  uint64_t count = 0;
+  size_t first = 0;  // numbers on the first line
+  uint32_t sets = 0;
+  int rc = 0;
+  while (std::fgets(line.data(), (int)line.size(), f)) {
+    const char* c = line.data();
+    while (*c == ' ' || *c == '\t' || *c == '\r' || *c == '\n') ++c;
+    if (*c == '\0') continue;  // blank line
+    // comma-separated numbers, each parsed straight to float (stof)
+    vals.clear();
+    char* p = line.data();
+    for (;;) {
+      char* end = nullptr;
+      errno = 0;
+      const float v = std::strtof(p, &end);
+      if (end == p) { rc = -1; break; }
+      vals.push_back(v);
+      p = end;
+      while (*p == ' ' || *p == '\t') ++p;
+      if (*p == ',') { ++p; continue; }
+      while (*p == ' ' || *p == '\t' || *p == '\r' || *p == '\n') ++p;
+      if (*p != '\0') rc = -1;
+      break;
+    }
+    if (rc == 0 && count == 0) {
+      first = vals.size();
+      sets = (uint32_t)((first + 2) / 3);
+    } else if (rc == 0 && vals.size() != first) {
+      rc = -1;
+    }
+    if (rc) { *n = count + 1; break; }
+    if (xyz) {
+      if (count >= n_in) { rc = -4; break; }  // the file grew since the first call
+      vals.resize(3 * (size_t)sets, 0.0f);  // a short first line is padded with zeros
+      for (uint32_t b = 0; b < sets; ++b)
+        for (int a = 0; a < 3; ++a) xyz[(b * n_in + count) * 3 + a] = vals[3 * b + a];
+    }
+    ++count;
+  }
+  std::fclose(f);
+  if (rc == 0) {
+    if (xyz && count != n_in) return -4;  // the file changed between the two calls
+    *n = count;
+    *nsets = sets;
+  }
+  return rc;
+}
+
 int rtnn_write_points(const char* path, const float* xyz, uint64_t n) {
   FILE* f = std::fopen(path, "wb");
   if (!f) return -2;

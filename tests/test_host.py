@@ -53,6 +53,36 @@ def test_read_points_matches_reference_parse(tmp_path):
     assert rtnn_amd.read_points(str(empty)).shape == (0, 3)
 
 
+def test_read_point_sets(tmp_path):
+    # Each line: 3 * m numbers = point i of m sets; every number parsed
+    # straight to float (stof), unlike the single-set reader's double.
+    p = tmp_path / "sets.txt"
+    p.write_text("0.1,2,3,4,5,6\n7, 8 ,9,10,11,0.3\n\n1e-3,-2,3.5,4,5,6\n")
+    got = rtnn_amd.read_point_sets(str(p))
+    assert got.shape == (2, 3, 3)
+    want0 = np.array([["0.1", "2", "3"], ["7", "8", "9"], ["1e-3", "-2", "3.5"]]).astype(np.float32)
+    want1 = np.array([[4, 5, 6], [10, 11, np.float32("0.3")], [4, 5, 6]], np.float32)
+    assert np.array_equal(got[0], want0) and np.array_equal(got[1], want1)
+    # decimal -> float directly: 0.1 parsed once, not via double
+    assert got[0, 0, 0] == np.float32("0.1")
+    # a first line whose count is not a multiple of 3 is padded with zeros
+    q = tmp_path / "pad.txt"
+    q.write_text("1,2,3,4\n5,6,7,8\n")
+    got = rtnn_amd.read_point_sets(str(q))
+    assert got.shape == (2, 2, 3) and got[1].tolist() == [[4, 0, 0], [8, 0, 0]]
+    # another count of numbers, or a non-number, is an error naming the line
+    for text in ("1,2,3,4,5,6\n1,2,3\n", "1,2,3\n4,x,6\n"):
+        bad = tmp_path / "bad.txt"
+        bad.write_text(text)
+        with pytest.raises(ValueError, match="line 2"):
+            rtnn_amd.read_point_sets(str(bad))
+    # one set = the single-set file format
+    src = os.path.join(ROOT, "tests", "golden", "samplepc.txt")
+    one = rtnn_amd.read_point_sets(src)
+    assert one.shape == (1, 8, 3)
+    assert np.allclose(one[0], rtnn_amd.read_points(src), rtol=1e-7, atol=0)
+
+
 def test_write_read_roundtrip(tmp_path):
     x = rtnn_amd.gen_uniform(3, 1000, -7.0, 1e4)
     p = str(tmp_path / "x.txt")
