@@ -14,12 +14,14 @@
 // lists the pairs whose distance lies within 1e-6 r of r.  Of the reference's
 // scheduling flags, -cr sets the cell factor (cell edge = r / crRatio, as
 // sort.cpp:80) and -ac 1 runs the autotuner (measured instead of modelled,
-// util.cpp:551-676); the others are accepted and do not change results.
+// util.cpp:551-676), -fq 1 reports the queries beyond the points' box + r
+// (their rows stay, empty); the others are accepted and do not change results.
 //
 // Output file (little endian): "RTNNRES1", u64 nq, u32 k, u32 flags (bit 0:
 // d2 present), then int32 idx[nq][k] (empty slot = -1), then float d2[nq][k].
 //
 // All GPU work goes through the C ABI of librtnn_hip.so (include/rtnn_hip.h).
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -50,6 +52,7 @@ struct Args {
   bool autotune = false;  // -ac 1: measure the grid configurations first
   int qpw = 0;
   std::string boundary;  // --boundary: boundary-pair report file
+  bool filter_queries = false;  // -fq 1: report queries beyond the points' box + r
 };
 
 struct Timer {
@@ -127,8 +130,10 @@ Args parse(int argc, char** argv) {
         std::fprintf(stderr, "crStep has to be greater than 1.\n");
         usage(argv[0], 1);
       }
+    } else if (s == "--filterQueries" || s == "-fq") {
+      a.filter_queries = std::atoi(need(i)) != 0;
     } else if (s == "--msr" || s == "-m" || s == "--deferFree" || s == "-df" ||
-               s == "--filterQueries" || s == "-fq" || s == "--numbatch" || s == "-nb" ||
+               s == "--numbatch" || s == "-nb" ||
                s == "--autobatch" || s == "-ab" ||
                s == "--interleave" || s == "-i" || s == "--pointsort" || s == "-ps" ||
                s == "--querysort" || s == "-qs" ||
@@ -264,6 +269,38 @@ int main(int argc, char** argv) {
               a.learned ? "true" : "false");
   std::printf("Device: %s\n========================================\n\n", rtnn_version());
 
+  if (a.filter_queries && !samepq) {
+    // optix.cpp:67-115 drops the queries outside the points' box grown by r
+    // (they have no neighbour).  Here their rows stay, holding only empty
+    // slots, so row i is still query i; the count is reported the same way.
+    // Boxes as computeMinMax (sort.cpp:34-71): floor of the minimum, floor of
+    // the maximum + 1, per axis.
+    auto box = [](const std::vector<float>& X, float mn[3], float mx[3]) {
+      for (int d = 0; d < 3; ++d) mn[d] = mx[d] = std::floor(X[d]);
+      for (size_t i = 0; i < X.size(); i += 3)
+        for (int d = 0; d < 3; ++d) {
+          mn[d] = std::min(mn[d], std::floor(X[i + d]));
+          mx[d] = std::max(mx[d], std::floor(X[i + d]));
+        }
+      for (int d = 0; d < 3; ++d) mx[d] += 1.0f;
+    };
+    float pmn[3], pmx[3], qmn[3], qmx[3];
+    box(P, pmn, pmx);
+    box(Q, qmn, qmx);
+    bool inside = true;
+    for (int d = 0; d < 3; ++d) inside = inside && qmn[d] >= pmn[d] && qmx[d] <= pmx[d];
+    if (!inside) {
+      uint64_t far = 0;
+      for (uint64_t i = 0; i < nq; ++i) {
+        bool in = true;
+        for (int d = 0; d < 3; ++d)
+          in = in && Q[3 * i + d] >= pmn[d] - a.radius && Q[3 * i + d] <= pmx[d] + a.radius;
+        far += in ? 0 : 1;
+      }
+      std::printf("Filter queries: %llu (%.3f)\n", (unsigned long long)far,
+                  100.0 * (double)far / (double)nq);
+    }
+  }
   rtnn_ctx ctx = nullptr;
   check_rc(nullptr, rtnn_create(a.device, &ctx), "rtnn_create");
   rtnn_options opt{};

@@ -99,3 +99,22 @@ def test_cli_crratio_autotune_and_boundary(tmp_path):
         assert cnt > 0 and len(got) == cnt
         assert np.array_equal(got[:, :2].astype(np.uint64), want)
     assert "autotune: cell_factor" in r.stdout
+
+
+@pytest.mark.gpu
+def test_cli_filter_queries_reports_far_queries(tmp_path):
+    # -fq 1: queries outside the points' (integer-rounded) box grown by r are
+    # counted as the reference's filterRemoteQueries does; their rows stay, empty.
+    rng = np.random.default_rng(4)
+    p = rng.uniform(0, 10, (3000, 3)).astype(np.float32)
+    q = np.concatenate([rng.uniform(0, 10, (300, 3)), rng.uniform(20, 30, (100, 3)),
+                        [[-3.2, 5, 5], [13.9, 5, 5]]]).astype(np.float32)
+    pf, qf, out = tmp_path / "p.txt", tmp_path / "q.txt", str(tmp_path / "o.bin")
+    np.savetxt(pf, p, fmt="%.9g", delimiter=",")
+    np.savetxt(qf, q, fmt="%.9g", delimiter=",")
+    r = run("-f", str(pf), "-q", str(qf), "-r", "3", "-k", "8", "-fq", "1", "-o", out)
+    assert r.returncode == 0, r.stderr
+    # box [0, 10] (floor(max) + 1 = 10), grown by 3: [-3, 13]; -3.2 and 13.9 fall outside
+    assert "Filter queries: 102 (" in r.stdout
+    idx, _ = read_result(out)
+    assert (idx[300:] == -1).all() and (idx[:300, 0] >= 0).all()
