@@ -63,6 +63,9 @@ typedef struct {
   int   group_buffer;     /* group kernel's LDS hit buffer per query: 0 auto / 2
                              wide (2k + G keys), 1 compact (k + 2G keys: less LDS,
                              more waves per CU, more selects on dense rows)     */
+  int   lds_tile;         /* group kernel on a raster grid: 1 = each wave stages the
+                             points of its queries' union block in LDS (512 per
+                             wave) and scans them there; implies compact buffers */
 } rtnn_options;
 
 /* Filled by search entry points when a non-NULL pointer is passed. */
@@ -241,7 +244,8 @@ int rtnn_boundary_pairs_batched(rtnn_ctx ctx, const float* xyz, const uint64_t* 
 #define RTNN_METHOD_LEARNED     2
 
 /* Times every candidate configuration of the method on this exact workload
- * (same queries, r, k): cell factor x queries per wave x group buffer, plus
+ * (same queries, r, k): cell factor x queries per wave x group buffer / LDS
+ * tile, plus
  * hierarchical grid levels x level alpha for the proposal methods, and for
  * the learned method then the MLP's radius margin.  Keeps the fastest in the
  * context's options (also returned in *best; *best_ms = its time per search)
@@ -255,7 +259,8 @@ int rtnn_autotune(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_device, i
                   rtnn_options* best, double* best_ms);
 /* The same for rtnn_search_batched (and so for graphs captured afterwards):
  * cell edge r / {0.5, 1, 2} x queries per wave {1, 4, 16, 64} x the group
- * kernel's LDS buffer (wide / compact), timed on this batch.  out_idx
+ * kernel's LDS use (wide buffer / compact buffer / compact buffer + a tile of
+ * 512 candidates per wave), timed on this batch.  out_idx
  * (required) and out_d2 (nullable) are device scratch of n x k. */
 int rtnn_autotune_batched(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets,
                           uint32_t nclouds, int mode, float r, uint32_t k,

@@ -493,6 +493,8 @@ int rtnn_set_options(rtnn_ctx ctx, const rtnn_options* opt) {
     return fail(ctx, RTNN_ERR_INVALID, "grid_levels must be in [0, 8]");
   if (opt->group_buffer < 0 || opt->group_buffer > 2)
     return fail(ctx, RTNN_ERR_INVALID, "group_buffer must be 0, 1 or 2");
+  if (opt->lds_tile < 0 || opt->lds_tile > 1)
+    return fail(ctx, RTNN_ERR_INVALID, "lds_tile must be 0 or 1");
   if (!(opt->level_alpha >= 0.0f) || !std::isfinite(opt->level_alpha))
     return fail(ctx, RTNN_ERR_INVALID, "level_alpha must be >= 0");
   ctx->opt = *opt;
@@ -566,7 +568,7 @@ static int search_impl(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_devi
   rc = output_targets(ctx, nq, k, out_idx, out_d2, out_on_device, o);
   if (rc) return rc;
   SearchArgs a = base_args(ctx, qproc, nq, r, k, o);
-  CK(launch_search(a, mode == RTNN_MODE_KNN, gl, ctx->st, ctx->num_cus, wide_buffers(ctx)));
+  CK(launch_search(a, mode == RTNN_MODE_KNN, gl, ctx->st, ctx->num_cus, wide_buffers(ctx), ctx->opt.lds_tile != 0));
   if (timing) CK(hipEventRecord(ctx->ev[3], ctx->st));
   rc = copy_out(ctx, nq, k, out_idx, out_d2, out_on_device, o);
   if (rc) return rc;
@@ -736,7 +738,7 @@ int proposal_search(rtnn_ctx ctx, Proposal kind, const float* qxyz, uint64_t nq,
   a.qradius = ctx->qradius.as<float>();
   a.short_rows = short1;
   a.short_count = &counters[0];
-  CK(launch_search(a, 1, gl, ctx->st, ctx->num_cus, wide_buffers(ctx)));
+  CK(launch_search(a, 1, gl, ctx->st, ctx->num_cus, wide_buffers(ctx), ctx->opt.lds_tile != 0));
   CK(hipEventRecord(ctx->ev[3], ctx->st));
   // Stage 2: short rows at min(r, 2 r'); stage 3: what is still short, at r.
   SearchArgs s2 = base_args(ctx, qproc, nq, r, k, o);
@@ -746,11 +748,11 @@ int proposal_search(rtnn_ctx ctx, Proposal kind, const float* qxyz, uint64_t nq,
   s2.qradius_scale = 2.0f;
   s2.short_rows = short2;
   s2.short_count = &counters[1];
-  CK(launch_search(s2, 1, gl, ctx->st, ctx->num_cus, wide_buffers(ctx)));
+  CK(launch_search(s2, 1, gl, ctx->st, ctx->num_cus, wide_buffers(ctx), ctx->opt.lds_tile != 0));
   SearchArgs s3 = base_args(ctx, qproc, nq, r, k, o);
   s3.qsel = short2;
   s3.nq_dev = &counters[1];
-  CK(launch_search(s3, 1, gl, ctx->st, ctx->num_cus, wide_buffers(ctx)));
+  CK(launch_search(s3, 1, gl, ctx->st, ctx->num_cus, wide_buffers(ctx), ctx->opt.lds_tile != 0));
   CK(hipEventRecord(ctx->ev[4], ctx->st));
   rc = copy_out(ctx, nq, k, out_idx, out_d2, out_on_device, o);
   if (rc) return rc;
@@ -1156,7 +1158,7 @@ int rtnn_search_batched(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets,
 
   OutBufs o{out_idx, out_d2};
   SearchArgs a = batched_args(ctx, n, r, k, o);
-  CK(launch_search(a, mode == RTNN_MODE_KNN, gl, ctx->st, ctx->num_cus, wide_buffers(ctx)));
+  CK(launch_search(a, mode == RTNN_MODE_KNN, gl, ctx->st, ctx->num_cus, wide_buffers(ctx), ctx->opt.lds_tile != 0));
   if (timing) CK(hipEventRecord(ctx->ev[2], ctx->st));
   CK(hipStreamSynchronize(ctx->st));
   if (stats) {
@@ -1278,19 +1280,22 @@ int rtnn_autotune(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_device, i
     float cf;
     int qpw, levels;
     float alpha;
-    int buffer;
+    int buffer, tile;
   };
   std::vector<Cand> cands;
   for (int i = 0; i < ncf; ++i)
     for (int j = 0; j < nqpw; ++j) {
       const bool group = qpws[j] > 0 && qpws[j] < 64;
-      cands.push_back({cfs[i], qpws[j], plain ? 0 : 1, 0.0f, group ? 2 : 0});
-      if (group) cands.push_back({cfs[i], qpws[j], plain ? 0 : 1, 0.0f, 1});
+      cands.push_back({cfs[i], qpws[j], plain ? 0 : 1, 0.0f, group ? 2 : 0, 0});
+      if (group) {
+        cands.push_back({cfs[i], qpws[j], plain ? 0 : 1, 0.0f, 1, 0});
+        cands.push_back({cfs[i], qpws[j], plain ? 0 : 1, 0.0f, 1, 1});
+      }
     }
   if (!plain)
     for (int lv : {3, 5})
       for (float al : {1.0f, 0.5f})
-        for (int qpw : {0, 16}) cands.push_back({0.0f, qpw, lv, al, 0});
+        for (int qpw : {0, 16}) cands.push_back({0.0f, qpw, lv, al, 0, 0});
   for (const Cand& cand : cands) {
     {
       const int qpw = cand.qpw;
@@ -1302,6 +1307,7 @@ int rtnn_autotune(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_device, i
       ctx->opt.grid_levels = cand.levels;
       ctx->opt.level_alpha = cand.alpha;
       ctx->opt.group_buffer = cand.buffer;
+      ctx->opt.lds_tile = cand.tile;
       rtnn_stats st{};
       int rc = RTNN_OK;
       for (int rep = 0; rep < 2 && rc == RTNN_OK; ++rep) {  // first run sizes the buffers
@@ -1361,16 +1367,18 @@ int rtnn_autotune_batched(rtnn_ctx ctx, const float* xyz, const uint64_t* offset
   double best_t = 1e300;
   rtnn_options best_opt = base;
   // Cell edge r / {0.5, 1, 2} x lanes per query {64, 16, 4, 1} (queries per
-  // wave 1, 4, 16, 64) x the group kernel's LDS buffer (wide, compact).
+  // wave 1, 4, 16, 64) x the group kernel's LDS use (wide buffer, compact
+  // buffer, compact buffer + wave tile of candidates).
   for (float cf : {0.5f, 1.0f, 2.0f})
     for (int qpw : {1, 4, 16, 64})
-      for (int buf : {2, 1}) {
+      for (int lds : {0, 1, 2}) {
         const int gl = 64 / qpw;
-        if (k > max_k_for_group(gl) || (gl == 1 && buf == 1)) continue;
+        if (k > max_k_for_group(gl) || (gl == 1 && lds > 0)) continue;
         ctx->opt = base;
         ctx->opt.cell_factor = cf;
         ctx->opt.queries_per_wave = qpw;
-        ctx->opt.group_buffer = gl == 1 ? 0 : buf;
+        ctx->opt.group_buffer = gl == 1 ? 0 : (lds == 0 ? 2 : 1);
+        ctx->opt.lds_tile = lds == 2 ? 1 : 0;
         rtnn_stats st{};
         for (int rep = 0; rep < 2 && rc == RTNN_OK; ++rep)
           rc = rtnn_search_batched(ctx, xyz, offsets, nclouds, mode, r, k, out_idx, out_d2, &st);
@@ -1492,7 +1500,8 @@ hipError_t enqueue_batched(rtnn_graph g, const float* xyz, float cell, int mode,
   a.out_d2 = out_d2;
   a.local_index = 1;
   a.persistent = 1;
-  return launch_search(a, mode == RTNN_MODE_KNN, gl, st, g->ctx->num_cus, wide_buffers(g->ctx));
+  return launch_search(a, mode == RTNN_MODE_KNN, gl, st, g->ctx->num_cus, wide_buffers(g->ctx),
+                       g->ctx->opt.lds_tile != 0);
 }
 
 }  // namespace

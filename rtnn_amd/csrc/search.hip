@@ -25,6 +25,7 @@
 //                 and later hits must beat the k-th.  The final rank sort
 //                 writes the row in (d2, index) order.
 #include <float.h>
+#include <limits.h>
 
 #include "common.h"
 #include "internal.h"
@@ -262,6 +263,81 @@ __device__ inline void rank_chunk(const uint64_t* buf, uint32_t cnt, uint32_t ba
 }
 constexpr int kRankChunk = 8;
 
+// One step of a group of G lanes: lane l tests candidate p (if valid) and
+// the group appends its hits to the LDS buffer (ballot + prefix popcount).
+// When the buffer nears CAP, the k smallest are kept: the k-th smallest key T
+// is found by ranking, the entries <= T are compacted to the front in place
+// (a round writes below the entries it has read), and later hits must beat T.
+template <int G, int CAP, bool KNN>
+__device__ inline void group_take(const float4& p, bool valid, const float4& qv, float R2,
+                                  uint64_t gmask, uint64_t* buf, uint32_t& cnt, uint64_t& thr,
+                                  uint32_t k, int l, uint64_t* thr_slot) {
+  bool hit = false;
+  uint64_t key = 0;
+  if (valid) {
+    const float d2 = d2_of(qv.x, qv.y, qv.z, p.x, p.y, p.z);
+    const bool ok = KNN ? (d2 > 0.0f && d2 < R2) : (d2 < R2);
+    key = pack_key(d2, __float_as_uint(p.w));
+    hit = ok && key < thr;
+  }
+  const uint64_t m = __ballot(hit) & gmask;
+  if (hit) buf[cnt + (uint32_t)__popcll(m & lanemask_lt64())] = key;
+  cnt += (uint32_t)__popcll(m);
+  if (cnt > (uint32_t)(CAP - G)) {
+    wave_sync();
+    for (uint32_t b0 = 0; b0 < cnt; b0 += G * kRankChunk) {
+      uint64_t kv[kRankChunk];
+      uint32_t rk[kRankChunk];
+      rank_chunk<G, kRankChunk>(buf, cnt, b0, l, kv, rk);
+#pragma unroll
+      for (int i = 0; i < kRankChunk; ++i)
+        if (rk[i] == k - 1) *thr_slot = kv[i];
+    }
+    wave_sync();
+    const uint64_t T = *thr_slot;
+    uint32_t kept = 0;
+    for (uint32_t e0 = 0; e0 < cnt; e0 += G) {
+      const uint32_t e = e0 + (uint32_t)l;
+      const uint64_t key2 = e < cnt ? buf[e] : ~0ull;
+      const bool keep = e < cnt && key2 <= T;
+      const uint64_t km = __ballot(keep) & gmask;
+      wave_sync();
+      if (keep) buf[kept + (uint32_t)__popcll(km & lanemask_lt64())] = key2;
+      kept += (uint32_t)__popcll(km);
+      wave_sync();
+    }
+    cnt = k;
+    thr = T;
+  }
+}
+
+// Writes a group's row: the buffer's entries by rank (the k smallest), then
+// empty slots; appends the query to the short list if it has fewer than k.
+template <int G>
+__device__ inline void group_finish(const SearchArgs& a, const uint64_t* buf, uint32_t cnt,
+                                    uint32_t k, int l, uint32_t row, uint32_t base,
+                                    uint32_t qs) {
+  wave_sync();
+  int32_t* oi = a.out_idx + (uint64_t)row * k;
+  float* od = a.out_d2 ? a.out_d2 + (uint64_t)row * k : nullptr;
+  for (uint32_t b0 = 0; b0 < cnt; b0 += G * kRankChunk) {
+    uint64_t kv[kRankChunk];
+    uint32_t rk[kRankChunk];
+    rank_chunk<G, kRankChunk>(buf, cnt, b0, l, kv, rk);
+#pragma unroll
+    for (int i = 0; i < kRankChunk; ++i)
+      if (rk[i] < k) {
+        oi[rk[i]] = (int32_t)(key_idx(kv[i]) - base);
+        if (od) od[rk[i]] = key_d2(kv[i]);
+      }
+  }
+  for (uint32_t s = cnt + (uint32_t)l; s < k; s += G) {
+    oi[s] = -1;
+    if (od) od[s] = __builtin_huge_valf();
+  }
+  if (a.short_rows) append_short(a, l == 0 && cnt < k, qs);
+}
+
 template <int G, int KB, bool KNN, bool HIER, bool WIDE>
 __global__ __launch_bounds__(kTileQ) void k_topk_group(const SearchArgs a) {
   using C = GroupCfg<G, KB, WIDE>;
@@ -352,75 +428,174 @@ __global__ __launch_bounds__(kTileQ) void k_topk_group(const SearchArgs a) {
           }
 #pragma unroll
           for (int u = 0; u < kUnroll; ++u) {
-          const uint32_t v = v0 + (uint32_t)(u * G + l);
-          bool hit = false;
-          uint64_t key = 0;
-          if (v < total) {
-            const float4 p = pv[u];
-            const float d2 = d2_of(qv.x, qv.y, qv.z, p.x, p.y, p.z);
-            const bool ok = KNN ? (d2 > 0.0f && d2 < R2) : (d2 < R2);
-            key = pack_key(d2, __float_as_uint(p.w));
-            hit = ok && key < thr;
-          }
-          const uint64_t m = __ballot(hit) & gmask;
-          if (hit) buf[cnt + (uint32_t)__popcll(m & lanemask_lt64())] = key;
-          cnt += (uint32_t)__popcll(m);
-          if (cnt > (uint32_t)(C::CAP - G)) {
-            // Keep the k smallest: find the k-th smallest key T, then compact
-            // the entries <= T to the front in place (a round writes below
-            // the entries it has read), and later hits must beat T.
-            wave_sync();
-            for (uint32_t b0 = 0; b0 < cnt; b0 += G * kRankChunk) {
-              uint64_t kv[kRankChunk];
-              uint32_t rk[kRankChunk];
-              rank_chunk<G, kRankChunk>(buf, cnt, b0, l, kv, rk);
-#pragma unroll
-              for (int i = 0; i < kRankChunk; ++i)
-                if (rk[i] == k - 1) s_thr[gq] = kv[i];
-            }
-            wave_sync();
-            const uint64_t T = s_thr[gq];
-            uint32_t kept = 0;
-            for (uint32_t e0 = 0; e0 < cnt; e0 += G) {
-              const uint32_t e = e0 + (uint32_t)l;
-              const uint64_t key2 = e < cnt ? buf[e] : ~0ull;
-              const bool keep = e < cnt && key2 <= T;
-              const uint64_t km = __ballot(keep) & gmask;
-              wave_sync();
-              if (keep) buf[kept + (uint32_t)__popcll(km & lanemask_lt64())] = key2;
-              kept += (uint32_t)__popcll(km);
-              wave_sync();
-            }
-            cnt = k;
-            thr = T;
-          }
+            const uint32_t v = v0 + (uint32_t)(u * G + l);
+            group_take<G, C::CAP, KNN>(pv[u], v < total, qv, R2, gmask, buf, cnt, thr, k, l,
+                                       &s_thr[gq]);
           }
         }
         wave_sync();
       }
 
-      // Final: rank-sort the buffer and write the row.
+      group_finish<G>(a, buf, cnt, k, l, row, a.local_index ? g.point_base : 0u, qs);
       wave_sync();
-      const uint32_t base = a.local_index ? g.point_base : 0u;
-      int32_t* oi = a.out_idx + (uint64_t)row * k;
-      float* od = a.out_d2 ? a.out_d2 + (uint64_t)row * k : nullptr;
-      for (uint32_t b0 = 0; b0 < cnt; b0 += G * kRankChunk) {
-        uint64_t kv[kRankChunk];
-        uint32_t rk[kRankChunk];
-        rank_chunk<G, kRankChunk>(buf, cnt, b0, l, kv, rk);
+    }
+  }
+}
+
+// --------------------------------------------------------------------------
+// G lanes per query, candidates staged per wave in LDS
+//
+// The 64/G queries of a wave are consecutive in Morton order, so their cell
+// blocks overlap.  The wave loads the points of the union of those blocks into
+// LDS once (one coalesced pass, 64 loads in flight) and every group scans the
+// tile with LDS reads.  A point of the union block outside a query's own block
+// is tested too and fails d2 < R^2: every point within R of the query lies in
+// its own block, each point appears once in the tile, so no hit is missed or
+// counted twice.  A wave whose union block spans more than 64 rows or holds
+// more than kTilePts points walks its rows from global memory instead (the
+// k_topk_group path).  Raster grids, compact group buffers.
+constexpr int kTilePts = 512;
+
+__device__ inline int wave_min_i(int v) {
 #pragma unroll
-        for (int i = 0; i < kRankChunk; ++i)
-          if (rk[i] < k) {
-            oi[rk[i]] = (int32_t)(key_idx(kv[i]) - base);
-            if (od) od[rk[i]] = key_d2(kv[i]);
+  for (int o = kWave / 2; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, kWave));
+  return v;
+}
+__device__ inline int wave_max_i(int v) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, kWave));
+  return v;
+}
+
+template <int G, int KB, bool KNN>
+__global__ __launch_bounds__(kTileQ) void k_topk_tiled(const SearchArgs a) {
+  using C = GroupCfg<G, KB, false>;
+  constexpr int kWaves = kTileQ / kWave;
+  __shared__ uint64_t s_buf[C::NG][C::CAP];
+  __shared__ uint32_t s_rs[C::NG][G];
+  __shared__ uint32_t s_ro[C::NG][G + 1];
+  __shared__ uint64_t s_thr[C::NG];
+  __shared__ float4 s_tile[kWaves][kTilePts];
+  __shared__ uint32_t s_trs[kWaves][kWave];
+  __shared__ uint32_t s_tro[kWaves][kWave + 1];
+
+  const int gq = threadIdx.x / G;
+  const int l = threadIdx.x % G;
+  const int w = threadIdx.x / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t gmask = G == kWave ? ~0ull : (((1ull << G) - 1ull) << (lane - l));
+  uint64_t* buf = s_buf[gq];
+  uint32_t* rs = s_rs[gq];
+  uint32_t* ro = s_ro[gq];
+  float4* tile = s_tile[w];
+  uint32_t* trs = s_trs[w];
+  uint32_t* tro = s_tro[w];
+  const uint32_t k = a.k;
+
+  const uint32_t nq = launch_nq(a);
+  for (TileIter it(a, nq); it.t < it.end; it.t += it.step) {
+    uint32_t q0, q1, gi;
+    tile_of(a, nq, it.t, q0, q1, gi);
+    const GridDesc g = a.grids[gi];
+    // Every group of the block runs the same number of rounds (the wave's
+    // union and tile are collective); groups past q1 only take part in them.
+    for (uint32_t qb = q0; qb < q1; qb += C::NG) {
+      const uint32_t qi = qb + (uint32_t)gq;
+      const bool active = qi < q1;
+      uint32_t qs = 0, row = 0;
+      float4 qv = make_float4(0.f, 0.f, 0.f, 0.f);
+      float R2 = 0.0f;
+      int lo0 = 0, hi0 = -1, lo1 = 0, hi1 = -1, lo2 = 0, hi2 = -1;
+      if (active) {
+        qs = a.qsel ? a.qsel[qi] : qi;
+        qv = a.queries[qs];
+        row = __float_as_uint(qv.w);
+        const float R = a.qradius ? fminf(a.r, a.qradius[qs] * a.qradius_scale) : a.r;
+        R2 = R * R;
+        cell_range(g, qv.x, R, 0, lo0, hi0);
+        cell_range(g, qv.y, R, 1, lo1, hi1);
+        cell_range(g, qv.z, R, 2, lo2, hi2);
+      }
+      const int nx = hi0 - lo0 + 1, ny = hi1 - lo1 + 1, nz = hi2 - lo2 + 1;
+      const bool mine = nx > 0 && ny > 0 && nz > 0;  // a non-empty block
+      // union block of the wave's non-empty blocks (wave-uniform)
+      const int ulo0 = wave_min_i(mine ? lo0 : INT_MAX), uhi0 = wave_max_i(mine ? hi0 : INT_MIN);
+      const int ulo1 = wave_min_i(mine ? lo1 : INT_MAX), uhi1 = wave_max_i(mine ? hi1 : INT_MIN);
+      const int ulo2 = wave_min_i(mine ? lo2 : INT_MAX), uhi2 = wave_max_i(mine ? hi2 : INT_MIN);
+      const bool any = uhi0 >= ulo0;  // some block is non-empty (then all axes are)
+      const int uny = any ? uhi1 - ulo1 + 1 : 0, unz = any ? uhi2 - ulo2 + 1 : 0;
+      const int urows = uny * unz;
+      bool tiled = urows <= kWave;
+      uint32_t T = 0;
+      if (tiled && urows > 0) {
+        uint32_t s0 = 0, len = 0;
+        if (lane < urows) {
+          const int cy = ulo1 + lane % uny, cz = ulo2 + lane / uny;
+          const uint32_t rowc = g.cell_base + ((uint32_t)cz * (uint32_t)g.dim[1] +
+                                               (uint32_t)cy) * (uint32_t)g.dim[0];
+          s0 = a.cell_start[rowc + ulo0];
+          len = a.cell_start[rowc + uhi0 + 1] - s0;
+        }
+        const uint32_t incl = group_incl_scan<kWave>(len, lane);
+        T = __shfl(incl, kWave - 1, kWave);
+        tiled = T <= (uint32_t)kTilePts;
+        if (tiled) {
+          trs[lane] = s0;
+          tro[lane + 1] = incl;
+          if (lane == 0) tro[0] = 0;
+          wave_sync();
+          int rp = 0;
+          for (uint32_t v = (uint32_t)lane; v < T; v += kWave) {
+            while (tro[rp + 1] <= v) ++rp;
+            tile[v] = a.spts[trs[rp] + (v - tro[rp])];
           }
+          wave_sync();
+        }
       }
-      for (uint32_t s = cnt + (uint32_t)l; s < k; s += G) {
-        oi[s] = -1;
-        if (od) od[s] = __builtin_huge_valf();
+
+      uint32_t cnt = 0;
+      uint64_t thr = ~0ull;
+      if (active && tiled) {
+        for (uint32_t v0 = 0; v0 < T; v0 += G) {
+          const uint32_t v = v0 + (uint32_t)l;
+          const float4 p = v < T ? tile[v] : make_float4(0.f, 0.f, 0.f, 0.f);
+          group_take<G, C::CAP, KNN>(p, v < T, qv, R2, gmask, buf, cnt, thr, k, l, &s_thr[gq]);
+        }
+      } else if (active && mine) {
+        const int nrows = ny * nz;
+        for (int rb = 0; rb < nrows; rb += G) {
+          const int j = rb + l;
+          uint32_t s0 = 0, len = 0;
+          if (j < nrows) {
+            const int cy = lo1 + j % ny, cz = lo2 + j / ny;
+            const uint32_t rowc = g.cell_base + ((uint32_t)cz * (uint32_t)g.dim[1] +
+                                                 (uint32_t)cy) * (uint32_t)g.dim[0];
+            s0 = a.cell_start[rowc + lo0];
+            len = a.cell_start[rowc + hi0 + 1] - s0;
+          }
+          const uint32_t incl = group_incl_scan<G>(len, l);
+          const uint32_t total = __shfl(incl, G - 1, G);
+          rs[l] = s0;
+          ro[l + 1] = incl;
+          if (l == 0) ro[0] = 0;
+          wave_sync();
+          int rp = 0;
+          for (uint32_t v0 = 0; v0 < total; v0 += G) {
+            const uint32_t v = v0 + (uint32_t)l;
+            float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (v < total) {
+              while (ro[rp + 1] <= v) ++rp;
+              p = a.spts[rs[rp] + (v - ro[rp])];
+            }
+            group_take<G, C::CAP, KNN>(p, v < total, qv, R2, gmask, buf, cnt, thr, k, l,
+                                       &s_thr[gq]);
+          }
+          wave_sync();
+        }
       }
-      if (a.short_rows) append_short(a, l == 0 && cnt < k, qs);
-      wave_sync();
+      if (active)
+        group_finish<G>(a, buf, cnt, k, l, row, a.local_index ? g.point_base : 0u, qs);
+      wave_sync();  // the tile and buffers are reused by the next round
     }
   }
 }
@@ -442,8 +617,16 @@ static void launch_thread(const SearchArgs& a, bool knn, dim3 grid, hipStream_t 
 }
 
 template <int G, int KB>
-static void launch_group(const SearchArgs& a, bool knn, bool wide, dim3 grid, hipStream_t st) {
+static void launch_group(const SearchArgs& a, bool knn, bool wide, bool tiled, dim3 grid,
+                         hipStream_t st) {
   const bool hier = a.nlevels > 0;
+  if (tiled && !hier) {
+    if (knn)
+      hipLaunchKernelGGL((k_topk_tiled<G, KB, true>), grid, dim3(kTileQ), 0, st, a);
+    else
+      hipLaunchKernelGGL((k_topk_tiled<G, KB, false>), grid, dim3(kTileQ), 0, st, a);
+    return;
+  }
   if (knn && hier)
     hipLaunchKernelGGL((k_topk_group<G, KB, true, true, true>), grid, dim3(kTileQ), 0, st, a);
   else if (knn && wide)
@@ -469,7 +652,7 @@ uint32_t max_k_for_group(int group_lanes) {
 }
 
 hipError_t launch_search(const SearchArgs& a0, int knn_mode, int group_lanes, hipStream_t st,
-                         int num_cus, bool wide) {
+                         int num_cus, bool wide, bool tiled) {
   SearchArgs a = a0;
   if (a.nq_dev) a.persistent = 1;  // tile count known only on the device
   const uint32_t k = a.k;
@@ -498,17 +681,17 @@ hipError_t launch_search(const SearchArgs& a0, int knn_mode, int group_lanes, hi
     else if (k <= 32) launch_thread<32>(a, knn, grid, st);
     else launch_thread<64>(a, knn, grid, st);
   } else if (group_lanes == 4) {
-    if (k <= 16) launch_group<4, 16>(a, knn, wide, grid, st);
-    else launch_group<4, 64>(a, knn, wide, grid, st);
+    if (k <= 16) launch_group<4, 16>(a, knn, wide, tiled, grid, st);
+    else launch_group<4, 64>(a, knn, wide, tiled, grid, st);
   } else if (group_lanes == 16) {
-    if (k <= 16) launch_group<16, 16>(a, knn, wide, grid, st);
-    else if (k <= 64) launch_group<16, 64>(a, knn, wide, grid, st);
-    else launch_group<16, 256>(a, knn, wide, grid, st);
+    if (k <= 16) launch_group<16, 16>(a, knn, wide, tiled, grid, st);
+    else if (k <= 64) launch_group<16, 64>(a, knn, wide, tiled, grid, st);
+    else launch_group<16, 256>(a, knn, wide, tiled, grid, st);
   } else {
-    if (k <= 16) launch_group<64, 16>(a, knn, wide, grid, st);
-    else if (k <= 64) launch_group<64, 64>(a, knn, wide, grid, st);
-    else if (k <= 256) launch_group<64, 256>(a, knn, wide, grid, st);
-    else launch_group<64, 1024>(a, knn, wide, grid, st);
+    if (k <= 16) launch_group<64, 16>(a, knn, wide, tiled, grid, st);
+    else if (k <= 64) launch_group<64, 64>(a, knn, wide, tiled, grid, st);
+    else if (k <= 256) launch_group<64, 256>(a, knn, wide, tiled, grid, st);
+    else launch_group<64, 1024>(a, knn, wide, tiled, grid, st);
   }
   return hipGetLastError();
 }

@@ -80,8 +80,9 @@ def test_kernel_variants_and_cell_factors(ctx, qpw, cf):
     pts = rng.uniform(0, 20, (20000, 3)).astype(np.float32)
     qs = np.concatenate([pts[:3000], rng.uniform(-3, 23, (2000, 3)).astype(np.float32)])
     ctx.set_points(pts)
-    for buf in ((0, 1) if qpw in (4, 16) else (0,)):
-        ctx.set_options(cell_factor=cf, queries_per_wave=qpw, group_buffer=buf)
+    # group kernels: wide and compact LDS buffers, and the wave-tiled kernel
+    for buf, tile in (((0, 0), (1, 0), (1, 1)) if qpw in (1, 4, 16) else ((0, 0),)):
+        ctx.set_options(cell_factor=cf, queries_per_wave=qpw, group_buffer=buf, lds_tile=tile)
         for mode in ("radius", "knn"):
             for k in (1, 5, 16, 33, 64):
                 check(ctx, pts, qs, mode, 1.5, k)
@@ -104,9 +105,10 @@ def test_group_buffer_overflow_path(ctx):
     rng = np.random.default_rng(8)
     pts = rng.normal(0, 1.0, (60000, 3)).astype(np.float32)
     ctx.set_points(pts)
-    for buf in (2, 1):  # wide (2k + G keys) and compact (k + 2G keys) buffers
+    # wide (2k + G keys) and compact (k + 2G keys) buffers, and the tiled kernel
+    for buf, tile in ((2, 0), (1, 0), (1, 1)):
         for qpw in (1, 4, 16):  # 64, 16 and 4 lanes per query
-            ctx.set_options(queries_per_wave=qpw, group_buffer=buf)
+            ctx.set_options(queries_per_wave=qpw, group_buffer=buf, lds_tile=tile)
             check(ctx, pts, pts[:1500], "radius", 2.0, 16)
             check(ctx, pts, pts[:1500], "knn", 2.0, 64)
             check(ctx, pts, pts[:300], "radius", 2.0, 5)
@@ -130,7 +132,9 @@ def test_spheres_knn16(ctx):
     check(ctx, pts, qs, "knn", 10.0, 16)
 
 
-def test_edge_cases(ctx):
+@pytest.mark.parametrize("opts", [{}, {"queries_per_wave": 4, "lds_tile": 1}])
+def test_edge_cases(ctx, opts):
+    ctx.set_options(**opts)
     rng = np.random.default_rng(12)
     # duplicates + one hot cell
     base = rng.uniform(0, 1e-3, (500, 3)).astype(np.float32)
@@ -217,11 +221,11 @@ def test_deterministic(ctx):
     assert np.array_equal(a, b) and np.array_equal(ad.view(np.int32), bd.view(np.int32))
 
 
-@pytest.mark.parametrize("qpw", [0, 1, 16])
-def test_batched_clouds(ctx, qpw):
+@pytest.mark.parametrize("qpw,tile", [(0, 0), (1, 0), (16, 0), (16, 1), (4, 1)])
+def test_batched_clouds(ctx, qpw, tile):
     import torch
     xyz, offs = rtnn_amd.gen_lattice_clouds(9, 40, 1000, 8000)
-    ctx.set_options(queries_per_wave=qpw)
+    ctx.set_options(queries_per_wave=qpw, lds_tile=tile)
     t = torch.from_numpy(xyz).cuda()
     bi, bd, st = ctx.search_batched(t, offs, mode="radius", radius=0.2, k=64)
     pi, pd, _ = ctx.search_percloud(t, offs, mode="radius", radius=0.2, k=64)
