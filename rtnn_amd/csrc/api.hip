@@ -85,6 +85,11 @@ struct rtnn_ctx_s {
   DevBuf qin, qkeys, qvals, qsorted, oidx, od2, shortrows, misc, tiles, offs, bboxes;
   hipEvent_t ev[8] = {};
   hipEvent_t ev_join = nullptr;  // orders rtnn_search_async work with the context stream
+  // The query ordering runs on st2 while the point grid is sorted on st:
+  // ev_desc (grid descriptors uploaded) starts it, ev_q joins it back.
+  hipStream_t st2 = nullptr;
+  hipEvent_t ev_desc = nullptr, ev_q = nullptr;
+  DevBuf tmp2;  // radix-sort scratch of the query ordering
 
   // radius proposals (learned / partitioned)
   DevBuf pgrid, pcounts, qradius;
@@ -109,6 +114,10 @@ struct rtnn_ctx_s {
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e);
     if (ev_join) (void)hipEventDestroy(ev_join);
+    if (ev_desc) (void)hipEventDestroy(ev_desc);
+    if (ev_q) (void)hipEventDestroy(ev_q);
+    tmp2.release();
+    if (st2) (void)hipStreamDestroy(st2);
     if (st) (void)hipStreamDestroy(st);
   }
 };
@@ -213,6 +222,7 @@ int build_grid(rtnn_ctx ctx, float cell) {
   const uint64_t n = ctx->n;
   CK(ctx->grids.ensure(sizeof(GridDesc)));
   CK(hipMemcpyAsync(ctx->grids.p, &ctx->hgrid, sizeof(GridDesc), hipMemcpyHostToDevice, ctx->st));
+  CK(hipEventRecord(ctx->ev_desc, ctx->st));
   CK(ctx->keys.ensure(4 * n));
   CK(ctx->vals.ensure(4 * n));
   CK(ctx->cell_start.ensure(4 * (ctx->ncells + 1)));
@@ -282,6 +292,7 @@ int build_hgrid(rtnn_ctx ctx, float r, int want_levels) {
   CK(ctx->grids.ensure(sizeof(GridDesc) * L));
   CK(hipMemcpyAsync(ctx->grids.p, lv.data(), sizeof(GridDesc) * L, hipMemcpyHostToDevice,
                     ctx->st));
+  CK(hipEventRecord(ctx->ev_desc, ctx->st));
   CK(ctx->keys.ensure(4 * n));
   CK(ctx->vals.ensure(4 * n));
   CK(ctx->cell_start.ensure(4 * total));
@@ -305,39 +316,49 @@ int build_hgrid(rtnn_ctx ctx, float r, int want_levels) {
   return RTNN_OK;
 }
 
+constexpr int kQueryKeyBits = 8;  // Morton bits per axis of the query order
+
 // Prepares queries in processing order (float4 with the output row in w).
-// Returns the processing-order array through *qproc.
+// Returns the processing-order array through *qproc.  The ordering runs on
+// ctx->st2, started by ctx->ev_desc (recorded by the grid build once its
+// descriptors are on the device), so it overlaps the point sort on ctx->st;
+// ctx->st waits for it before returning work that reads *qproc.
 int prepare_queries(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_device,
                     const float4** qproc) {
   if (!qxyz) {  // the points themselves, already in cell order
     *qproc = ctx->spts.as<float4>();
     return RTNN_OK;
   }
-  const float* dq = qxyz;
-  if (!on_device) {
-    CK(ctx->qin.ensure(12 * nq));
-    CK(hipMemcpyAsync(ctx->qin.p, qxyz, 12 * nq, hipMemcpyHostToDevice, ctx->st));
-    dq = ctx->qin.as<float>();
-  }
   CK(ctx->qkeys.ensure(4 * nq));
   CK(ctx->qvals.ensure(4 * nq));
   CK(ctx->qsorted.ensure(16 * nq));
-  uint64_t tmpw = radix_tmp_words(nq);
-  if (ctx->tmp.cap < 4 * tmpw) {
-    // keep the grid's scratch contents irrelevant: tmp is only scratch
-    CK(ctx->tmp.ensure(4 * tmpw));
+  CK(ctx->tmp2.ensure(4 * radix_tmp_words(nq)));
+  if (!on_device) CK(ctx->qin.ensure(12 * nq));
+  hipStream_t s2 = ctx->st2;
+  CK(hipStreamWaitEvent(s2, ctx->ev_desc, 0));
+  const float* dq = qxyz;
+  if (!on_device) {
+    CK(hipMemcpyAsync(ctx->qin.p, qxyz, 12 * nq, hipMemcpyHostToDevice, s2));
+    dq = ctx->qin.as<float>();
   }
   if (ctx->opt.query_order == 1) {
-    CK(iota_u32(ctx->qvals.as<uint32_t>(), nq, ctx->st));
+    CK(iota_u32(ctx->qvals.as<uint32_t>(), nq, s2));
   } else {
+    // Morton order of the query cells, 8 bits per axis (cells merged in
+    // aligned 2^shift blocks on finer grids): 3 radix passes, and a wave's
+    // queries still share their neighbourhood.
     int maxdim = std::max(ctx->hgrid.dim[0], std::max(ctx->hgrid.dim[1], ctx->hgrid.dim[2]));
-    int shift = std::max(0, bits_for((uint64_t)(maxdim - 1)) - 10);
+    const int bits = bits_for((uint64_t)(maxdim - 1));
+    const int shift = std::max(0, bits - kQueryKeyBits);
     CK(query_morton_keys(dq, nq, ctx->grids.as<GridDesc>(), shift, ctx->qkeys.as<uint32_t>(),
-                         ctx->qvals.as<uint32_t>(), ctx->st));
-    CK(radix_sort_pairs(ctx->qkeys.as<uint32_t>(), ctx->qvals.as<uint32_t>(), nq, 30,
-                        ctx->tmp.as<uint32_t>(), ctx->st, nullptr));
+                         ctx->qvals.as<uint32_t>(), s2));
+    CK(radix_sort_pairs(ctx->qkeys.as<uint32_t>(), ctx->qvals.as<uint32_t>(), nq,
+                        3 * std::min(bits, kQueryKeyBits), ctx->tmp2.as<uint32_t>(), s2,
+                        nullptr));
   }
-  CK(gather_points(dq, ctx->qvals.as<uint32_t>(), nq, ctx->qsorted.as<float4>(), ctx->st));
+  CK(gather_points(dq, ctx->qvals.as<uint32_t>(), nq, ctx->qsorted.as<float4>(), s2));
+  CK(hipEventRecord(ctx->ev_q, s2));
+  CK(hipStreamWaitEvent(ctx->st, ctx->ev_q, 0));
   *qproc = ctx->qsorted.as<float4>();
   return RTNN_OK;
 }
@@ -459,8 +480,16 @@ int rtnn_create(int device, rtnn_ctx* out) {
     delete c;
     return RTNN_ERR_HIP;
   }
+  e = hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    g_create_err = std::string("hipStreamCreate: ") + hipGetErrorString(e);
+    delete c;
+    return RTNN_ERR_HIP;
+  }
   for (auto& ev : c->ev) (void)hipEventCreate(&ev);
   (void)hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming);
+  (void)hipEventCreateWithFlags(&c->ev_desc, hipEventDisableTiming);
+  (void)hipEventCreateWithFlags(&c->ev_q, hipEventDisableTiming);
   int cus = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
       cus > 0)
