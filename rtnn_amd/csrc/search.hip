@@ -244,15 +244,16 @@ struct GroupCfg {
 };
 
 // Ranks among buf[0 .. cnt) of this lane's entries e = base + l + G*i,
-// i < CH (rank ~0 marks no entry), in one pass over the buffer.  Keys are
-// distinct (the index is part of the key).
+// i < CH, in one pass over the buffer.  Keys are distinct (the index is part
+// of the key).  A slot past cnt gets key 0 (no key is below it, so its rank
+// is never incremented) and rank ~0, which matches no rank < k.
 template <int G, int CH>
 __device__ inline void rank_chunk(const uint64_t* buf, uint32_t cnt, uint32_t base, int l,
                                   uint64_t (&kv)[CH], uint32_t (&rk)[CH]) {
 #pragma unroll
   for (int i = 0; i < CH; ++i) {
     const uint32_t e = base + (uint32_t)(l + G * i);
-    kv[i] = e < cnt ? buf[e] : ~0ull;
+    kv[i] = e < cnt ? buf[e] : 0ull;
     rk[i] = e < cnt ? 0u : ~0u;
   }
   for (uint32_t j = 0; j < cnt; ++j) {
