@@ -770,6 +770,9 @@ int proposal_search(rtnn_ctx ctx, Proposal kind, const float* qxyz, uint64_t nq,
   CK(launch_search(a, 1, gl, ctx->st, ctx->num_cus, wide_buffers(ctx), ctx->opt.lds_tile != 0));
   CK(hipEventRecord(ctx->ev[3], ctx->st));
   // Stage 2: short rows at min(r, 2 r'); stage 3: what is still short, at r.
+  // These rows are few and their radii large (many candidates each): the
+  // thread-per-query kernel, whose register top-k rejects most candidates
+  // with one compare, beats the group kernels there (k <= 64 here).
   SearchArgs s2 = base_args(ctx, qproc, nq, r, k, o);
   s2.qsel = short1;
   s2.nq_dev = &counters[0];
@@ -777,11 +780,11 @@ int proposal_search(rtnn_ctx ctx, Proposal kind, const float* qxyz, uint64_t nq,
   s2.qradius_scale = 2.0f;
   s2.short_rows = short2;
   s2.short_count = &counters[1];
-  CK(launch_search(s2, 1, gl, ctx->st, ctx->num_cus, wide_buffers(ctx), ctx->opt.lds_tile != 0));
+  CK(launch_search(s2, 1, 1, ctx->st, ctx->num_cus));
   SearchArgs s3 = base_args(ctx, qproc, nq, r, k, o);
   s3.qsel = short2;
   s3.nq_dev = &counters[1];
-  CK(launch_search(s3, 1, gl, ctx->st, ctx->num_cus, wide_buffers(ctx), ctx->opt.lds_tile != 0));
+  CK(launch_search(s3, 1, 1, ctx->st, ctx->num_cus));
   CK(hipEventRecord(ctx->ev[4], ctx->st));
   rc = copy_out(ctx, nq, k, out_idx, out_d2, out_on_device, o);
   if (rc) return rc;

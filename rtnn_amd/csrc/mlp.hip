@@ -115,40 +115,49 @@ __device__ inline float feature(const GridDesc& g, const uint32_t* __restrict__ 
 }
 
 // The 8 features 16ks + 4*grp + j (ks < 2, j < 4) of one query, as the two
-// BF16 B-operand fragments of layer 1: every count address is formed first
-// and all loads are issued before the first is used (branch-free; an out of
-// range cell loads entry 0 and is masked), so a lane waits on one memory
-// round trip per tile instead of one per feature.  Same values as feature().
-__device__ inline void features_frag(const GridDesc& g, const uint32_t* __restrict__ counts,
-                                     const QueryCell& qc, int grp, float count_scale,
-                                     bf16x4 (&x)[2]) {
+// BF16 B-operand fragments of layer 1, in two steps so a caller can overlap
+// the count loads with other work: feature_loads forms every count address
+// and issues all loads (branch-free; an out of range cell loads entry 0 and
+// is masked), feature_frag turns the counts into the fragments.  Same values
+// as feature().
+struct FeatureLoads {
+  uint32_t c0[8], c1[8];
+  uint32_t valid;  // bit t: feature t is a count inside the grid
+};
+
+__device__ inline void feature_loads(const GridDesc& g, const uint32_t* __restrict__ counts,
+                                     const QueryCell& qc, int grp, FeatureLoads& fl) {
   uint32_t id[8];
-  bool valid[8];
+  fl.valid = 0;
 #pragma unroll
   for (int t = 0; t < 8; ++t) {
     const int f = 16 * (t / 4) + 4 * grp + (t % 4);
     const int cx = qc.cx + f % 3 - 1, cy = qc.cy + (f / 3) % 3 - 1, cz = qc.cz + f / 9 - 1;
-    valid[t] = f < 27 && cx >= 0 && cy >= 0 && cz >= 0 && cx < g.dim[0] && cy < g.dim[1] &&
-               cz < g.dim[2];
+    const bool v = f < 27 && cx >= 0 && cy >= 0 && cz >= 0 && cx < g.dim[0] && cy < g.dim[1] &&
+                   cz < g.dim[2];
     uint32_t cell;
     if (g.morton)
       cell = g.cell_base + morton3((uint32_t)cx, (uint32_t)cy, (uint32_t)cz);
     else
       cell = ((uint32_t)cz * (uint32_t)g.dim[1] + (uint32_t)cy) * (uint32_t)g.dim[0] +
              (uint32_t)cx;
-    id[t] = valid[t] ? cell : 0u;
+    id[t] = v ? cell : 0u;
+    fl.valid |= v ? (1u << t) : 0u;
   }
-  uint32_t c0[8], c1[8];
 #pragma unroll
   for (int t = 0; t < 8; ++t) {
-    c0[t] = counts[id[t]];
-    c1[t] = g.morton ? counts[id[t] + 1] : 0u;
+    fl.c0[t] = counts[id[t]];
+    fl.c1[t] = g.morton ? counts[id[t] + 1] : 0u;
   }
+}
+
+__device__ inline void feature_frag(const GridDesc& g, const QueryCell& qc, int grp,
+                                    float count_scale, const FeatureLoads& fl, bf16x4 (&x)[2]) {
 #pragma unroll
   for (int t = 0; t < 8; ++t) {
     const int f = 16 * (t / 4) + 4 * grp + (t % 4);
-    const uint32_t c = g.morton ? c1[t] - c0[t] : c0[t];
-    float v = valid[t] ? __logf(1.0f + (float)c) * count_scale : 0.0f;
+    const uint32_t c = g.morton ? fl.c1[t] - fl.c0[t] : fl.c0[t];
+    float v = (fl.valid >> t) & 1u ? __logf(1.0f + (float)c) * count_scale : 0.0f;
     v = f == 27 ? qc.ox : (f == 28 ? qc.oy : (f == 29 ? qc.oz : v));
     x[t / 4][t % 4] = (short)f2bf(v);
   }
@@ -288,22 +297,39 @@ __global__ __launch_bounds__(256) void k_mlp_propose(const float* __restrict__ P
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / 64;
   const uint32_t nwaves = gridDim.x * blockDim.x / 64;
   const uint32_t ntiles = (nq + 15) / 16;
-  for (uint32_t t = wave; t < ntiles; t += nwaves) {
-    const uint32_t qi = t * 16 + row;
+  // Software pipeline over this wave's tiles: while tile t runs its MFMA
+  // layers, the count loads of tile t + 1 and the query of tile t + 2 are in
+  // flight.  A position past nq reads query 0 (its result is not written).
+  auto load_q = [&](uint32_t t) -> float4 {
+    const uint64_t qi = (uint64_t)t * 16 + row;
+    return qi < nq ? qproc[qi] : qproc[0];
+  };
+  uint32_t t = wave;
+  if (t >= ntiles) return;
+  QueryCell qa = query_cell(g, 0.f, 0.f, 0.f);
+  {
+    const float4 q = load_q(t);
+    qa = query_cell(g, q.x, q.y, q.z);
+  }
+  FeatureLoads la;
+  feature_loads(g, counts, qa, grp, la);
+  float4 qb = load_q(t + nwaves);
+  for (; t < ntiles; t += nwaves) {
+    const QueryCell qn = query_cell(g, qb.x, qb.y, qb.z);
+    FeatureLoads ln;
+    feature_loads(g, counts, qn, grp, ln);
+    const float4 qc = load_q(t + 2 * nwaves);
     bf16x4 x[2];
-    if (qi < nq) {
-      const float4 q = qproc[qi];
-      QueryCell qc = query_cell(g, q.x, q.y, q.z);
-      features_frag(g, counts, qc, grp, count_scale, x);
-    } else {
-#pragma unroll
-      for (int ks = 0; ks < 2; ++ks) x[ks] = bf16x4{0, 0, 0, 0};
-    }
-    float o = mlp_tile(w, x);
+    feature_frag(g, qa, grp, count_scale, la, x);
+    const float o = mlp_tile(w, x);
+    const uint32_t qi = t * 16 + row;
     if (grp == 0 && qi < nq) {
-      float frac = fminf(fmaxf(o * margin, kMinFrac), 1.0f);
+      const float frac = fminf(fmaxf(o * margin, kMinFrac), 1.0f);
       qradius[qi] = fminf(r * frac, r);
     }
+    qa = qn;
+    la = ln;
+    qb = qc;
   }
 }
 
