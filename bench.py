@@ -79,6 +79,7 @@ def workload(name):
         return dict(pts=rtnn_amd.gen_levy(5, 30_000_000, alpha=1.5, min_step=0.01, box=1000.0),
                     qs=rtnn_amd.gen_uniform(51, 30_000_000, 0.0, 1000.0), mode="knn", r=100.0,
                     k=32, methods=["partitioned", "learned"], train_on_queries=True,
+                    autotune=False,  # each candidate search at r = 100 takes seconds
                     cfg={"workload": "KNN K=32 r=100: 30M Levy-flight points (Pareto alpha 1.5, "
                                      "min step 0.01, periodic [0,1000]^3), 30M uniform queries",
                          "points": 30_000_000, "queries": 30_000_000, "k": 32, "radius": 100.0})
@@ -245,7 +246,7 @@ def main():
             tuned = dict(batched_opts[0], search_ms=batched_opts[1], seconds=batched_opts[2])
         if m == "graph":  # capture with the options in force (tuned or default)
             graph = ctx.graph_batched(dp, w["offsets"], mode=mode, radius=r, k=k, with_d2=False)
-        if args.autotune and m in ("plain", "partitioned", "learned"):
+        if args.autotune and w.get("autotune", True) and m in ("plain", "partitioned", "learned"):
             t0 = time.perf_counter()
             opt, tms = ctx.autotune(dq, method=m, mode=mode, radius=r, k=k)
             tuned = dict(opt, search_ms=round(tms, 4), seconds=round(time.perf_counter() - t0, 3))

@@ -711,7 +711,7 @@ int compute_proposals(rtnn_ctx ctx, Proposal kind, const float* qxyz, uint64_t n
       coarse = ctx->pgrid.as<GridDesc>();
       counts = ctx->pcounts.as<uint32_t>();
     }
-    CK(mlp_propose(ctx->mlp, qproc, (uint32_t)nq, coarse, counts, r, ctx->qradius.as<float>(),
+    CK(mlp_propose(ctx->mlp, qproc, (uint32_t)nq, coarse, hier, counts, r, ctx->qradius.as<float>(),
                    ctx->st, ctx->num_cus));
   } else if (hier) {
     CK(partition_hier(qproc, (uint32_t)nq, ctx->grids.as<GridDesc>(), ctx->hlevels,
@@ -1294,12 +1294,12 @@ int rtnn_autotune(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_device, i
   }
   const rtnn_options base = ctx->opt;
   static const float kPlainCf[] = {1.0f, 2.0f, 3.0f};
-  static const float kPropCf[] = {1.5f, 2.0f, 2.5f, 3.0f};
+  static const float kPropCf[] = {1.5f, 2.0f, 2.5f, 3.0f, 4.0f};
   static const int kQpwKnn[] = {0, 16, 4};
   static const int kQpwRange[] = {64, 16, 4, 1};
   const bool plain = method == RTNN_METHOD_PLAIN;
   const float* cfs = plain ? kPlainCf : kPropCf;
-  const int ncf = plain ? 3 : 4;
+  const int ncf = plain ? 3 : 5;
   const bool range = plain && mode == RTNN_MODE_RADIUS;
   const int* qpws = range ? kQpwRange : kQpwKnn;
   const int nqpw = range ? 4 : 3;

@@ -50,10 +50,12 @@ hipError_t mlp_features(const float* qxyz, uint64_t n, const GridDesc* coarse,
 hipError_t mlp_forward(const MlpState& m, const float* feats, uint64_t n, float* out,
                        hipStream_t st, int num_cus);
 // Fused features + MFMA forward for queries in processing order; writes the
-// proposed radius per processing position.
+// proposed radius per processing position.  morton: *coarse is a level of a
+// hierarchical Morton grid (counts = its table: end - start), else a raster
+// count grid.
 hipError_t mlp_propose(const MlpState& m, const float4* qproc, uint32_t nq,
-                       const GridDesc* coarse, const uint32_t* counts, float r, float* qradius,
-                       hipStream_t st, int num_cus);
+                       const GridDesc* coarse, bool morton, const uint32_t* counts, float r,
+                       float* qradius, hipStream_t st, int num_cus);
 // Pinball-loss gradient partials over `count` samples (rows sel[i] of
 // feats/targets, or rows 0..count-1 if sel == nullptr, or seeded random rows
 // of [0, n) if rand_seed != 0).  norm = 1 / batch size.

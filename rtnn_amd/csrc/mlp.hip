@@ -18,9 +18,9 @@
 //
 // GELU is the tanh form of Hendrycks & Gimpel (2016),
 //   gelu(x) = 0.5 x (1 + tanh(sqrt(2/pi) (x + 0.044715 x^3))),
-// with tanh(u) = 1 - 2 / (1 + exp(2u)) evaluated through the hardware exp and
-// reciprocal (v_exp_f32, v_rcp_f32; about 1 ulp each, far below the BF16
-// rounding of the activations).
+// evaluated as x / (1 + exp(-2u)) through the hardware exp2 and reciprocal
+// (v_exp_f32, v_rcp_f32; about 1 ulp each, far below the BF16 rounding of the
+// activations); its derivative through tanh(u) = 1 - 2 / (1 + exp(2u)).
 #include <math.h>
 
 #include "../../include/rtnn_hip.h"
@@ -33,16 +33,17 @@ namespace rtnn {
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-// FP32 -> BF16 with round-to-nearest-even (finite inputs).
+// FP32 -> BF16 with round-to-nearest-even (finite inputs): on the device the
+// gfx950 conversion instruction (v_cvt_pk_bf16_f32, round to nearest even).
 __host__ __device__ inline uint16_t f2bf(float f) {
-  uint32_t u;
 #ifdef __HIP_DEVICE_COMPILE__
-  u = __float_as_uint(f);
+  return __builtin_bit_cast(uint16_t, (__bf16)f);
 #else
+  uint32_t u;
   __builtin_memcpy(&u, &f, 4);
-#endif
   u += 0x7fffu + ((u >> 16) & 1u);
   return (uint16_t)(u >> 16);
+#endif
 }
 __device__ inline float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 __device__ inline float round_bf(float f) { return bf2f(f2bf(f)); }
@@ -53,9 +54,15 @@ constexpr float kGeluA = 0.044715f;
 __device__ inline float tanh_fast(float u) {
   return 1.0f - 2.0f * __builtin_amdgcn_rcpf(1.0f + __expf(2.0f * u));
 }
+// 0.5 x (1 + tanh(u)) = x / (1 + exp(-2u)): one hardware exp2 and one
+// reciprocal, u expanded so exp2 takes x * (k1 + k2 x^2) directly
+// (k1 = -2 log2(e) c, k2 = k1 a).  exp2 overflowing to +inf gives 0 (the
+// limit for x -> -inf).
+constexpr float kGeluK1 = -2.0f * 1.4426950408889634f * kGeluC;
+constexpr float kGeluK2 = kGeluK1 * kGeluA;
 __device__ inline float gelu(float x) {
-  float u = kGeluC * (x + kGeluA * x * x * x);
-  return 0.5f * x * (1.0f + tanh_fast(u));
+  const float e = __builtin_amdgcn_exp2f(x * fmaf(kGeluK2, x * x, kGeluK1));
+  return x * __builtin_amdgcn_rcpf(1.0f + e);
 }
 __device__ inline float gelu_grad(float x) {
   float u = kGeluC * (x + kGeluA * x * x * x);
@@ -125,6 +132,7 @@ struct FeatureLoads {
   uint32_t valid;  // bit t: feature t is a count inside the grid
 };
 
+template <bool MORTON>
 __device__ inline void feature_loads(const GridDesc& g, const uint32_t* __restrict__ counts,
                                      const QueryCell& qc, int grp, FeatureLoads& fl) {
   uint32_t id[8];
@@ -136,7 +144,7 @@ __device__ inline void feature_loads(const GridDesc& g, const uint32_t* __restri
     const bool v = f < 27 && cx >= 0 && cy >= 0 && cz >= 0 && cx < g.dim[0] && cy < g.dim[1] &&
                    cz < g.dim[2];
     uint32_t cell;
-    if (g.morton)
+    if (MORTON)
       cell = g.cell_base + morton3((uint32_t)cx, (uint32_t)cy, (uint32_t)cz);
     else
       cell = ((uint32_t)cz * (uint32_t)g.dim[1] + (uint32_t)cy) * (uint32_t)g.dim[0] +
@@ -147,16 +155,17 @@ __device__ inline void feature_loads(const GridDesc& g, const uint32_t* __restri
 #pragma unroll
   for (int t = 0; t < 8; ++t) {
     fl.c0[t] = counts[id[t]];
-    fl.c1[t] = g.morton ? counts[id[t] + 1] : 0u;
+    fl.c1[t] = MORTON ? counts[id[t] + 1] : 0u;
   }
 }
 
-__device__ inline void feature_frag(const GridDesc& g, const QueryCell& qc, int grp,
-                                    float count_scale, const FeatureLoads& fl, bf16x4 (&x)[2]) {
+template <bool MORTON>
+__device__ inline void feature_frag(const QueryCell& qc, int grp, float count_scale,
+                                    const FeatureLoads& fl, bf16x4 (&x)[2]) {
 #pragma unroll
   for (int t = 0; t < 8; ++t) {
     const int f = 16 * (t / 4) + 4 * grp + (t % 4);
-    const uint32_t c = g.morton ? fl.c1[t] - fl.c0[t] : fl.c0[t];
+    const uint32_t c = MORTON ? fl.c1[t] - fl.c0[t] : fl.c0[t];
     float v = (fl.valid >> t) & 1u ? __logf(1.0f + (float)c) * count_scale : 0.0f;
     v = f == 27 ? qc.ox : (f == 28 ? qc.oy : (f == 29 ? qc.oz : v));
     x[t / 4][t % 4] = (short)f2bf(v);
@@ -226,10 +235,12 @@ __device__ inline float mlp_tile(const MfmaLds& w, const bf16x4 (&x)[2]) {
   // Compiler-only barrier: keeps the weight reads inside the tile loop (hoisted
   // out of it they would occupy ~100 VGPRs and halve the waves per SIMD).
   __asm__ __volatile__("" ::: "memory");
+  // The biases start the accumulators: D = W X + b in one MFMA chain.
   f32x4 acc[4];
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
-    acc[s] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[s][j] = w.b1[16 * s + 4 * grp + j];
 #pragma unroll
     for (int ks = 0; ks < 2; ++ks)
       acc[s] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w.a1[s][ks][l], x[ks], acc[s], 0, 0, 0);
@@ -238,11 +249,11 @@ __device__ inline float mlp_tile(const MfmaLds& w, const bf16x4 (&x)[2]) {
 #pragma unroll
   for (int s = 0; s < 4; ++s)
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
-      h1[s][j] = (short)f2bf(gelu(acc[s][j] + w.b1[16 * s + 4 * grp + j]));
+    for (int j = 0; j < 4; ++j) h1[s][j] = (short)f2bf(gelu(acc[s][j]));
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
-    acc[s] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[s][j] = w.b2[16 * s + 4 * grp + j];
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks)
       acc[s] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w.a2[s][ks][l], h1[ks], acc[s], 0, 0, 0);
@@ -251,10 +262,8 @@ __device__ inline float mlp_tile(const MfmaLds& w, const bf16x4 (&x)[2]) {
 #pragma unroll
   for (int s = 0; s < 4; ++s)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int o = 16 * s + 4 * grp + j;
-      part += w.w3[o] * round_bf(gelu(acc[s][j] + w.b2[o]));
-    }
+    for (int j = 0; j < 4; ++j)
+      part = fmaf(w.w3[16 * s + 4 * grp + j], round_bf(gelu(acc[s][j])), part);
   part += __shfl_xor(part, 16, 64);
   part += __shfl_xor(part, 32, 64);
   return part + w.b3;
@@ -283,6 +292,7 @@ __global__ __launch_bounds__(256) void k_mlp_forward(const float* __restrict__ P
   }
 }
 
+template <bool MORTON>
 __global__ __launch_bounds__(256) void k_mlp_propose(const float* __restrict__ P,
                                                      const float4* __restrict__ qproc,
                                                      uint32_t nq,
@@ -312,15 +322,15 @@ __global__ __launch_bounds__(256) void k_mlp_propose(const float* __restrict__ P
     qa = query_cell(g, q.x, q.y, q.z);
   }
   FeatureLoads la;
-  feature_loads(g, counts, qa, grp, la);
+  feature_loads<MORTON>(g, counts, qa, grp, la);
   float4 qb = load_q(t + nwaves);
   for (; t < ntiles; t += nwaves) {
     const QueryCell qn = query_cell(g, qb.x, qb.y, qb.z);
     FeatureLoads ln;
-    feature_loads(g, counts, qn, grp, ln);
+    feature_loads<MORTON>(g, counts, qn, grp, ln);
     const float4 qc = load_q(t + 2 * nwaves);
     bf16x4 x[2];
-    feature_frag(g, qa, grp, count_scale, la, x);
+    feature_frag<MORTON>(qa, grp, count_scale, la, x);
     const float o = mlp_tile(w, x);
     const uint32_t qi = t * 16 + row;
     if (grp == 0 && qi < nq) {
@@ -575,11 +585,16 @@ hipError_t mlp_forward(const MlpState& m, const float* feats, uint64_t n, float*
 }
 
 hipError_t mlp_propose(const MlpState& m, const float4* qproc, uint32_t nq,
-                       const GridDesc* coarse, const uint32_t* counts, float r, float* qradius,
-                       hipStream_t st, int num_cus) {
+                       const GridDesc* coarse, bool morton, const uint32_t* counts, float r,
+                       float* qradius, hipStream_t st, int num_cus) {
   if (nq == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_mlp_propose, persistent_grid((nq + 15) / 16, num_cus), dim3(256), 0, st,
-                     m.params, qproc, nq, coarse, counts, m.count_scale, m.margin, r, qradius);
+  const dim3 grid = persistent_grid((nq + 15) / 16, num_cus);
+  if (morton)
+    hipLaunchKernelGGL(k_mlp_propose<true>, grid, dim3(256), 0, st, m.params, qproc, nq, coarse,
+                       counts, m.count_scale, m.margin, r, qradius);
+  else
+    hipLaunchKernelGGL(k_mlp_propose<false>, grid, dim3(256), 0, st, m.params, qproc, nq, coarse,
+                       counts, m.count_scale, m.margin, r, qradius);
   return hipGetLastError();
 }
 

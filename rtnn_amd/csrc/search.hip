@@ -97,8 +97,10 @@ __device__ inline void append_short(const SearchArgs& a, bool is_short, uint32_t
   if (is_short) a.short_rows[base + (uint32_t)__popcll(m & lanemask_lt64())] = qs;
 }
 
-// Candidates loaded per lane before the first is tested.
+// Candidates loaded per lane before the first is tested (thread kernel;
+// group kernel: its G lanes share a query, so each lane takes more).
 constexpr int kUnroll = 4;
+constexpr int kGroupUnroll = 8;
 
 // --------------------------------------------------------------------------
 // thread per query
@@ -414,12 +416,12 @@ __global__ __launch_bounds__(kTileQ) void k_topk_group(const SearchArgs a) {
         if (l == 0) ro[0] = 0;
         wave_sync();
         int rp = 0;
-        for (uint32_t v0 = 0; v0 < total; v0 += G * kUnroll) {
-          // Issue the loads of kUnroll candidates per lane before testing any,
-          // so each wave keeps several L2/HBM requests in flight.
-          float4 pv[kUnroll];
+        for (uint32_t v0 = 0; v0 < total; v0 += G * kGroupUnroll) {
+          // Issue the loads of kGroupUnroll candidates per lane before testing
+          // any, so each wave keeps several L2/HBM requests in flight.
+          float4 pv[kGroupUnroll];
 #pragma unroll
-          for (int u = 0; u < kUnroll; ++u) {
+          for (int u = 0; u < kGroupUnroll; ++u) {
             const uint32_t v = v0 + (uint32_t)(u * G + l);
             pv[u] = make_float4(0.f, 0.f, 0.f, 0.f);
             if (v < total) {
@@ -428,7 +430,7 @@ __global__ __launch_bounds__(kTileQ) void k_topk_group(const SearchArgs a) {
             }
           }
 #pragma unroll
-          for (int u = 0; u < kUnroll; ++u) {
+          for (int u = 0; u < kGroupUnroll; ++u) {
             const uint32_t v = v0 + (uint32_t)(u * G + l);
             group_take<G, C::CAP, KNN>(pv[u], v < total, qv, R2, gmask, buf, cnt, thr, k, l,
                                        &s_thr[gq]);
