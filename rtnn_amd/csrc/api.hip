@@ -753,11 +753,13 @@ int proposal_search(rtnn_ctx ctx, Proposal kind, const float* qxyz, uint64_t nq,
   OutBufs o;
   rc = output_targets(ctx, nq, k, out_idx, out_d2, out_on_device, o);
   if (rc) return rc;
-  // Two short-row lists (positions into qproc) and their counters.
-  CK(ctx->shortrows.ensure(4 * (2 * nq + 64)));
+  // Two short-row lists (positions into qproc), the hits each row of the
+  // first list found, and the two counters.
+  CK(ctx->shortrows.ensure(4 * (3 * nq + 64)));
   uint32_t* short1 = ctx->shortrows.as<uint32_t>();
   uint32_t* short2 = short1 + nq;
-  uint32_t* counters = short1 + 2 * nq;
+  uint32_t* hits1 = short1 + 2 * nq;
+  uint32_t* counters = short1 + 3 * nq;
   CK(fill_u32(counters, 2, 0, ctx->st));
   const int gl = group_lanes_from_opt(ctx, RTNN_MODE_KNN, k);
   if (gl < 0 || k > max_k_for_group(gl))
@@ -767,14 +769,17 @@ int proposal_search(rtnn_ctx ctx, Proposal kind, const float* qxyz, uint64_t nq,
   a.qradius = ctx->qradius.as<float>();
   a.short_rows = short1;
   a.short_count = &counters[0];
+  a.short_hits = hits1;
   CK(launch_search(a, 1, gl, ctx->st, ctx->num_cus, wide_buffers(ctx), ctx->opt.lds_tile != 0));
   CK(hipEventRecord(ctx->ev[3], ctx->st));
-  // Stage 2: short rows at min(r, 2 r'); stage 3: what is still short, at r.
-  // These rows are few and their radii large (many candidates each): the
-  // thread-per-query kernel, whose register top-k rejects most candidates
-  // with one compare, beats the group kernels there (k <= 64 here).
+  // Stage 2: short rows at min(r, r' * s), s from the hits the row found
+  // (hit_scale: 1.15 .. 2); stage 3: what is still short, at r.  These rows
+  // are few and their radii large (many candidates each): the thread-per-query
+  // kernel, whose register top-k rejects most candidates with one compare,
+  // beats the group kernels there (k <= 64 here).
   SearchArgs s2 = base_args(ctx, qproc, nq, r, k, o);
   s2.qsel = short1;
+  s2.qsel_hits = hits1;
   s2.nq_dev = &counters[0];
   s2.qradius = ctx->qradius.as<float>();
   s2.qradius_scale = 2.0f;

@@ -57,6 +57,12 @@ struct SearchArgs {
   float* out_d2;             // nullable
   uint32_t* short_rows;      // nullable: positions (in queries[]) with < k hits
   uint32_t* short_count;     // counter for short_rows
+  uint32_t* short_hits;      // nullable: hits found by each short row (beside short_rows)
+  // Fallback launch (qsel = a short list of an earlier launch at radius
+  // qradius[.]): when set, query i of this launch found qsel_hits[i] < k hits
+  // there, and its radius grows by hit_scale(qsel_hits[i], k) instead of
+  // qradius_scale.
+  const uint32_t* qsel_hits;
   int local_index;           // batched: report cloud-local point indices
   int persistent;            // grid-stride over tiles (XCD-chunked)
   // Hierarchical Morton grid (nlevels > 0): grids[l], l = 0 (finest) ..
@@ -69,6 +75,18 @@ struct SearchArgs {
 };
 
 constexpr int kTileQ = 256;  // queries per tile (= threads of a KNN block)
+
+// Radius growth of a row that found m < k hits within R: the k-th distance
+// scales like (k / m)^(1/2) for points on surfaces and (k / m)^(1/3) in volume,
+// so R * 1.15 * sqrt(k / m), kept in [1.15 R, 2 R] (2 R when nothing was
+// found), usually holds the k-th neighbour at a fraction of the volume of 2 R.
+// A row that still falls short goes on to the full radius, so the result does
+// not depend on this estimate.
+__device__ inline float hit_scale(uint32_t m, uint32_t k) {
+  if (m == 0) return 2.0f;
+  const float s = 1.15f * sqrtf((float)k / (float)m);
+  return fminf(fmaxf(s, 1.15f), 2.0f);
+}
 
 __host__ __device__ inline float d2_of(float qx, float qy, float qz, float px, float py,
                                        float pz) {

@@ -87,14 +87,27 @@ struct TileIter {
 // Appends the processing positions of short rows (fewer than k hits) with
 // one atomic per wave; a wave's entries stay together and in lane order, so
 // the re-run over the list keeps the Morton locality of the queries.
-__device__ inline void append_short(const SearchArgs& a, bool is_short, uint32_t qs) {
+__device__ inline void append_short(const SearchArgs& a, bool is_short, uint32_t qs,
+                                    uint32_t hits) {
   const uint64_t m = __ballot(is_short);
   if (m == 0) return;
   const int leader = __ffsll((unsigned long long)m) - 1;
   uint32_t base = 0;
-  if ((int)(threadIdx.x & (kWave - 1)) == leader) base = atomicAdd(a.short_count, (uint32_t)__popcll(m));
+  if ((int)(threadIdx.x & (kWave - 1)) == leader)
+    base = atomicAdd(a.short_count, (uint32_t)__popcll(m));
   base = __shfl(base, leader, kWave);
-  if (is_short) a.short_rows[base + (uint32_t)__popcll(m & lanemask_lt64())] = qs;
+  if (is_short) {
+    const uint32_t pos = base + (uint32_t)__popcll(m & lanemask_lt64());
+    a.short_rows[pos] = qs;
+    if (a.short_hits) a.short_hits[pos] = hits;
+  }
+}
+
+// Search radius of launch position qi (query qs).
+__device__ inline float query_radius(const SearchArgs& a, uint32_t qi, uint32_t qs) {
+  if (!a.qradius) return a.r;
+  const float sc = a.qsel_hits ? hit_scale(a.qsel_hits[qi], a.k) : a.qradius_scale;
+  return fminf(a.r, a.qradius[qs] * sc);
 }
 
 // Candidates loaded per lane before the first is tested (thread kernel;
@@ -128,7 +141,7 @@ __global__ __launch_bounds__(kTileQ) void k_topk_thread(const SearchArgs a) {
     const GridDesc g = a.grids[gi];
     const float4 qv = active ? a.queries[qs] : make_float4(0.f, 0.f, 0.f, 0.f);
     const uint32_t row = __float_as_uint(qv.w);
-    const float R = a.qradius && active ? fminf(a.r, a.qradius[qs] * a.qradius_scale) : a.r;
+    const float R = active ? query_radius(a, qi, qs) : a.r;
     const float R2 = R * R;
     int lo0, hi0, lo1, hi1, lo2, hi2;
     cell_range(g, qv.x, R, 0, lo0, hi0);
@@ -213,11 +226,10 @@ __global__ __launch_bounds__(kTileQ) void k_topk_thread(const SearchArgs a) {
       }
     }
     if (a.short_rows) {
-      bool is_short = false;
+      uint32_t hits = 0;
 #pragma unroll
-      for (int i = 0; i < KC; ++i)
-        if ((uint32_t)i == k - 1) is_short = list[i] == ~0ull;
-      append_short(a, is_short, qs);
+      for (int i = 0; i < KC; ++i) hits += ((uint32_t)i < k && list[i] != ~0ull) ? 1u : 0u;
+      append_short(a, hits < k, qs, hits);
     }
   }
 }
@@ -338,7 +350,7 @@ __device__ inline void group_finish(const SearchArgs& a, const uint64_t* buf, ui
     oi[s] = -1;
     if (od) od[s] = __builtin_huge_valf();
   }
-  if (a.short_rows) append_short(a, l == 0 && cnt < k, qs);
+  if (a.short_rows) append_short(a, l == 0 && cnt < k, qs, cnt);
 }
 
 template <int G, int KB, bool KNN, bool HIER, bool WIDE>
@@ -367,7 +379,7 @@ __global__ __launch_bounds__(kTileQ) void k_topk_group(const SearchArgs a) {
       const uint32_t qs = a.qsel ? a.qsel[qi] : qi;
       const float4 qv = a.queries[qs];
       const uint32_t row = __float_as_uint(qv.w);
-      const float R = a.qradius ? fminf(a.r, a.qradius[qs] * a.qradius_scale) : a.r;
+      const float R = query_radius(a, qi, qs);
       const float R2 = R * R;
       int lo0, hi0, lo1, hi1, lo2, hi2;
       cell_range(g, qv.x, R, 0, lo0, hi0);
@@ -513,7 +525,7 @@ __global__ __launch_bounds__(kTileQ) void k_topk_tiled(const SearchArgs a) {
         qs = a.qsel ? a.qsel[qi] : qi;
         qv = a.queries[qs];
         row = __float_as_uint(qv.w);
-        const float R = a.qradius ? fminf(a.r, a.qradius[qs] * a.qradius_scale) : a.r;
+        const float R = query_radius(a, qi, qs);
         R2 = R * R;
         cell_range(g, qv.x, R, 0, lo0, hi0);
         cell_range(g, qv.y, R, 1, lo1, hi1);
