@@ -214,3 +214,23 @@ def test_train_on_query_sample_exact(ctx):
     assert np.isfinite(loss)
     _plain_vs(ctx, qs, 15.0, 8, ctx.search_learned)
     _plain_vs(ctx, qs, 15.0, 8, ctx.search_partitioned)
+
+
+def test_levy_uniform_queries_k32_large_radius(ctx):
+    # Config-5 shape at 1/30 of the points: Levy-flight points (alpha 1.5,
+    # min step 0.01, periodic [0,1000)^3), uniform queries, K = 32, r = 100
+    # (near-unbounded).  Learned (trained on a query sample) and partitioned
+    # KNN equal the plain KNN on every row; sampled rows equal the oracle.
+    import oracle
+    pts = rtnn_amd.gen_levy(5, 1_000_000, alpha=1.5, min_step=0.01, box=1000.0)
+    qs = rtnn_amd.gen_uniform(51, 60_000, 0.0, 1000.0)
+    ctx.set_points(pts)
+    pi, pd, _ = ctx.search(qs, mode="knn", radius=100.0, k=32)
+    ctx.mlp_train(100.0, 32, steps=300, batch=2048, seed=2, queries=qs[:20000])
+    for fn in (ctx.search_learned, ctx.search_partitioned):
+        li, ld, _ = fn(qs, radius=100.0, k=32)
+        assert np.array_equal(pi, li) and np.array_equal(pd, ld)
+    sel = np.random.default_rng(3).choice(len(qs), 300, replace=False).astype(np.uint64)
+    wi, wd = oracle.bruteforce(pts, qs, oracle.MODE_KNN, 100.0, 32, qsel=sel, threads=8)
+    s = sel.astype(np.int64)
+    assert np.array_equal(pi[s], wi) and np.array_equal(pd[s], wd)
