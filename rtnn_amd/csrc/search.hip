@@ -111,9 +111,11 @@ __device__ inline float query_radius(const SearchArgs& a, uint32_t qi, uint32_t 
 }
 
 // Candidates loaded per lane before the first is tested (thread kernel;
-// group kernel: its G lanes share a query, so each lane takes more).
+// group kernel: its G lanes share a query, so each lane takes more), and
+// runs (rows or cells) whose bounds each group lane loads per batch.
 constexpr int kUnroll = 4;
 constexpr int kGroupUnroll = 8;
+constexpr int kRunsPerLane = 4;
 
 // --------------------------------------------------------------------------
 // thread per query
@@ -357,8 +359,11 @@ template <int G, int KB, bool KNN, bool HIER, bool WIDE>
 __global__ __launch_bounds__(kTileQ) void k_topk_group(const SearchArgs a) {
   using C = GroupCfg<G, KB, WIDE>;
   __shared__ uint64_t s_buf[C::NG][C::CAP];
-  __shared__ uint32_t s_rs[C::NG][G];
-  __shared__ uint32_t s_ro[C::NG][G + 1];
+  // Each lane sets up kRunsPerLane runs per batch (their bounds loaded
+  // together: one memory round trip for up to G * kRunsPerLane runs).
+  constexpr int RB = G * kRunsPerLane;
+  __shared__ uint32_t s_rs[C::NG][RB];
+  __shared__ uint32_t s_ro[C::NG][RB + 1];
   __shared__ uint64_t s_thr[C::NG];
 
   const int gq = threadIdx.x / G;
@@ -403,29 +408,48 @@ __global__ __launch_bounds__(kTileQ) void k_topk_group(const SearchArgs a) {
 
       uint32_t cnt = 0;
       uint64_t thr = ~0ull;
-      for (int rb = 0; rb < nrows; rb += G) {
-        const int j = rb + l;
-        uint32_t s0 = 0, len = 0;
-        if (j < nrows) {
-          if (HIER) {
-            const int cx = lo0 + j % nx, t = j / nx;
-            const int cy = lo1 + t % ny, cz = lo2 + t / ny;
-            const uint32_t id = lvl_base + morton3((uint32_t)cx, (uint32_t)cy, (uint32_t)cz);
-            s0 = a.cell_start[id];
-            len = a.cell_start[id + 1] - s0;
-          } else {
-            const int cy = lo1 + j % ny, cz = lo2 + j / ny;
-            const uint32_t rowc = lvl_base + ((uint32_t)cz * (uint32_t)g.dim[1] + (uint32_t)cy) *
-                                                 (uint32_t)g.dim[0];
-            s0 = a.cell_start[rowc + lo0];
-            len = a.cell_start[rowc + hi0 + 1] - s0;
+      for (int rb = 0; rb < nrows; rb += RB) {
+        // runs rb + l * kRunsPerLane + u of this batch
+        uint32_t s0[kRunsPerLane], len[kRunsPerLane];
+#pragma unroll
+        for (int u = 0; u < kRunsPerLane; ++u) {
+          const int j = rb + l * kRunsPerLane + u;
+          s0[u] = 0;
+          len[u] = 0;
+          if (j < nrows) {
+            uint32_t b0, b1;
+            if (HIER) {
+              const int cx = lo0 + j % nx, t = j / nx;
+              const int cy = lo1 + t % ny, cz = lo2 + t / ny;
+              b0 = lvl_base + morton3((uint32_t)cx, (uint32_t)cy, (uint32_t)cz);
+              b1 = b0 + 1;
+            } else {
+              const int cy = lo1 + j % ny, cz = lo2 + j / ny;
+              const uint32_t rowc = lvl_base + ((uint32_t)cz * (uint32_t)g.dim[1] +
+                                                (uint32_t)cy) * (uint32_t)g.dim[0];
+              b0 = rowc + lo0;
+              b1 = rowc + hi0 + 1;
+            }
+            s0[u] = a.cell_start[b0];
+            len[u] = a.cell_start[b1];
           }
         }
-        const uint32_t incl = group_incl_scan<G>(len, l);
+        uint32_t mine = 0;
+#pragma unroll
+        for (int u = 0; u < kRunsPerLane; ++u) {
+          len[u] -= s0[u];
+          mine += len[u];
+        }
+        const uint32_t incl = group_incl_scan<G>(mine, l);
         const uint32_t total = __shfl(incl, G - 1, G);
-        rs[l] = s0;
-        ro[l + 1] = incl;
-        if (l == 0) ro[0] = 0;
+        uint32_t run = incl - mine;
+#pragma unroll
+        for (int u = 0; u < kRunsPerLane; ++u) {
+          rs[l * kRunsPerLane + u] = s0[u];
+          ro[l * kRunsPerLane + u] = run;
+          run += len[u];
+        }
+        if (l == G - 1) ro[RB] = total;
         wave_sync();
         int rp = 0;
         for (uint32_t v0 = 0; v0 < total; v0 += G * kGroupUnroll) {
