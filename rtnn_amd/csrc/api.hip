@@ -199,7 +199,7 @@ int group_lanes_from_opt(rtnn_ctx ctx, int mode, uint32_t k) {
   int qpw = ctx->opt.queries_per_wave;
   if (qpw <= 0) return default_group_lanes(mode, k);
   int gl = 64 / qpw;
-  if (gl != 1 && gl != 4 && gl != 16 && gl != 64) return -1;
+  if (gl != 1 && gl != 2 && gl != 4 && gl != 8 && gl != 16 && gl != 64) return -1;
   return gl;
 }
 
@@ -1300,14 +1300,14 @@ int rtnn_autotune(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_device, i
   const rtnn_options base = ctx->opt;
   static const float kPlainCf[] = {1.0f, 2.0f, 3.0f};
   static const float kPropCf[] = {1.5f, 2.0f, 2.5f, 3.0f, 4.0f};
-  static const int kQpwKnn[] = {0, 16, 4};
-  static const int kQpwRange[] = {64, 16, 4, 1};
+  static const int kQpwKnn[] = {0, 32, 16, 8, 4};
+  static const int kQpwRange[] = {64, 32, 16, 8, 4, 1};
   const bool plain = method == RTNN_METHOD_PLAIN;
   const float* cfs = plain ? kPlainCf : kPropCf;
   const int ncf = plain ? 3 : 5;
   const bool range = plain && mode == RTNN_MODE_RADIUS;
   const int* qpws = range ? kQpwRange : kQpwKnn;
-  const int nqpw = range ? 4 : 3;
+  const int nqpw = range ? 6 : 5;
   double best_t = 1e300;
   rtnn_options best_opt = base;
   // Candidates: raster grids (cell factor x lanes per query); for the radius
@@ -1326,7 +1326,8 @@ int rtnn_autotune(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_device, i
       cands.push_back({cfs[i], qpws[j], plain ? 0 : 1, 0.0f, group ? 2 : 0, 0});
       if (group) {
         cands.push_back({cfs[i], qpws[j], plain ? 0 : 1, 0.0f, 1, 0});
-        cands.push_back({cfs[i], qpws[j], plain ? 0 : 1, 0.0f, 1, 1});
+        if (qpws[j] != 32)  // 2-lane groups have no tiled kernel
+          cands.push_back({cfs[i], qpws[j], plain ? 0 : 1, 0.0f, 1, 1});
       }
     }
   if (!plain)
@@ -1407,10 +1408,10 @@ int rtnn_autotune_batched(rtnn_ctx ctx, const float* xyz, const uint64_t* offset
   // wave 1, 4, 16, 64) x the group kernel's LDS use (wide buffer, compact
   // buffer, compact buffer + wave tile of candidates).
   for (float cf : {0.5f, 1.0f, 2.0f})
-    for (int qpw : {1, 4, 16, 64})
+    for (int qpw : {1, 4, 8, 16, 32, 64})
       for (int lds : {0, 1, 2}) {
         const int gl = 64 / qpw;
-        if (k > max_k_for_group(gl) || (gl == 1 && lds > 0)) continue;
+        if (k > max_k_for_group(gl) || (gl == 1 && lds > 0) || (gl == 2 && lds == 2)) continue;
         ctx->opt = base;
         ctx->opt.cell_factor = cf;
         ctx->opt.queries_per_wave = qpw;

@@ -17,7 +17,7 @@
 // k_topk_thread : one query per lane.  A sorted array of k 64-bit keys
 //                 (d2 bits << 32 | index) lives in VGPRs; insertion is an
 //                 unrolled compare/select pass, so nothing spills to scratch.
-// k_topk_group  : G lanes per query (G = 4, 16, 64).  The lanes stride over the
+// k_topk_group  : G lanes per query (G = 2, 4, 8, 16, 64).  The lanes stride over the
 //                 query's runs flattened into one index space, test 16-byte
 //                 candidates with coalesced loads, and append hits to an LDS
 //                 buffer with one wave64 ballot + prefix popcount.  When the
@@ -115,7 +115,7 @@ __device__ inline float query_radius(const SearchArgs& a, uint32_t qi, uint32_t 
 // runs (rows or cells) whose bounds each group lane loads per batch.
 constexpr int kUnroll = 4;
 constexpr int kGroupUnroll = 8;
-constexpr int kRunsPerLane = 4;
+constexpr int kRunsPerLane = 1;
 
 // --------------------------------------------------------------------------
 // thread per query
@@ -659,12 +659,14 @@ template <int G, int KB>
 static void launch_group(const SearchArgs& a, bool knn, bool wide, bool tiled, dim3 grid,
                          hipStream_t st) {
   const bool hier = a.nlevels > 0;
-  if (tiled && !hier) {
-    if (knn)
-      hipLaunchKernelGGL((k_topk_tiled<G, KB, true>), grid, dim3(kTileQ), 0, st, a);
-    else
-      hipLaunchKernelGGL((k_topk_tiled<G, KB, false>), grid, dim3(kTileQ), 0, st, a);
-    return;
+  if constexpr (G >= 4) {  // 2-lane groups: no room for a tile beside their buffers
+    if (tiled && !hier) {
+      if (knn)
+        hipLaunchKernelGGL((k_topk_tiled<G, KB, true>), grid, dim3(kTileQ), 0, st, a);
+      else
+        hipLaunchKernelGGL((k_topk_tiled<G, KB, false>), grid, dim3(kTileQ), 0, st, a);
+      return;
+    }
   }
   if (knn && hier)
     hipLaunchKernelGGL((k_topk_group<G, KB, true, true, true>), grid, dim3(kTileQ), 0, st, a);
@@ -683,7 +685,9 @@ static void launch_group(const SearchArgs& a, bool knn, bool wide, bool tiled, d
 uint32_t max_k_for_group(int group_lanes) {
   switch (group_lanes) {
     case 1: return 64;
+    case 2: return 16;
     case 4: return 64;
+    case 8: return 64;
     case 16: return 256;
     case 64: return 1024;
     default: return 0;
@@ -719,9 +723,14 @@ hipError_t launch_search(const SearchArgs& a0, int knn_mode, int group_lanes, hi
     else if (k <= 16) launch_thread<16>(a, knn, grid, st);
     else if (k <= 32) launch_thread<32>(a, knn, grid, st);
     else launch_thread<64>(a, knn, grid, st);
+  } else if (group_lanes == 2) {
+    launch_group<2, 16>(a, knn, wide, tiled, grid, st);
   } else if (group_lanes == 4) {
     if (k <= 16) launch_group<4, 16>(a, knn, wide, tiled, grid, st);
     else launch_group<4, 64>(a, knn, wide, tiled, grid, st);
+  } else if (group_lanes == 8) {
+    if (k <= 16) launch_group<8, 16>(a, knn, wide, tiled, grid, st);
+    else launch_group<8, 64>(a, knn, wide, tiled, grid, st);
   } else if (group_lanes == 16) {
     if (k <= 16) launch_group<16, 16>(a, knn, wide, tiled, grid, st);
     else if (k <= 64) launch_group<16, 64>(a, knn, wide, tiled, grid, st);

@@ -18,7 +18,7 @@ pytestmark = pytest.mark.gpu
 
 MODES = {"radius": oracle.MODE_RADIUS, "knn": oracle.MODE_KNN}
 # lanes per query (64 / queries_per_wave) -> largest k of that kernel
-MAX_K = {1: 64, 4: 64, 16: 256, 64: 1024}
+MAX_K = {1: 64, 2: 16, 4: 64, 8: 64, 16: 256, 64: 1024}
 
 
 def check(ctx, pts, qs, mode, r, k, qsel=None, **search_kw):
@@ -45,7 +45,7 @@ def _reset_opts(ctx):
 def test_samplepc_golden(ctx, golden, samplepc):
     ctx.set_points(samplepc)
     for case in golden["cases"]:
-        for qpw in (0, 1, 4, 16, 64):
+        for qpw in (0, 1, 4, 8, 16, 32, 64):
             if qpw and case["k"] > MAX_K[64 // qpw]:
                 continue
             ctx.set_options(queries_per_wave=qpw)
@@ -73,7 +73,7 @@ def test_config1_range_full(ctx):
     check(ctx, pts, None, "knn", 2.0, 50)
 
 
-@pytest.mark.parametrize("qpw", [1, 4, 16, 64])
+@pytest.mark.parametrize("qpw", [1, 4, 8, 16, 32, 64])
 @pytest.mark.parametrize("cf", [0.5, 1.0, 2.0, 3.0])
 def test_kernel_variants_and_cell_factors(ctx, qpw, cf):
     rng = np.random.default_rng(int(cf * 10) + qpw)
@@ -81,11 +81,13 @@ def test_kernel_variants_and_cell_factors(ctx, qpw, cf):
     qs = np.concatenate([pts[:3000], rng.uniform(-3, 23, (2000, 3)).astype(np.float32)])
     ctx.set_points(pts)
     # group kernels: wide and compact LDS buffers, and the wave-tiled kernel
-    for buf, tile in (((0, 0), (1, 0), (1, 1)) if qpw in (1, 4, 16) else ((0, 0),)):
+    for buf, tile in (((0, 0), (1, 0), (1, 1)) if qpw in (1, 4, 8, 16) else
+                      ((0, 0), (1, 0)) if qpw == 32 else ((0, 0),)):
         ctx.set_options(cell_factor=cf, queries_per_wave=qpw, group_buffer=buf, lds_tile=tile)
         for mode in ("radius", "knn"):
             for k in (1, 5, 16, 33, 64):
-                check(ctx, pts, qs, mode, 1.5, k)
+                if k <= MAX_K[64 // qpw]:
+                    check(ctx, pts, qs, mode, 1.5, k)
 
 
 def test_large_k_group_kernel(ctx):
@@ -107,10 +109,13 @@ def test_group_buffer_overflow_path(ctx):
     ctx.set_points(pts)
     # wide (2k + G keys) and compact (k + 2G keys) buffers, and the tiled kernel
     for buf, tile in ((2, 0), (1, 0), (1, 1)):
-        for qpw in (1, 4, 16):  # 64, 16 and 4 lanes per query
+        for qpw in (1, 4, 8, 16, 32):  # 64, 16, 8, 4 and 2 lanes per query
+            if qpw == 32 and tile:
+                continue
             ctx.set_options(queries_per_wave=qpw, group_buffer=buf, lds_tile=tile)
             check(ctx, pts, pts[:1500], "radius", 2.0, 16)
-            check(ctx, pts, pts[:1500], "knn", 2.0, 64)
+            if MAX_K[64 // qpw] >= 64:
+                check(ctx, pts, pts[:1500], "knn", 2.0, 64)
             check(ctx, pts, pts[:300], "radius", 2.0, 5)
 
 
