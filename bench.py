@@ -171,9 +171,15 @@ def main():
     ap.add_argument("--train-steps", type=int, default=2000)
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="time the default configuration instead of the autotuned one")
+    ap.add_argument("--options", default=None,
+                    help="JSON of rtnn_options fields to time instead of autotuning, "
+                         "e.g. '{\"queries_per_wave\": 16, \"group_buffer\": 1}'")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (scale factor for HBM counters)")
     args = ap.parse_args()
+    fixed_opts = json.loads(args.options) if args.options else None
+    if fixed_opts is not None:
+        args.autotune = False
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has a single MI355X and the "
               "search needs no collective (queries shard with replicated points); run --gpus 1.",
@@ -234,7 +240,7 @@ def main():
                      "lr": 1e-3, "tau": 0.9}
             log(f"[bench] MLP trained on the GPU in {train['seconds']} s, loss {loss:.5f}")
             fn = lambda: ctx.search_learned(dq, radius=r, k=k, out_device=True)
-        ctx.set_options()
+        ctx.set_options(**(fixed_opts or {}))
         tuned = None
         if args.autotune and m in ("batched", "graph"):
             if batched_opts is None:

@@ -51,7 +51,9 @@ typedef struct rtnn_ctx_s* rtnn_ctx;
 /* Tuning knobs; zero means "choose automatically". */
 typedef struct {
   float cell_factor;      /* grid cell edge = r / cell_factor                       */
-  int   queries_per_wave; /* range kernel lanes per query = 64 / queries_per_wave  */
+  int   queries_per_wave; /* lanes per query = 64 / queries_per_wave: 64 = one query
+                             per lane (k <= 64), 32/16/8/4/1 = groups of 2/4/8/16/64
+                             lanes (k <= 16/64/64/256/1024); 0 = auto             */
   int   query_order;      /* 0 auto, 1 input order, 2 Morton order of query cells  */
   int   time_stages;      /* 1: record hipEvent timings into rtnn_stats             */
   uint32_t max_cells;     /* cap on grid cells (cell edge grows to respect it)      */

@@ -513,9 +513,9 @@ int rtnn_set_options(rtnn_ctx ctx, const rtnn_options* opt) {
   if (!ctx || !opt) return RTNN_ERR_INVALID;
   if (opt->cell_factor < 0 || !std::isfinite(opt->cell_factor))
     return fail(ctx, RTNN_ERR_INVALID, "cell_factor must be >= 0");
-  if (opt->queries_per_wave != 0 && opt->queries_per_wave != 1 && opt->queries_per_wave != 4 &&
-      opt->queries_per_wave != 16 && opt->queries_per_wave != 64)
-    return fail(ctx, RTNN_ERR_INVALID, "queries_per_wave must be 0, 1, 4, 16 or 64");
+  const int qpw = opt->queries_per_wave;
+  if (qpw != 0 && qpw != 1 && qpw != 4 && qpw != 8 && qpw != 16 && qpw != 32 && qpw != 64)
+    return fail(ctx, RTNN_ERR_INVALID, "queries_per_wave must be 0, 1, 4, 8, 16, 32 or 64");
   if (opt->query_order < 0 || opt->query_order > 2)
     return fail(ctx, RTNN_ERR_INVALID, "query_order must be 0, 1 or 2");
   if (opt->grid_levels < 0 || opt->grid_levels > 8)
