@@ -195,11 +195,11 @@ __global__ __launch_bounds__(kTileQ) void k_topk_thread(const SearchArgs a) {
       const uint32_t s = s_next, e = e_next;
       if (rr + 1 < nrows) row_bounds(rr + 1, s_next, e_next);
       for (uint32_t j0 = s; j0 < e; j0 += kUnroll) {
-        // kUnroll loads in flight before the first test
+        // kUnroll loads in flight before the first test: unconditional loads
+        // (a slot past the run reloads its last point and is not tested).
         float4 pv[kUnroll];
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u)
-          pv[u] = j0 + u < e ? a.spts[j0 + u] : make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int u = 0; u < kUnroll; ++u) pv[u] = a.spts[min(j0 + (uint32_t)u, e - 1)];
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u) {
           if (j0 + u >= e) break;
@@ -453,18 +453,23 @@ __global__ __launch_bounds__(kTileQ) void k_topk_group(const SearchArgs a) {
         wave_sync();
         int rp = 0;
         for (uint32_t v0 = 0; v0 < total; v0 += G * kGroupUnroll) {
-          // Issue the loads of kGroupUnroll candidates per lane before testing
-          // any, so each wave keeps several L2/HBM requests in flight.
-          float4 pv[kGroupUnroll];
+          // Positions of kGroupUnroll candidates per lane first (LDS walk),
+          // then their loads as one straight-line batch with no branch in
+          // between, so all of them are in flight before the first test.  A
+          // slot past the end loads position 0 (valid) and is not tested.
+          uint32_t pos[kGroupUnroll];
 #pragma unroll
           for (int u = 0; u < kGroupUnroll; ++u) {
             const uint32_t v = v0 + (uint32_t)(u * G + l);
-            pv[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+            pos[u] = 0;
             if (v < total) {
               while (ro[rp + 1] <= v) ++rp;
-              pv[u] = a.spts[rs[rp] + (v - ro[rp])];
+              pos[u] = rs[rp] + (v - ro[rp]);
             }
           }
+          float4 pv[kGroupUnroll];
+#pragma unroll
+          for (int u = 0; u < kGroupUnroll; ++u) pv[u] = a.spts[pos[u]];
 #pragma unroll
           for (int u = 0; u < kGroupUnroll; ++u) {
             const uint32_t v = v0 + (uint32_t)(u * G + l);
