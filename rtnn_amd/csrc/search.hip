@@ -249,6 +249,63 @@ __device__ inline uint32_t group_incl_scan(uint32_t v, int l) {
   return v;
 }
 
+// DPP lane moves (VALU only, no LDS round trip).  CTRL: 0x00-0xff quad_perm,
+// 0x110 + n row_shr:n, 0x141 row_half_mirror.
+template <int CTRL>
+__device__ inline uint32_t dpp_mov(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xf, 0xf, true);
+}
+
+// Inclusive scan over the G lanes of a group.  G <= 16 divides the 16-lane
+// DPP row, so row_shr never reads across a group boundary for lanes l >= n.
+template <int G>
+__device__ inline uint32_t group_scan_fast(uint32_t v, int l) {
+  if constexpr (G > 16) {
+    return group_incl_scan<G>(v, l);
+  } else {
+    // Every lane executes each move (a lane with EXEC off would read as 0);
+    // only the add is predicated.
+    uint32_t t = dpp_mov<0x111>(v);
+    v += l >= 1 ? t : 0u;
+    if constexpr (G > 2) {
+      t = dpp_mov<0x112>(v);
+      v += l >= 2 ? t : 0u;
+    }
+    if constexpr (G > 4) {
+      t = dpp_mov<0x114>(v);
+      v += l >= 4 ? t : 0u;
+    }
+    if constexpr (G > 8) {
+      t = dpp_mov<0x118>(v);
+      v += l >= 8 ? t : 0u;
+    }
+    return v;
+  }
+}
+
+// Value of the group's last lane, in every lane of the group.
+template <int G>
+__device__ inline uint32_t group_last(uint32_t v, int l) {
+  if constexpr (G == 2) {
+    return dpp_mov<0xF5>(v);  // quad_perm [1, 1, 3, 3]
+  } else if constexpr (G == 4) {
+    return dpp_mov<0xFF>(v);  // quad_perm [3, 3, 3, 3]
+  } else if constexpr (G == 8) {
+    const uint32_t y = dpp_mov<0xFF>(v);  // lanes 4-7 hold lane 7
+    const uint32_t z = dpp_mov<0x141>(y);  // lanes 0-3 read lanes 7-4
+    return l >= 4 ? y : z;
+  } else {
+    return __shfl(v, G - 1, G);
+  }
+}
+
+// d2 < R2 as d2 <= r2_limit(R2): the float just below R2 (-1 for R2 <= 0,
+// FLT_MAX for +inf).  After a select the limit becomes the k-th key's d2:
+// ties with it are appended too and resolved by the (d2, index) rank.
+__device__ inline float r2_limit(float R2) {
+  return R2 > 0.0f ? __uint_as_float(__float_as_uint(R2) - 1u) : -1.0f;
+}
+
 template <int G, int KB, bool WIDE>
 struct GroupCfg {
   static constexpr int NG = kTileQ / G;  // groups per block
@@ -282,20 +339,20 @@ constexpr int kRankChunk = 8;
 
 // One step of a group of G lanes: lane l tests candidate p (if valid) and
 // the group appends its hits to the LDS buffer (ballot + prefix popcount).
-// When the buffer nears CAP, the k smallest are kept: the k-th smallest key T
-// is found by ranking, the entries <= T are compacted to the front in place
-// (a round writes below the entries it has read), and later hits must beat T.
+// Entry: cnt <= CAP - G.  When the buffer nears CAP, the k smallest are kept:
+// the k-th smallest key T is found by ranking, the entries <= T are
+// compacted to the front in place (a round writes below the entries it has
+// read), and later hits must have d2 <= d2(T).
 template <int G, int CAP, bool KNN>
-__device__ inline void group_take(const float4& p, bool valid, const float4& qv, float R2,
-                                  uint64_t gmask, uint64_t* buf, uint32_t& cnt, uint64_t& thr,
+__device__ inline void group_take(const float4& p, bool valid, const float4& qv,
+                                  uint64_t gmask, uint64_t* buf, uint32_t& cnt, float& lim,
                                   uint32_t k, int l, uint64_t* thr_slot) {
   bool hit = false;
   uint64_t key = 0;
   if (valid) {
     const float d2 = d2_of(qv.x, qv.y, qv.z, p.x, p.y, p.z);
-    const bool ok = KNN ? (d2 > 0.0f && d2 < R2) : (d2 < R2);
+    hit = KNN ? (d2 > 0.0f && d2 <= lim) : (d2 <= lim);
     key = pack_key(d2, __float_as_uint(p.w));
-    hit = ok && key < thr;
   }
   const uint64_t m = __ballot(hit) & gmask;
   if (hit) buf[cnt + (uint32_t)__popcll(m & lanemask_lt64())] = key;
@@ -324,7 +381,7 @@ __device__ inline void group_take(const float4& p, bool valid, const float4& qv,
       wave_sync();
     }
     cnt = k;
-    thr = T;
+    lim = key_d2(T);
   }
 }
 
@@ -362,8 +419,10 @@ __global__ __launch_bounds__(kTileQ) void k_topk_group(const SearchArgs a) {
   // Each lane sets up kRunsPerLane runs per batch (their bounds loaded
   // together: one memory round trip for up to G * kRunsPerLane runs).
   constexpr int RB = G * kRunsPerLane;
-  __shared__ uint32_t s_rs[C::NG][RB];
-  __shared__ uint32_t s_ro[C::NG][RB + 1];
+  constexpr bool kRegRuns = RB <= 8;
+  __shared__ uint32_t s_rs[kRegRuns ? 1 : C::NG][RB];
+  __shared__ uint32_t s_ro[kRegRuns ? 1 : C::NG][RB + 1];
+  __shared__ __align__(16) uint2 s_rd[kRegRuns ? C::NG : 1][RB];
   __shared__ uint64_t s_thr[C::NG];
 
   const int gq = threadIdx.x / G;
@@ -371,8 +430,8 @@ __global__ __launch_bounds__(kTileQ) void k_topk_group(const SearchArgs a) {
   const int lane = threadIdx.x & (kWave - 1);
   const uint64_t gmask = G == kWave ? ~0ull : (((1ull << G) - 1ull) << (lane - l));
   uint64_t* buf = s_buf[gq];
-  uint32_t* rs = s_rs[gq];
-  uint32_t* ro = s_ro[gq];
+  uint32_t* rs = s_rs[kRegRuns ? 0 : gq];
+  uint32_t* ro = s_ro[kRegRuns ? 0 : gq];
   const uint32_t k = a.k;
 
   const uint32_t nq = launch_nq(a);
@@ -407,7 +466,7 @@ __global__ __launch_bounds__(kTileQ) void k_topk_group(const SearchArgs a) {
       const int nrows = (nx > 0 && ny > 0 && nz > 0) ? (HIER ? nx : 1) * ny * nz : 0;
 
       uint32_t cnt = 0;
-      uint64_t thr = ~0ull;
+      float lim = r2_limit(R2);
       for (int rb = 0; rb < nrows; rb += RB) {
         // runs rb + l * kRunsPerLane + u of this batch
         uint32_t s0[kRunsPerLane], len[kRunsPerLane];
@@ -440,41 +499,95 @@ __global__ __launch_bounds__(kTileQ) void k_topk_group(const SearchArgs a) {
           len[u] -= s0[u];
           mine += len[u];
         }
-        const uint32_t incl = group_incl_scan<G>(mine, l);
-        const uint32_t total = __shfl(incl, G - 1, G);
-        uint32_t run = incl - mine;
+        const uint32_t incl = group_scan_fast<G>(mine, l);
+        const uint32_t total = group_last<G>(incl, l);
+        // Run table: candidate v of the batch lies in run j = the last with
+        // off_j <= v, at point index v + delta_j (delta_j = start_j - off_j,
+        // mod 2^32).  Up to 8 runs it is held in VGPRs and searched with
+        // selects; beyond, walked in LDS.
+        uint2 rt[kRegRuns ? RB : 1];
+        {
+          uint32_t run = incl - mine;
 #pragma unroll
-        for (int u = 0; u < kRunsPerLane; ++u) {
-          rs[l * kRunsPerLane + u] = s0[u];
-          ro[l * kRunsPerLane + u] = run;
-          run += len[u];
+          for (int u = 0; u < kRunsPerLane; ++u) {
+            if constexpr (kRegRuns) {
+              s_rd[gq][l * kRunsPerLane + u] = make_uint2(run, s0[u] - run);
+            } else {
+              rs[l * kRunsPerLane + u] = s0[u];
+              ro[l * kRunsPerLane + u] = run;
+            }
+            run += len[u];
+          }
+          if constexpr (!kRegRuns) {
+            if (l == G - 1) ro[RB] = total;
+          }
         }
-        if (l == G - 1) ro[RB] = total;
         wave_sync();
+        if constexpr (kRegRuns) {
+          const uint4* t4 = reinterpret_cast<const uint4*>(s_rd[gq]);
+#pragma unroll
+          for (int j = 0; j < RB / 2; ++j) {
+            const uint4 w = t4[j];
+            rt[2 * j] = make_uint2(w.x, w.y);
+            rt[2 * j + 1] = make_uint2(w.z, w.w);
+          }
+        }
         int rp = 0;
         for (uint32_t v0 = 0; v0 < total; v0 += G * kGroupUnroll) {
-          // Positions of kGroupUnroll candidates per lane first (LDS walk),
-          // then their loads as one straight-line batch with no branch in
-          // between, so all of them are in flight before the first test.  A
-          // slot past the end loads position 0 (valid) and is not tested.
+          // Positions of kGroupUnroll candidates per lane first, then their
+          // loads as one straight-line batch with no branch in between, so
+          // all of them are in flight before the first test.  A slot past the
+          // end loads position 0 (valid) and is not a hit.
           uint32_t pos[kGroupUnroll];
 #pragma unroll
           for (int u = 0; u < kGroupUnroll; ++u) {
             const uint32_t v = v0 + (uint32_t)(u * G + l);
-            pos[u] = 0;
-            if (v < total) {
-              while (ro[rp + 1] <= v) ++rp;
-              pos[u] = rs[rp] + (v - ro[rp]);
+            if constexpr (kRegRuns) {
+              uint32_t d = rt[0].y;
+#pragma unroll
+              for (int j = 1; j < RB; ++j) d = v >= rt[j].x ? rt[j].y : d;
+              pos[u] = v < total ? v + d : 0u;
+            } else {
+              pos[u] = 0;
+              if (v < total) {
+                while (ro[rp + 1] <= v) ++rp;
+                pos[u] = rs[rp] + (v - ro[rp]);
+              }
             }
           }
           float4 pv[kGroupUnroll];
 #pragma unroll
           for (int u = 0; u < kGroupUnroll; ++u) pv[u] = a.spts[pos[u]];
+          // Hits of this lane as a bit mask, then one group scan of the
+          // counts places them; the buffer keeps CAP - G free of this batch
+          // (the entry condition of group_take).  A batch that does not fit
+          // goes through group_take, one candidate at a time.
+          uint64_t key[kGroupUnroll];
+          uint32_t bits = 0;
 #pragma unroll
           for (int u = 0; u < kGroupUnroll; ++u) {
             const uint32_t v = v0 + (uint32_t)(u * G + l);
-            group_take<G, C::CAP, KNN>(pv[u], v < total, qv, R2, gmask, buf, cnt, thr, k, l,
-                                       &s_thr[gq]);
+            const float d2 = d2_of(qv.x, qv.y, qv.z, pv[u].x, pv[u].y, pv[u].z);
+            const bool ok = (KNN ? (d2 > 0.0f && d2 <= lim) : (d2 <= lim)) && v < total;
+            key[u] = pack_key(d2, __float_as_uint(pv[u].w));
+            bits |= ok ? 1u << u : 0u;
+          }
+          const uint32_t nh = (uint32_t)__popc(bits);
+          const uint32_t hincl = group_scan_fast<G>(nh, l);
+          const uint32_t htot = group_last<G>(hincl, l);
+          if (cnt + htot <= (uint32_t)(C::CAP - G)) {
+            uint32_t o = cnt + hincl - nh;
+#pragma unroll
+            for (int u = 0; u < kGroupUnroll; ++u)
+              if (bits & (1u << u)) buf[o++] = key[u];
+            cnt += htot;
+          } else {
+#pragma unroll
+            for (int u = 0; u < kGroupUnroll; ++u) {
+              const uint32_t v = v0 + (uint32_t)(u * G + l);
+              group_take<G, C::CAP, KNN>(pv[u], v < total, qv, gmask, buf, cnt, lim, k, l,
+                                         &s_thr[gq]);
+            }
           }
         }
         wave_sync();
@@ -598,12 +711,12 @@ __global__ __launch_bounds__(kTileQ) void k_topk_tiled(const SearchArgs a) {
       }
 
       uint32_t cnt = 0;
-      uint64_t thr = ~0ull;
+      float lim = r2_limit(R2);
       if (active && tiled) {
         for (uint32_t v0 = 0; v0 < T; v0 += G) {
           const uint32_t v = v0 + (uint32_t)l;
           const float4 p = v < T ? tile[v] : make_float4(0.f, 0.f, 0.f, 0.f);
-          group_take<G, C::CAP, KNN>(p, v < T, qv, R2, gmask, buf, cnt, thr, k, l, &s_thr[gq]);
+          group_take<G, C::CAP, KNN>(p, v < T, qv, gmask, buf, cnt, lim, k, l, &s_thr[gq]);
         }
       } else if (active && mine) {
         const int nrows = ny * nz;
@@ -631,7 +744,7 @@ __global__ __launch_bounds__(kTileQ) void k_topk_tiled(const SearchArgs a) {
               while (ro[rp + 1] <= v) ++rp;
               p = a.spts[rs[rp] + (v - ro[rp])];
             }
-            group_take<G, C::CAP, KNN>(p, v < total, qv, R2, gmask, buf, cnt, thr, k, l,
+            group_take<G, C::CAP, KNN>(p, v < total, qv, gmask, buf, cnt, lim, k, l,
                                        &s_thr[gq]);
           }
           wave_sync();
