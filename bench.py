@@ -153,6 +153,7 @@ def timed(fn, steps, warmup, label):
     agg["fallback_queries"] = int(stats[-1].fallback_queries)
     agg["lanes_per_query"] = int(stats[-1].lanes_per_query)
     agg["cell_size"] = float(stats[-1].cell_size)
+    agg["sort_passes"] = int(stats[-1].sort_passes)
     log(f"[bench] {label}: {steps} steps, {ms:.3f} ms/step; " +
         ", ".join(f"{k}={v:.3f}" for k, v in agg.items() if k.startswith("ms_")))
     return ms, agg, out
@@ -174,6 +175,8 @@ def main():
     ap.add_argument("--options", default=None,
                     help="JSON of rtnn_options fields to time instead of autotuning, "
                          "e.g. '{\"queries_per_wave\": 16, \"group_buffer\": 1}'")
+    ap.add_argument("--methods", default=None,
+                    help="comma-separated subset of the workload's methods to run, in order")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (scale factor for HBM counters)")
     args = ap.parse_args()
@@ -200,6 +203,13 @@ def main():
         return
 
     w = workload(args.workload)
+    if args.methods:
+        want = args.methods.split(",")
+        bad = [m for m in want if m not in w["methods"]]
+        if bad:
+            raise SystemExit(f"workload {args.workload} has no method {bad}; "
+                             f"it has {w['methods']}")
+        w["methods"] = want
     if args.cpu_sample is None:
         args.cpu_sample = CPU_SAMPLE.get(args.workload, 4000)
     pts, qs = w["pts"], w["qs"]
@@ -387,6 +397,15 @@ def main():
         "methods": results, "sampled_mismatches": mism, "mlp_training": train,
         "host_buffers": host_leg,
     }
+    # grid build of the headline method: sorted keys/s (points sorted into the
+    # cell grid; for the batched methods all clouds' points in one sort)
+    gms = results[kroof]["ms_grid"]
+    if gms > 0:
+        out["grid_build"] = {"keys": len(pts), "ms": round(gms, 4),
+                             "keys_per_s": round(len(pts) / (gms * 1e-3), 1),
+                             "radix_passes": results[kroof]["sort_passes"],
+                             "includes": "bbox, cell keys, radix sort, cell table, gather"
+                                         + (", query ordering" if "offsets" not in w else "")}
     base = w["methods"][0]
     if head != base:
         out["speedup_vs_" + base] = round(results[base]["ms_per_step"] /

@@ -732,6 +732,8 @@ int compute_proposals(rtnn_ctx ctx, Proposal kind, const float* qxyz, uint64_t n
 // rows with fewer than k hits within r' are searched again at r.  If a row
 // has >= k hits with d2 < r'^2 <= r^2, any other point has d2 >= r'^2, so
 // those k are the k nearest within r.
+constexpr int kFallbackLanes = 4;  // lanes per query of the fallback stages
+
 int proposal_search(rtnn_ctx ctx, Proposal kind, const float* qxyz, uint64_t nq, int on_device,
                     float r, uint32_t k, int32_t* out_idx, float* out_d2, int out_on_device,
                     rtnn_stats* stats) {
@@ -774,9 +776,9 @@ int proposal_search(rtnn_ctx ctx, Proposal kind, const float* qxyz, uint64_t nq,
   CK(hipEventRecord(ctx->ev[3], ctx->st));
   // Stage 2: short rows at min(r, r' * s), s from the hits the row found
   // (hit_scale: 1.15 .. 2); stage 3: what is still short, at r.  These rows
-  // are few and their radii large (many candidates each): the thread-per-query
-  // kernel, whose register top-k rejects most candidates with one compare,
-  // beats the group kernels there (k <= 64 here).
+  // are few and their radii large (many candidates each): four lanes per
+  // query with wide buffers (k <= 64 here) measured faster than one lane
+  // (C3: 1.25 -> 1.06 ms) and than two.
   SearchArgs s2 = base_args(ctx, qproc, nq, r, k, o);
   s2.qsel = short1;
   s2.qsel_hits = hits1;
@@ -785,11 +787,11 @@ int proposal_search(rtnn_ctx ctx, Proposal kind, const float* qxyz, uint64_t nq,
   s2.qradius_scale = 2.0f;
   s2.short_rows = short2;
   s2.short_count = &counters[1];
-  CK(launch_search(s2, 1, 1, ctx->st, ctx->num_cus));
+  CK(launch_search(s2, 1, kFallbackLanes, ctx->st, ctx->num_cus));
   SearchArgs s3 = base_args(ctx, qproc, nq, r, k, o);
   s3.qsel = short2;
   s3.nq_dev = &counters[1];
-  CK(launch_search(s3, 1, 1, ctx->st, ctx->num_cus));
+  CK(launch_search(s3, 1, kFallbackLanes, ctx->st, ctx->num_cus));
   CK(hipEventRecord(ctx->ev[4], ctx->st));
   rc = copy_out(ctx, nq, k, out_idx, out_d2, out_on_device, o);
   if (rc) return rc;
