@@ -329,7 +329,16 @@ __device__ inline void rank_chunk(const uint64_t* buf, uint32_t cnt, uint32_t ba
     kv[i] = e < cnt ? buf[e] : 0ull;
     rk[i] = e < cnt ? 0u : ~0u;
   }
-  for (uint32_t j = 0; j < cnt; ++j) {
+  // Four buffer keys per LDS round trip.
+  uint32_t j = 0;
+  for (; j + 4 <= cnt; j += 4) {
+    const uint64_t b0 = buf[j], b1 = buf[j + 1], b2 = buf[j + 2], b3 = buf[j + 3];
+#pragma unroll
+    for (int i = 0; i < CH; ++i)
+      rk[i] += (b0 < kv[i] ? 1u : 0u) + (b1 < kv[i] ? 1u : 0u) + (b2 < kv[i] ? 1u : 0u) +
+               (b3 < kv[i] ? 1u : 0u);
+  }
+  for (; j < cnt; ++j) {
     const uint64_t b = buf[j];
 #pragma unroll
     for (int i = 0; i < CH; ++i) rk[i] += b < kv[i] ? 1u : 0u;
