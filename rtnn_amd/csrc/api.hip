@@ -777,8 +777,9 @@ int proposal_search(rtnn_ctx ctx, Proposal kind, const float* qxyz, uint64_t nq,
   // Stage 2: short rows at min(r, r' * s), s from the hits the row found
   // (hit_scale: 1.15 .. 2); stage 3: what is still short, at r.  These rows
   // are few and their radii large (many candidates each): four lanes per
-  // query with wide buffers (k <= 64 here) measured faster than one lane
-  // (C3: 1.25 -> 1.06 ms) and than two.
+  // query with wide, deep buffers (k <= 64 here; room for 2 max(k, 32) keys)
+  // measured faster than one lane (C3: 1.25 -> 0.84 ms), two lanes, or the
+  // k <= 16 buffer.
   SearchArgs s2 = base_args(ctx, qproc, nq, r, k, o);
   s2.qsel = short1;
   s2.qsel_hits = hits1;
@@ -787,11 +788,11 @@ int proposal_search(rtnn_ctx ctx, Proposal kind, const float* qxyz, uint64_t nq,
   s2.qradius_scale = 2.0f;
   s2.short_rows = short2;
   s2.short_count = &counters[1];
-  CK(launch_search(s2, 1, kFallbackLanes, ctx->st, ctx->num_cus));
+  CK(launch_search(s2, 1, kFallbackLanes, ctx->st, ctx->num_cus, true, false, true));
   SearchArgs s3 = base_args(ctx, qproc, nq, r, k, o);
   s3.qsel = short2;
   s3.nq_dev = &counters[1];
-  CK(launch_search(s3, 1, kFallbackLanes, ctx->st, ctx->num_cus));
+  CK(launch_search(s3, 1, kFallbackLanes, ctx->st, ctx->num_cus, true, false, true));
   CK(hipEventRecord(ctx->ev[4], ctx->st));
   rc = copy_out(ctx, nq, k, out_idx, out_d2, out_on_device, o);
   if (rc) return rc;

@@ -822,7 +822,7 @@ uint32_t max_k_for_group(int group_lanes) {
 }
 
 hipError_t launch_search(const SearchArgs& a0, int knn_mode, int group_lanes, hipStream_t st,
-                         int num_cus, bool wide, bool tiled) {
+                         int num_cus, bool wide, bool tiled, bool deep) {
   SearchArgs a = a0;
   if (a.nq_dev) a.persistent = 1;  // tile count known only on the device
   const uint32_t k = a.k;
@@ -853,7 +853,8 @@ hipError_t launch_search(const SearchArgs& a0, int knn_mode, int group_lanes, hi
   } else if (group_lanes == 2) {
     launch_group<2, 16>(a, knn, wide, tiled, grid, st);
   } else if (group_lanes == 4) {
-    if (k <= 16) launch_group<4, 16>(a, knn, wide, tiled, grid, st);
+    if (k <= 16 && !deep) launch_group<4, 16>(a, knn, wide, tiled, grid, st);
+    else if (k <= 32) launch_group<4, 32>(a, knn, wide, tiled, grid, st);
     else launch_group<4, 64>(a, knn, wide, tiled, grid, st);
   } else if (group_lanes == 8) {
     if (k <= 16) launch_group<8, 16>(a, knn, wide, tiled, grid, st);
