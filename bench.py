@@ -71,7 +71,7 @@ def workload(name):
     if name == "c2":
         return dict(pts=rtnn_amd.gen_gaussian_mixture(2, 20, 1_000_000),
                     qs=rtnn_amd.gen_gaussian_mixture(2, 21, 1_000_000), mode="knn", r=20.0,
-                    k=8, methods=["plain", "partitioned"],
+                    k=8, methods=["plain", "partitioned", "learned"],
                     cfg={"workload": "KNN K=8 r=20: 1M Gaussian-mixture points (64 x sigma 5 "
                                      "+ 20% background), 1M queries",
                          "points": 1_000_000, "queries": 1_000_000, "k": 8, "radius": 20.0})

@@ -733,6 +733,7 @@ int compute_proposals(rtnn_ctx ctx, Proposal kind, const float* qxyz, uint64_t n
 // has >= k hits with d2 < r'^2 <= r^2, any other point has d2 >= r'^2, so
 // those k are the k nearest within r.
 constexpr int kFallbackLanes = 4;  // lanes per query of the fallback stages
+constexpr int kGrowStages = 4;     // radius-growth stages before the last one at r
 
 int proposal_search(rtnn_ctx ctx, Proposal kind, const float* qxyz, uint64_t nq, int on_device,
                     float r, uint32_t k, int32_t* out_idx, float* out_d2, int out_on_device,
@@ -755,44 +756,46 @@ int proposal_search(rtnn_ctx ctx, Proposal kind, const float* qxyz, uint64_t nq,
   OutBufs o;
   rc = output_targets(ctx, nq, k, out_idx, out_d2, out_on_device, o);
   if (rc) return rc;
-  // Two short-row lists (positions into qproc), the hits each row of the
-  // first list found, and the two counters.
-  CK(ctx->shortrows.ensure(4 * (3 * nq + 64)));
-  uint32_t* short1 = ctx->shortrows.as<uint32_t>();
-  uint32_t* short2 = short1 + nq;
-  uint32_t* hits1 = short1 + 2 * nq;
-  uint32_t* counters = short1 + 3 * nq;
-  CK(fill_u32(counters, 2, 0, ctx->st));
+  // Two short-row lists (positions into qproc) used in turn, the hits each
+  // entry found, and one counter per list-producing stage.
+  CK(ctx->shortrows.ensure(4 * (4 * nq + 64)));
+  uint32_t* lists[2] = {ctx->shortrows.as<uint32_t>(), ctx->shortrows.as<uint32_t>() + nq};
+  uint32_t* hits[2] = {lists[0] + 2 * nq, lists[0] + 3 * nq};
+  uint32_t* counters = lists[0] + 4 * nq;
+  CK(fill_u32(counters, kGrowStages + 1, 0, ctx->st));
   const int gl = group_lanes_from_opt(ctx, RTNN_MODE_KNN, k);
   if (gl < 0 || k > max_k_for_group(gl))
     return fail(ctx, RTNN_ERR_UNSUPPORTED, "k=%u not supported by the selected kernel", k);
   // Stage 1: every query at its proposed radius r'.
   SearchArgs a = base_args(ctx, qproc, nq, r, k, o);
   a.qradius = ctx->qradius.as<float>();
-  a.short_rows = short1;
+  a.short_rows = lists[0];
   a.short_count = &counters[0];
-  a.short_hits = hits1;
+  a.short_hits = hits[0];
   CK(launch_search(a, 1, gl, ctx->st, ctx->num_cus, wide_buffers(ctx), ctx->opt.lds_tile != 0));
   CK(hipEventRecord(ctx->ev[3], ctx->st));
-  // Stage 2: short rows at min(r, r' * s), s from the hits the row found
-  // (hit_scale: 1.15 .. 2); stage 3: what is still short, at r.  These rows
+  // Growth stages: a row short at radius R (m < k hits) is searched again at
+  // min(r, R * hit_scale(m, k)), hit_scale in [1.15, 2]; the radius it used is
+  // stored for the next stage.  A row searched at r is final and leaves the
+  // lists, and the last stage takes every remaining row at r.  These rows
   // are few and their radii large (many candidates each): four lanes per
   // query with wide, deep buffers (k <= 64 here; room for 2 max(k, 32) keys)
   // measured faster than one lane (C3: 1.25 -> 0.84 ms), two lanes, or the
   // k <= 16 buffer.
-  SearchArgs s2 = base_args(ctx, qproc, nq, r, k, o);
-  s2.qsel = short1;
-  s2.qsel_hits = hits1;
-  s2.nq_dev = &counters[0];
-  s2.qradius = ctx->qradius.as<float>();
-  s2.qradius_scale = 2.0f;
-  s2.short_rows = short2;
-  s2.short_count = &counters[1];
-  CK(launch_search(s2, 1, kFallbackLanes, ctx->st, ctx->num_cus, true, false, true));
-  SearchArgs s3 = base_args(ctx, qproc, nq, r, k, o);
-  s3.qsel = short2;
-  s3.nq_dev = &counters[1];
-  CK(launch_search(s3, 1, kFallbackLanes, ctx->st, ctx->num_cus, true, false, true));
+  for (int j = 0; j <= kGrowStages; ++j) {
+    SearchArgs s = base_args(ctx, qproc, nq, r, k, o);
+    s.qsel = lists[j & 1];
+    s.nq_dev = &counters[j];
+    if (j < kGrowStages) {
+      s.qsel_hits = hits[j & 1];
+      s.qradius = ctx->qradius.as<float>();
+      s.radius_out = ctx->qradius.as<float>();
+      s.short_rows = lists[(j + 1) & 1];
+      s.short_hits = hits[(j + 1) & 1];
+      s.short_count = &counters[j + 1];
+    }
+    CK(launch_search(s, 1, kFallbackLanes, ctx->st, ctx->num_cus, true, false, true));
+  }
   CK(hipEventRecord(ctx->ev[4], ctx->st));
   rc = copy_out(ctx, nq, k, out_idx, out_d2, out_on_device, o);
   if (rc) return rc;
@@ -808,7 +811,7 @@ int proposal_search(rtnn_ctx ctx, Proposal kind, const float* qxyz, uint64_t nq,
     stats->ms_fallback = ev_ms(ctx->ev[3], ctx->ev[4]);
     stats->ms_total = ev_ms(ctx->ev[0], ctx->ev[5]);
     stats->fallback_queries = nshort;
-    stats->search_launches = 3;
+    stats->search_launches = 2 + kGrowStages;
     stats->lanes_per_query = (uint32_t)gl;
     fill_stats(ctx, stats);
   }

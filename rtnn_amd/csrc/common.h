@@ -63,6 +63,9 @@ struct SearchArgs {
   // there, and its radius grows by hit_scale(qsel_hits[i], k) instead of
   // qradius_scale.
   const uint32_t* qsel_hits;
+  float* radius_out;         // nullable: a short row stores the radius it was
+                             // searched at in radius_out[queries position]
+                             // (the next fallback stage grows it from there)
   int local_index;           // batched: report cloud-local point indices
   int persistent;            // grid-stride over tiles (XCD-chunked)
   // Hierarchical Morton grid (nlevels > 0): grids[l], l = 0 (finest) ..

@@ -88,7 +88,7 @@ struct TileIter {
 // one atomic per wave; a wave's entries stay together and in lane order, so
 // the re-run over the list keeps the Morton locality of the queries.
 __device__ inline void append_short(const SearchArgs& a, bool is_short, uint32_t qs,
-                                    uint32_t hits) {
+                                    uint32_t hits, float R) {
   const uint64_t m = __ballot(is_short);
   if (m == 0) return;
   const int leader = __ffsll((unsigned long long)m) - 1;
@@ -100,6 +100,7 @@ __device__ inline void append_short(const SearchArgs& a, bool is_short, uint32_t
     const uint32_t pos = base + (uint32_t)__popcll(m & lanemask_lt64());
     a.short_rows[pos] = qs;
     if (a.short_hits) a.short_hits[pos] = hits;
+    if (a.radius_out) a.radius_out[qs] = R;
   }
 }
 
@@ -231,7 +232,7 @@ __global__ __launch_bounds__(kTileQ) void k_topk_thread(const SearchArgs a) {
       uint32_t hits = 0;
 #pragma unroll
       for (int i = 0; i < KC; ++i) hits += ((uint32_t)i < k && list[i] != ~0ull) ? 1u : 0u;
-      append_short(a, hits < k, qs, hits);
+      append_short(a, hits < k && R < a.r, qs, hits, R);  // a row searched at r is final
     }
   }
 }
@@ -395,11 +396,13 @@ __device__ inline void group_take(const float4& p, bool valid, const float4& qv,
 }
 
 // Writes a group's row: the buffer's entries by rank (the k smallest), then
-// empty slots; appends the query to the short list if it has fewer than k.
+// empty slots; appends the query to the short list if it has fewer than k
+// and was searched below r (a row searched at r is final: it holds every
+// point within r).
 template <int G>
 __device__ inline void group_finish(const SearchArgs& a, const uint64_t* buf, uint32_t cnt,
                                     uint32_t k, int l, uint32_t row, uint32_t base,
-                                    uint32_t qs) {
+                                    uint32_t qs, float R) {
   wave_sync();
   int32_t* oi = a.out_idx + (uint64_t)row * k;
   float* od = a.out_d2 ? a.out_d2 + (uint64_t)row * k : nullptr;
@@ -418,7 +421,7 @@ __device__ inline void group_finish(const SearchArgs& a, const uint64_t* buf, ui
     oi[s] = -1;
     if (od) od[s] = __builtin_huge_valf();
   }
-  if (a.short_rows) append_short(a, l == 0 && cnt < k, qs, cnt);
+  if (a.short_rows) append_short(a, l == 0 && cnt < k && R < a.r, qs, cnt, R);
 }
 
 template <int G, int KB, bool KNN, bool HIER, bool WIDE>
@@ -602,7 +605,7 @@ __global__ __launch_bounds__(kTileQ) void k_topk_group(const SearchArgs a) {
         wave_sync();
       }
 
-      group_finish<G>(a, buf, cnt, k, l, row, a.local_index ? g.point_base : 0u, qs);
+      group_finish<G>(a, buf, cnt, k, l, row, a.local_index ? g.point_base : 0u, qs, R);
       wave_sync();
     }
   }
@@ -670,13 +673,13 @@ __global__ __launch_bounds__(kTileQ) void k_topk_tiled(const SearchArgs a) {
       const bool active = qi < q1;
       uint32_t qs = 0, row = 0;
       float4 qv = make_float4(0.f, 0.f, 0.f, 0.f);
-      float R2 = 0.0f;
+      float R = a.r, R2 = 0.0f;
       int lo0 = 0, hi0 = -1, lo1 = 0, hi1 = -1, lo2 = 0, hi2 = -1;
       if (active) {
         qs = a.qsel ? a.qsel[qi] : qi;
         qv = a.queries[qs];
         row = __float_as_uint(qv.w);
-        const float R = query_radius(a, qi, qs);
+        R = query_radius(a, qi, qs);
         R2 = R * R;
         cell_range(g, qv.x, R, 0, lo0, hi0);
         cell_range(g, qv.y, R, 1, lo1, hi1);
@@ -760,7 +763,7 @@ __global__ __launch_bounds__(kTileQ) void k_topk_tiled(const SearchArgs a) {
         }
       }
       if (active)
-        group_finish<G>(a, buf, cnt, k, l, row, a.local_index ? g.point_base : 0u, qs);
+        group_finish<G>(a, buf, cnt, k, l, row, a.local_index ? g.point_base : 0u, qs, R);
       wave_sync();  // the tile and buffers are reused by the next round
     }
   }
