@@ -794,7 +794,10 @@ int proposal_search(rtnn_ctx ctx, Proposal kind, const float* qxyz, uint64_t nq,
       s.short_hits = hits[(j + 1) & 1];
       s.short_count = &counters[j + 1];
     }
-    CK(launch_search(s, 1, kFallbackLanes, ctx->st, ctx->num_cus, true, false, true));
+    // On the hierarchical grid the octree walk (one lane per query) prunes
+    // by its shrinking limit, which pays most for these wide searches.
+    CK(launch_search(s, 1, ctx->hlevels > 0 ? 1 : kFallbackLanes, ctx->st, ctx->num_cus, true,
+                     false, true));
   }
   CK(hipEventRecord(ctx->ev[4], ctx->st));
   rc = copy_out(ctx, nq, k, out_idx, out_d2, out_on_device, o);

@@ -43,6 +43,13 @@ __device__ inline uint64_t lanemask_lt64() {
   return lane == 0 ? 0ull : (~0ull >> (kWave - lane));
 }
 
+// d2 < R2 as d2 <= r2_limit(R2): the float just below R2 (-1 for R2 <= 0,
+// FLT_MAX for +inf).  After a select the limit becomes the k-th key's d2:
+// ties with it are appended too and resolved by the (d2, index) rank.
+__device__ inline float r2_limit(float R2) {
+  return R2 > 0.0f ? __uint_as_float(__float_as_uint(R2) - 1u) : -1.0f;
+}
+
 // Resolves tile t into its query range and grid.
 __device__ inline void tile_of(const SearchArgs& a, uint32_t nq, uint32_t t, uint32_t& q0,
                                uint32_t& q1, uint32_t& gi) {
@@ -238,6 +245,148 @@ __global__ __launch_bounds__(kTileQ) void k_topk_thread(const SearchArgs a) {
 }
 
 // --------------------------------------------------------------------------
+// KNN on the hierarchical Morton grid: octree walk, one query per lane
+//
+// A level-l cell m has children 8m .. 8m+7 at level l-1 (the Morton key
+// nests), and every cell's points are one contiguous run of the sorted
+// points, so the levels form an implicit octree.  From each top-level cell
+// within [q - R, q + R] the walk goes depth first, the child holding the
+// query first (child j visits octant j ^ the query's octant), without a
+// stack: the position among siblings is recovered from the key.  A cell
+// whose padded box lies farther than the current limit is skipped; a cell of
+// the finest built level, or holding at most kTreeLeaf points, is scanned.
+// The limit starts at the float below R^2 and, once k keys are held, drops
+// to the k-th key's d2, so a dense cluster beyond the k nearest is cut off
+// at its coarsest cell.  The result is the same as the cube walk's: only
+// cells that cannot hold a hit (d2 <= limit) are skipped.
+constexpr uint32_t kTreeLeaf = 16;
+
+__device__ inline uint32_t compact3(uint32_t x) {
+  x &= 0x09249249u;
+  x = (x | (x >> 2)) & 0x030c30c3u;
+  x = (x | (x >> 4)) & 0x0300f00fu;
+  x = (x | (x >> 8)) & 0xff0000ffu;
+  x = (x | (x >> 16)) & 0x000003ffu;
+  return x;
+}
+
+// Squared distance from q to cell (ix, iy, iz) of edge c, the box grown by
+// pad on every side (a lower bound for any point the FP32 cell mapping puts
+// in the cell).
+__device__ inline float box_d2(const float4& q, const GridDesc& g0, float c, float pad,
+                               uint32_t ix, uint32_t iy, uint32_t iz) {
+  const float lx = g0.mn[0] + (float)ix * c - pad, hx = lx + c + 2.0f * pad;
+  const float ly = g0.mn[1] + (float)iy * c - pad, hy = ly + c + 2.0f * pad;
+  const float lz = g0.mn[2] + (float)iz * c - pad, hz = lz + c + 2.0f * pad;
+  const float dx = fmaxf(fmaxf(lx - q.x, q.x - hx), 0.0f);
+  const float dy = fmaxf(fmaxf(ly - q.y, q.y - hy), 0.0f);
+  const float dz = fmaxf(fmaxf(lz - q.z, q.z - hz), 0.0f);
+  return dx * dx + dy * dy + dz * dz;
+}
+
+template <int KC>
+__global__ __launch_bounds__(kTileQ) void k_topk_tree(const SearchArgs a) {
+  __shared__ uint32_t s_base[8];
+  if (threadIdx.x < 8) s_base[threadIdx.x] = (int)threadIdx.x < a.nlevels ?
+                                               a.grids[threadIdx.x].cell_base : 0u;
+  __syncthreads();
+  const uint32_t nq = launch_nq(a);
+  const int T = a.nlevels - 1, L0 = a.level_min;
+  const GridDesc g0 = a.grids[0];
+  for (TileIter it(a, nq); it.t < it.end; it.t += it.step) {
+    uint32_t q0, q1, gi;
+    tile_of(a, nq, it.t, q0, q1, gi);
+    const uint32_t qi = q0 + threadIdx.x;
+    if (qi >= q1) continue;
+    const uint32_t qs = a.qsel ? a.qsel[qi] : qi;
+    const float4 qv = a.queries[qs];
+    const uint32_t row = __float_as_uint(qv.w);
+    const float R = query_radius(a, qi, qs);
+    float lim = r2_limit(R * R);
+    uint64_t list[KC];
+#pragma unroll
+    for (int i = 0; i < KC; ++i) list[i] = ~0ull;
+    // top-level cells of [q - R, q + R] (the level-0 range shifted by T)
+    int lo0, hi0, lo1, hi1, lo2, hi2;
+    cell_range(g0, qv.x, R, 0, lo0, hi0);
+    cell_range(g0, qv.y, R, 1, lo1, hi1);
+    cell_range(g0, qv.z, R, 2, lo2, hi2);
+    // the query's level-0 cell (clamped): its octant at every level
+    const uint32_t qm = morton3((uint32_t)min(max((int)floorf((qv.x - g0.mn[0]) * g0.inv), 0), g0.dim[0] - 1),
+                                (uint32_t)min(max((int)floorf((qv.y - g0.mn[1]) * g0.inv), 0), g0.dim[1] - 1),
+                                (uint32_t)min(max((int)floorf((qv.z - g0.mn[2]) * g0.inv), 0), g0.dim[2] - 1));
+    const float mag = fabsf(g0.mn[0]) + fabsf(g0.mn[1]) + fabsf(g0.mn[2]) + fabsf(qv.x) +
+                      fabsf(qv.y) + fabsf(qv.z) + (float)g0.dim[0] * g0.cell;
+    if (hi0 >= lo0 && hi1 >= lo1 && hi2 >= lo2) {
+      const int tl0 = lo0 >> T, th0 = hi0 >> T, tl1 = lo1 >> T, th1 = hi1 >> T;
+      const int tl2 = lo2 >> T, th2 = hi2 >> T;
+      for (int tz = tl2; tz <= th2; ++tz)
+        for (int ty = tl1; ty <= th1; ++ty)
+          for (int tx = tl0; tx <= th0; ++tx) {
+            int l = T;
+            uint32_t m = morton3((uint32_t)tx, (uint32_t)ty, (uint32_t)tz);
+            for (;;) {
+              const float c = ldexpf(g0.cell, l);
+              const float pad = c * 0x1.0p-12f + mag * 0x1.0p-19f;
+              const float bd = box_d2(qv, g0, c, pad, compact3(m), compact3(m >> 1),
+                                      compact3(m >> 2));
+              bool down = false;
+              if (bd <= lim * 1.001f) {
+                const uint32_t cb = s_base[l] + m;
+                const uint32_t s0 = a.cell_start[cb], e = a.cell_start[cb + 1];
+                if (e > s0) {
+                  if (l > L0 && e - s0 > kTreeLeaf) {
+                    down = true;
+                  } else {
+                    for (uint32_t j = s0; j < e; ++j) {
+                      const float4 p = a.spts[j];
+                      const float d2 = d2_of(qv.x, qv.y, qv.z, p.x, p.y, p.z);
+                      if (d2 > 0.0f && d2 <= lim) {
+                        const uint64_t key = pack_key(d2, __float_as_uint(p.w));
+                        if (key < list[KC - 1]) {
+                          topk_insert<KC>(list, key);
+                          if (list[KC - 1] != ~0ull) lim = fminf(lim, key_d2(list[KC - 1]));
+                        }
+                      }
+                    }
+                  }
+                }
+              }
+              if (down) {
+                --l;
+                m = (m << 3) | ((qm >> (3 * l)) & 7u);
+                continue;
+              }
+              // next sibling in visiting order, climbing past last children
+              while (l < T && (((m & 7u) ^ ((qm >> (3 * l)) & 7u)) == 7u)) {
+                m >>= 3;
+                ++l;
+              }
+              if (l == T) break;
+              const uint32_t o = (qm >> (3 * l)) & 7u;
+              m = (m & ~7u) | ((((m & 7u) ^ o) + 1u) ^ o);
+            }
+          }
+    }
+    const uint32_t k = a.k;
+    const uint32_t base = a.local_index ? g0.point_base : 0u;
+    int32_t* oi = a.out_idx + (uint64_t)row * k;
+    float* od = a.out_d2 ? a.out_d2 + (uint64_t)row * k : nullptr;
+    uint32_t hits = 0;
+#pragma unroll
+    for (int i = 0; i < KC; ++i) {
+      if ((uint32_t)i < k) {
+        const bool have = list[i] != ~0ull;
+        hits += have ? 1u : 0u;
+        oi[i] = have ? (int32_t)(key_idx(list[i]) - base) : -1;
+        if (od) od[i] = have ? key_d2(list[i]) : __builtin_huge_valf();
+      }
+    }
+    if (a.short_rows) append_short(a, hits < k && R < a.r, qs, hits, R);
+  }
+}
+
+// --------------------------------------------------------------------------
 // G lanes per query
 
 template <int G>
@@ -300,12 +449,6 @@ __device__ inline uint32_t group_last(uint32_t v, int l) {
   }
 }
 
-// d2 < R2 as d2 <= r2_limit(R2): the float just below R2 (-1 for R2 <= 0,
-// FLT_MAX for +inf).  After a select the limit becomes the k-th key's d2:
-// ties with it are appended too and resolved by the (d2, index) rank.
-__device__ inline float r2_limit(float R2) {
-  return R2 > 0.0f ? __uint_as_float(__float_as_uint(R2) - 1u) : -1.0f;
-}
 
 template <int G, int KB, bool WIDE>
 struct GroupCfg {
@@ -776,7 +919,7 @@ template <int KC>
 static void launch_thread(const SearchArgs& a, bool knn, dim3 grid, hipStream_t st) {
   const bool hier = a.nlevels > 0;
   if (knn && hier)
-    hipLaunchKernelGGL((k_topk_thread<KC, true, true>), grid, dim3(kTileQ), 0, st, a);
+    hipLaunchKernelGGL((k_topk_tree<KC>), grid, dim3(kTileQ), 0, st, a);
   else if (knn)
     hipLaunchKernelGGL((k_topk_thread<KC, true, false>), grid, dim3(kTileQ), 0, st, a);
   else if (hier)

@@ -231,6 +231,14 @@ def test_levy_uniform_queries_k32_large_radius(ctx):
     for fn in (ctx.search_learned, ctx.search_partitioned):
         li, ld, _ = fn(qs, radius=100.0, k=32)
         assert np.array_equal(pi, li) and np.array_equal(pd, ld)
+    # the octree walk of the hierarchical grid (one lane per query)
+    ctx.set_options(grid_levels=7, queries_per_wave=0)
+    try:
+        for fn in (ctx.search_learned, ctx.search_partitioned):
+            li, ld, _ = fn(qs, radius=100.0, k=32)
+            assert np.array_equal(pi, li) and np.array_equal(pd, ld)
+    finally:
+        ctx.set_options()
     sel = np.random.default_rng(3).choice(len(qs), 300, replace=False).astype(np.uint64)
     wi, wd = oracle.bruteforce(pts, qs, oracle.MODE_KNN, 100.0, 32, qsel=sel, threads=8)
     s = sel.astype(np.int64)
