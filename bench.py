@@ -80,6 +80,9 @@ def workload(name):
                     qs=rtnn_amd.gen_uniform(51, 30_000_000, 0.0, 1000.0), mode="knn", r=100.0,
                     k=32, methods=["partitioned", "learned"], train_on_queries=True,
                     autotune=False,  # each candidate search at r = 100 takes seconds
+                    # the octree walk of a 7-level hierarchical grid (fixed
+                    # configuration, not tuned)
+                    options={"grid_levels": 7, "queries_per_wave": 0},
                     cfg={"workload": "KNN K=32 r=100: 30M Levy-flight points (Pareto alpha 1.5, "
                                      "min step 0.01, periodic [0,1000]^3), 30M uniform queries",
                          "points": 30_000_000, "queries": 30_000_000, "k": 32, "radius": 100.0})
@@ -250,7 +253,7 @@ def main():
                      "lr": 1e-3, "tau": 0.9}
             log(f"[bench] MLP trained on the GPU in {train['seconds']} s, loss {loss:.5f}")
             fn = lambda: ctx.search_learned(dq, radius=r, k=k, out_device=True)
-        ctx.set_options(**(fixed_opts or {}))
+        ctx.set_options(**(fixed_opts or w.get("options", {})))
         tuned = None
         if args.autotune and m in ("batched", "graph"):
             if batched_opts is None:
