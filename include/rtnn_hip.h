@@ -77,8 +77,8 @@ typedef struct {
   double ms_qsort;        /* query Morton keys + sort + gather                      */
   double ms_search;       /* search kernel(s) only                                  */
   double ms_mlp;          /* learned radius: features + MLP inference               */
-  double ms_fallback;     /* learned radius: re-run of short queries at r          */
-  uint64_t fallback_queries; /* queries re-run at r (learned search)               */
+  double ms_fallback;     /* radius proposals: the fallback stages of short rows    */
+  uint64_t fallback_queries; /* rows short after the first stage (proposal search) */
   uint32_t grid_dims[3];
   float    cell_size;
   uint32_t sort_passes;   /* LSD radix passes of the point sort                     */
@@ -129,9 +129,10 @@ int rtnn_search_async(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int mode, fl
 
 /* KNN with the reference's exact density partition (sort.cpp:140-219,346-391
  * with approx mode 0): per query cell the smallest megacell holding >= k+1
- * points gives a launch radius min(r, sqrt(3)*(iter+1)*cell); queries whose
- * sphere holds fewer than k hits are re-run at r, so the result equals
- * rtnn_search(..., RTNN_MODE_KNN, ...). */
+ * points gives a launch radius min(r, sqrt(3)*(iter+1)*cell); a query whose
+ * sphere holds fewer than k hits is searched again at a grown radius (up to
+ * four growth stages, then r; a row searched at r is final), so the result
+ * equals rtnn_search(..., RTNN_MODE_KNN, ...). */
 int rtnn_search_partitioned(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_device,
                             float r, uint32_t k,
                             int32_t* out_idx, float* out_d2, int out_on_device,
@@ -164,8 +165,8 @@ int rtnn_mlp_get_weights(rtnn_ctx ctx, float* blob /* RTNN_MLP_NBLOB */);
 int rtnn_mlp_set_weights(rtnn_ctx ctx, const float* blob /* RTNN_MLP_NBLOB */);
 
 /* KNN where the MLP proposes r' <= r per query; queries with fewer than k hits
- * within r' are re-run at r (stats->fallback_queries), so the result equals
- * rtnn_search(..., RTNN_MODE_KNN, ...). */
+ * within r' (stats->fallback_queries) are searched again at grown radii and
+ * finally at r, so the result equals rtnn_search(..., RTNN_MODE_KNN, ...). */
 int rtnn_search_learned(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_device,
                         float r, uint32_t k,
                         int32_t* out_idx, float* out_d2, int out_on_device,
