@@ -65,9 +65,10 @@ typedef struct {
   int   group_buffer;     /* group kernel's LDS hit buffer per query: 0 auto / 2
                              wide (2k + G keys), 1 compact (k + 2G keys: less LDS,
                              more waves per CU, more selects on dense rows)     */
-  int   lds_tile;         /* group kernel on a raster grid: 1 = each wave stages the
-                             points of its queries' union block in LDS (512 per
-                             wave) and scans them there; implies compact buffers */
+  int   lds_tile;         /* group kernel on a raster grid: 256, 512 or 1024 = each
+                             wave stages the points of its queries' union block in
+                             LDS (up to that many per wave) and scans them there
+                             (1 = 512; 0 = off); implies compact buffers        */
 } rtnn_options;
 
 /* Filled by search entry points when a non-NULL pointer is passed. */

@@ -194,6 +194,8 @@ int default_group_lanes(int mode, uint32_t k) {
 
 // Group kernel LDS buffer: compact (k + 2G keys per query) only on request.
 bool wide_buffers(rtnn_ctx ctx) { return ctx->opt.group_buffer != 1; }
+// Points per wave of the LDS candidate tile (0: no tile; option 1 = 512).
+int tile_points(rtnn_ctx ctx) { return ctx->opt.lds_tile == 1 ? 512 : ctx->opt.lds_tile; }
 
 int group_lanes_from_opt(rtnn_ctx ctx, int mode, uint32_t k) {
   int qpw = ctx->opt.queries_per_wave;
@@ -522,8 +524,9 @@ int rtnn_set_options(rtnn_ctx ctx, const rtnn_options* opt) {
     return fail(ctx, RTNN_ERR_INVALID, "grid_levels must be in [0, 8]");
   if (opt->group_buffer < 0 || opt->group_buffer > 2)
     return fail(ctx, RTNN_ERR_INVALID, "group_buffer must be 0, 1 or 2");
-  if (opt->lds_tile < 0 || opt->lds_tile > 1)
-    return fail(ctx, RTNN_ERR_INVALID, "lds_tile must be 0 or 1");
+  if (opt->lds_tile != 0 && opt->lds_tile != 1 && opt->lds_tile != 256 && opt->lds_tile != 512 &&
+      opt->lds_tile != 1024)
+    return fail(ctx, RTNN_ERR_INVALID, "lds_tile must be 0, 1 (= 512), 256, 512 or 1024");
   if (!(opt->level_alpha >= 0.0f) || !std::isfinite(opt->level_alpha))
     return fail(ctx, RTNN_ERR_INVALID, "level_alpha must be >= 0");
   ctx->opt = *opt;
@@ -597,7 +600,7 @@ static int search_impl(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_devi
   rc = output_targets(ctx, nq, k, out_idx, out_d2, out_on_device, o);
   if (rc) return rc;
   SearchArgs a = base_args(ctx, qproc, nq, r, k, o);
-  CK(launch_search(a, mode == RTNN_MODE_KNN, gl, ctx->st, ctx->num_cus, wide_buffers(ctx), ctx->opt.lds_tile != 0));
+  CK(launch_search(a, mode == RTNN_MODE_KNN, gl, ctx->st, ctx->num_cus, wide_buffers(ctx), tile_points(ctx)));
   if (timing) CK(hipEventRecord(ctx->ev[3], ctx->st));
   rc = copy_out(ctx, nq, k, out_idx, out_d2, out_on_device, o);
   if (rc) return rc;
@@ -772,7 +775,7 @@ int proposal_search(rtnn_ctx ctx, Proposal kind, const float* qxyz, uint64_t nq,
   a.short_rows = lists[0];
   a.short_count = &counters[0];
   a.short_hits = hits[0];
-  CK(launch_search(a, 1, gl, ctx->st, ctx->num_cus, wide_buffers(ctx), ctx->opt.lds_tile != 0));
+  CK(launch_search(a, 1, gl, ctx->st, ctx->num_cus, wide_buffers(ctx), tile_points(ctx)));
   CK(hipEventRecord(ctx->ev[3], ctx->st));
   // Growth stages: a row short at radius R (m < k hits) is searched again at
   // min(r, R * hit_scale(m, k)), hit_scale in [1.15, 2]; the radius it used is
@@ -1204,7 +1207,7 @@ int rtnn_search_batched(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets,
 
   OutBufs o{out_idx, out_d2};
   SearchArgs a = batched_args(ctx, n, r, k, o);
-  CK(launch_search(a, mode == RTNN_MODE_KNN, gl, ctx->st, ctx->num_cus, wide_buffers(ctx), ctx->opt.lds_tile != 0));
+  CK(launch_search(a, mode == RTNN_MODE_KNN, gl, ctx->st, ctx->num_cus, wide_buffers(ctx), tile_points(ctx)));
   if (timing) CK(hipEventRecord(ctx->ev[2], ctx->st));
   CK(hipStreamSynchronize(ctx->st));
   if (stats) {
@@ -1336,7 +1339,8 @@ int rtnn_autotune(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_device, i
       if (group) {
         cands.push_back({cfs[i], qpws[j], plain ? 0 : 1, 0.0f, 1, 0});
         if (qpws[j] != 32)  // 2-lane groups have no tiled kernel
-          cands.push_back({cfs[i], qpws[j], plain ? 0 : 1, 0.0f, 1, 1});
+          for (int tp : {256, 512, 1024})
+            cands.push_back({cfs[i], qpws[j], plain ? 0 : 1, 0.0f, 1, tp});
       }
     }
   if (!plain)
@@ -1413,19 +1417,19 @@ int rtnn_autotune_batched(rtnn_ctx ctx, const float* xyz, const uint64_t* offset
   const rtnn_options base = ctx->opt;
   double best_t = 1e300;
   rtnn_options best_opt = base;
-  // Cell edge r / {0.5, 1, 2} x lanes per query {64, 16, 4, 1} (queries per
-  // wave 1, 4, 16, 64) x the group kernel's LDS use (wide buffer, compact
-  // buffer, compact buffer + wave tile of candidates).
+  // Cell edge r / {0.5, 1, 2} x queries per wave {1, 4, 8, 16, 32, 64} x the
+  // group kernel's LDS use: wide buffer, compact buffer, or compact buffer +
+  // a wave tile of 256, 512 or 1024 candidate points.
   for (float cf : {0.5f, 1.0f, 2.0f})
     for (int qpw : {1, 4, 8, 16, 32, 64})
-      for (int lds : {0, 1, 2}) {
+      for (int lds : {0, 1, 256, 512, 1024}) {
         const int gl = 64 / qpw;
-        if (k > max_k_for_group(gl) || (gl == 1 && lds > 0) || (gl == 2 && lds == 2)) continue;
+        if (k > max_k_for_group(gl) || (gl == 1 && lds > 0) || (gl == 2 && lds > 1)) continue;
         ctx->opt = base;
         ctx->opt.cell_factor = cf;
         ctx->opt.queries_per_wave = qpw;
         ctx->opt.group_buffer = gl == 1 ? 0 : (lds == 0 ? 2 : 1);
-        ctx->opt.lds_tile = lds == 2 ? 1 : 0;
+        ctx->opt.lds_tile = lds > 1 ? lds : 0;
         rtnn_stats st{};
         for (int rep = 0; rep < 2 && rc == RTNN_OK; ++rep)
           rc = rtnn_search_batched(ctx, xyz, offsets, nclouds, mode, r, k, out_idx, out_d2, &st);
@@ -1548,7 +1552,7 @@ hipError_t enqueue_batched(rtnn_graph g, const float* xyz, float cell, int mode,
   a.local_index = 1;
   a.persistent = 1;
   return launch_search(a, mode == RTNN_MODE_KNN, gl, st, g->ctx->num_cus, wide_buffers(g->ctx),
-                       g->ctx->opt.lds_tile != 0);
+                       tile_points(g->ctx));
 }
 
 }  // namespace

@@ -83,14 +83,15 @@ hipError_t partition_hier(const float4* qproc, uint32_t nq, const GridDesc* leve
 // (register-resident sorted top-k, k <= 64); 4, 16 or 64 select the group
 // kernel (ballot compaction into an LDS buffer, rank-sorted output).
 // wide = false selects the compact LDS buffer of the group kernel (k + 2G keys
-// per query instead of 2k + G; raster grids only); tiled = true the group
-// kernel that stages each wave's candidates in LDS (k_topk_tiled: raster
-// grids, compact buffers).  deep = true sizes the 4-lane kernel's buffer for
+// per query instead of 2k + G; raster grids only); tiled > 0 the group kernel
+// that stages each wave's candidates in LDS, tiled points per wave (256, 512
+// or 1024; k_topk_tiled: raster grids, compact buffers, G >= 4).  deep = true
+// sizes the 4-lane kernel's buffer for
 // k <= 16 as for k = 32 (fewer selects where rows have many hits: the
 // fallback stages).  Returns hipErrorInvalidValue when no kernel variant
 // covers (k, group_lanes).
 hipError_t launch_search(const SearchArgs& a, int knn_mode, int group_lanes, hipStream_t st,
-                         int num_cus, bool wide = true, bool tiled = false, bool deep = false);
+                         int num_cus, bool wide = true, int tiled = 0, bool deep = false);
 // Largest k supported for a group width (0 = unsupported width).
 uint32_t max_k_for_group(int group_lanes);
 // Boundary-pair report over a raster grid (a.r = r): count pass when offs is

@@ -766,9 +766,9 @@ __global__ __launch_bounds__(kTileQ) void k_topk_group(const SearchArgs a) {
 // is tested too and fails d2 < R^2: every point within R of the query lies in
 // its own block, each point appears once in the tile, so no hit is missed or
 // counted twice.  A wave whose union block spans more than 64 rows or holds
-// more than kTilePts points walks its rows from global memory instead (the
-// k_topk_group path).  Raster grids, compact group buffers.
-constexpr int kTilePts = 512;
+// more than TP points (the tile: 256, 512 or 1024 per wave) walks its rows
+// from global memory instead (the k_topk_group path).  Raster grids, compact
+// group buffers.
 
 __device__ inline int wave_min_i(int v) {
 #pragma unroll
@@ -781,7 +781,7 @@ __device__ inline int wave_max_i(int v) {
   return v;
 }
 
-template <int G, int KB, bool KNN>
+template <int G, int KB, bool KNN, int TP>
 __global__ __launch_bounds__(kTileQ) void k_topk_tiled(const SearchArgs a) {
   using C = GroupCfg<G, KB, false>;
   constexpr int kWaves = kTileQ / kWave;
@@ -789,7 +789,7 @@ __global__ __launch_bounds__(kTileQ) void k_topk_tiled(const SearchArgs a) {
   __shared__ uint32_t s_rs[C::NG][G];
   __shared__ uint32_t s_ro[C::NG][G + 1];
   __shared__ uint64_t s_thr[C::NG];
-  __shared__ float4 s_tile[kWaves][kTilePts];
+  __shared__ float4 s_tile[kWaves][TP];
   __shared__ uint32_t s_trs[kWaves][kWave];
   __shared__ uint32_t s_tro[kWaves][kWave + 1];
 
@@ -852,7 +852,7 @@ __global__ __launch_bounds__(kTileQ) void k_topk_tiled(const SearchArgs a) {
         }
         const uint32_t incl = group_incl_scan<kWave>(len, lane);
         T = __shfl(incl, kWave - 1, kWave);
-        tiled = T <= (uint32_t)kTilePts;
+        tiled = T <= (uint32_t)TP;
         if (tiled) {
           trs[lane] = s0;
           tro[lane + 1] = incl;
@@ -930,16 +930,23 @@ static void launch_thread(const SearchArgs& a, bool knn, dim3 grid, hipStream_t 
     hipLaunchKernelGGL((k_topk_thread<KC, false, false>), grid, dim3(kTileQ), 0, st, a);
 }
 
+template <int G, int KB, int TP>
+static void launch_tiled(const SearchArgs& a, bool knn, dim3 grid, hipStream_t st) {
+  if (knn)
+    hipLaunchKernelGGL((k_topk_tiled<G, KB, true, TP>), grid, dim3(kTileQ), 0, st, a);
+  else
+    hipLaunchKernelGGL((k_topk_tiled<G, KB, false, TP>), grid, dim3(kTileQ), 0, st, a);
+}
+
 template <int G, int KB>
-static void launch_group(const SearchArgs& a, bool knn, bool wide, bool tiled, dim3 grid,
+static void launch_group(const SearchArgs& a, bool knn, bool wide, int tile, dim3 grid,
                          hipStream_t st) {
   const bool hier = a.nlevels > 0;
   if constexpr (G >= 4) {  // 2-lane groups: no room for a tile beside their buffers
-    if (tiled && !hier) {
-      if (knn)
-        hipLaunchKernelGGL((k_topk_tiled<G, KB, true>), grid, dim3(kTileQ), 0, st, a);
-      else
-        hipLaunchKernelGGL((k_topk_tiled<G, KB, false>), grid, dim3(kTileQ), 0, st, a);
+    if (tile > 0 && !hier) {
+      if (tile <= 256) launch_tiled<G, KB, 256>(a, knn, grid, st);
+      else if (tile <= 512) launch_tiled<G, KB, 512>(a, knn, grid, st);
+      else launch_tiled<G, KB, 1024>(a, knn, grid, st);
       return;
     }
   }
@@ -970,7 +977,7 @@ uint32_t max_k_for_group(int group_lanes) {
 }
 
 hipError_t launch_search(const SearchArgs& a0, int knn_mode, int group_lanes, hipStream_t st,
-                         int num_cus, bool wide, bool tiled, bool deep) {
+                         int num_cus, bool wide, int tiled, bool deep) {
   SearchArgs a = a0;
   if (a.nq_dev) a.persistent = 1;  // tile count known only on the device
   const uint32_t k = a.k;

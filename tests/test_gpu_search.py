@@ -81,7 +81,7 @@ def test_kernel_variants_and_cell_factors(ctx, qpw, cf):
     qs = np.concatenate([pts[:3000], rng.uniform(-3, 23, (2000, 3)).astype(np.float32)])
     ctx.set_points(pts)
     # group kernels: wide and compact LDS buffers, and the wave-tiled kernel
-    for buf, tile in (((0, 0), (1, 0), (1, 1)) if qpw in (1, 4, 8, 16) else
+    for buf, tile in (((0, 0), (1, 0), (1, 1), (1, 256), (1, 1024)) if qpw in (1, 4, 8, 16) else
                       ((0, 0), (1, 0)) if qpw == 32 else ((0, 0),)):
         ctx.set_options(cell_factor=cf, queries_per_wave=qpw, group_buffer=buf, lds_tile=tile)
         for mode in ("radius", "knn"):
@@ -216,6 +216,8 @@ def test_errors(ctx, samplepc):
         ctx.set_points(bad)
     with pytest.raises(rtnn_amd.RtnnError):
         ctx.set_options(queries_per_wave=3)
+    with pytest.raises(rtnn_amd.RtnnError, match="lds_tile"):
+        ctx.set_options(lds_tile=300)
 
 
 def test_deterministic(ctx):
@@ -226,7 +228,8 @@ def test_deterministic(ctx):
     assert np.array_equal(a, b) and np.array_equal(ad.view(np.int32), bd.view(np.int32))
 
 
-@pytest.mark.parametrize("qpw,tile", [(0, 0), (1, 0), (16, 0), (16, 1), (4, 1)])
+@pytest.mark.parametrize("qpw,tile", [(0, 0), (1, 0), (16, 0), (16, 1), (4, 1), (16, 256),
+                                      (4, 1024)])
 def test_batched_clouds(ctx, qpw, tile):
     import torch
     xyz, offs = rtnn_amd.gen_lattice_clouds(9, 40, 1000, 8000)
