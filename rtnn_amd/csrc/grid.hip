@@ -257,9 +257,16 @@ __global__ __launch_bounds__(256) void k_cell_keys(const float* __restrict__ xyz
                                                    uint32_t* __restrict__ keys,
                                                    uint32_t* __restrict__ vals,
                                                    uint32_t* __restrict__ counts) {
-  uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  // The block's first cloud by one binary search; a thread steps forward from
+  // it (clouds are contiguous, so a block of 256 points spans few of them).
+  __shared__ uint32_t s_c0;
+  const uint64_t i0 = (uint64_t)blockIdx.x * 256;
+  const uint64_t i = i0 + threadIdx.x;
+  if (threadIdx.x == 0) s_c0 = nclouds > 1 ? find_cloud(off, nclouds, i0) : 0u;
+  __syncthreads();
   if (i >= n) return;
-  uint32_t c = nclouds > 1 ? find_cloud(off, nclouds, i) : 0;
+  uint32_t c = s_c0;
+  while (c + 1 < nclouds && off[c + 1] <= i) ++c;
   const GridDesc g = grids[c];
   int cx = cell_of(xyz[3 * i], g.mn[0], g.inv, g.dim[0]);
   int cy = cell_of(xyz[3 * i + 1], g.mn[1], g.inv, g.dim[1]);

@@ -151,10 +151,24 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_hist(const uint32_t* __restri
   h[threadIdx.x] = 0;
   __syncthreads();
   uint64_t base = (uint64_t)blockIdx.x * kRsTile;
+  const int lane = threadIdx.x & (kWave - 1);
 #pragma unroll 4
   for (int i = 0; i < kRsItems; ++i) {
     uint64_t e = base + (uint64_t)i * kRsThreads + threadIdx.x;
-    if (e < n) atomicAdd(&h[(keys[e] >> shift) & 255u], 1u);
+    const bool valid = e < n;
+    const uint32_t d = valid ? (keys[e] >> shift) & 255u : 0u;
+    // A wave whose keys share one digit (common for nearly sorted input:
+    // clouds laid out cell by cell) adds its count with one atomic instead
+    // of 64 to the same LDS word.
+    const uint32_t d0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)d);
+    const uint64_t act = __ballot(valid);
+    const uint64_t same = __ballot(valid && d == d0);
+    if (same == act) {
+      if (act != 0 && lane == __ffsll((unsigned long long)act) - 1)
+        atomicAdd(&h[d0], (uint32_t)__popcll(act));
+    } else if (valid) {
+      atomicAdd(&h[d], 1u);
+    }
   }
   __syncthreads();
   hist[(uint64_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
