@@ -300,6 +300,20 @@ def main():
                     "queries_per_s": round(nq / (hms * 1e-3), 1),
                     "includes": "query upload and idx + d2 download over PCIe"}
         log(f"[bench] {head_m} with host buffers: {hms:.3f} ms/step")
+        if qh is not None:
+            # the same through the streamed entry point: chunks of 2M queries,
+            # one chunk's rows crossing PCIe while the next is searched
+            ch = 1 << 21
+            sargs = dict(mode=mode, method=head_m, radius=r, k=k, chunk=ch)
+            ctx.search_stream(qh, **sargs)
+            t0 = time.perf_counter()
+            for _ in range(2):
+                ctx.search_stream(qh, **sargs)
+            sms = (time.perf_counter() - t0) * 1e3 / 2
+            host_leg["streamed"] = {"ms_per_step": round(sms, 3),
+                                    "queries_per_s": round(nq / (sms * 1e-3), 1),
+                                    "chunk_queries": ch}
+            log(f"[bench] {head_m} streamed from/to host buffers: {sms:.3f} ms/step")
         if qh is None:
             ctx.set_points(dp)
 

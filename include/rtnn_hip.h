@@ -42,6 +42,7 @@ typedef struct rtnn_ctx_s* rtnn_ctx;
 #define RTNN_ERR_OOM         -3   /* device allocation failed */
 #define RTNN_ERR_STATE       -4   /* call order (e.g. search before set_points) */
 #define RTNN_ERR_UNSUPPORTED -5   /* size/feature outside what the build supports */
+#define RTNN_ERR_IO          -6   /* a file could not be opened, read or written */
 
 #define RTNN_MODE_RADIUS 0
 #define RTNN_MODE_KNN    1
@@ -172,6 +173,22 @@ int rtnn_search_learned(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_dev
                         float r, uint32_t k,
                         int32_t* out_idx, float* out_d2, int out_on_device,
                         rtnn_stats* stats);
+
+/* Streamed search for results larger than one wants to hold at once (C4's
+ * 39 GB of rows, C5's 7.7 GB): host queries qxyz[nq][3] are searched in
+ * chunks of `chunk` queries (0: 4M) through two slots of device and pinned
+ * buffers, so one chunk's rows travel over PCIe while the next chunk is
+ * searched.  Rows go to host arrays out_idx[nq][k] (+ out_d2, nullable) or,
+ * when out_path is set, to a result file in the CLI's format: "RTNNRES1",
+ * u64 nq, u32 k, u32 flags (bit 0: d2 present), int32 idx[nq][k], float
+ * d2[nq][k], little endian, empty slot -1 / +inf.  method: RTNN_METHOD_PLAIN
+ * (mode range or KNN), _PARTITIONED or _LEARNED (KNN).  The rows equal one
+ * search over all queries.  Replaces the reference's whole-result pinned
+ * host buffers (search.cpp:43-56) and its TODO on the Q*K limit
+ * (main.cpp:126). */
+int rtnn_search_stream(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int mode, int method,
+                       float r, uint32_t k, uint64_t chunk, int32_t* out_idx, float* out_d2,
+                       const char* out_path, rtnn_stats* stats);
 
 /* The first-stage launch radius of each query (input order) under method
  * RTNN_METHOD_PARTITIONED (the megacell rule, sort.cpp:366-371 with approx

@@ -12,6 +12,7 @@ Rows are original query indices, values original point indices in (d2, index)
 order, empty slots -1 (the reference's UINT_MAX sentinel, search.cpp:14-18).
 """
 import ctypes
+import struct
 import os
 import weakref
 
@@ -102,6 +103,8 @@ SIGNATURES = {
     "rtnn_search_learned": (_I, [_VP, _VP, _U64, _I, _F, _U32, _VP, _VP, _I,
                                  ctypes.POINTER(Stats)]),
     "rtnn_propose_radius": (_I, [_VP, _VP, _U64, _I, _I, _F, _U32, _VP, _I]),
+    "rtnn_search_stream": (_I, [_VP, _VP, _U64, _I, _I, _F, _U32, _U64, _VP, _VP,
+                                ctypes.c_char_p, ctypes.POINTER(Stats)]),
     "rtnn_search_batched": (_I, [_VP, _VP, _VP, _U32, _I, _F, _U32, _VP, _VP,
                                  ctypes.POINTER(Stats)]),
     "rtnn_search_percloud": (_I, [_VP, _VP, _VP, _U32, _I, _F, _U32, _VP, _VP,
@@ -308,6 +311,27 @@ class Context:
 
     def search_learned(self, queries=None, radius=10.0, k=16, with_d2=True, out_device=None):
         return self._run(self._L.rtnn_search_learned, queries, radius, k, with_d2, out_device)
+
+    def search_stream(self, queries, mode="knn", method="plain", radius=2.0, k=50, chunk=0,
+                      with_d2=True, path=None):
+        """Host queries in, host rows out (or a result file at `path`), searched
+        chunk by chunk with the copies of one chunk overlapping the search of
+        the next -> (idx, d2 or None, Stats); (None, None, Stats) with a path."""
+        q = np.ascontiguousarray(np.asarray(queries, dtype=np.float32).reshape(-1, 3))
+        meth = {"plain": 0, "partitioned": 1, "learned": 2}[method]
+        nq = len(q)
+        stats = Stats()
+        idx = d2 = None
+        if path is None:
+            idx = np.empty((nq, k), dtype=np.int32)
+            d2 = np.empty((nq, k), dtype=np.float32) if with_d2 else None
+        rc = self._L.rtnn_search_stream(
+            self._h, q.ctypes.data_as(ctypes.c_void_p), nq, _mode(mode), meth, radius, k, chunk,
+            None if idx is None else idx.ctypes.data_as(ctypes.c_void_p),
+            None if d2 is None else d2.ctypes.data_as(ctypes.c_void_p),
+            None if path is None else os.fsencode(path), ctypes.byref(stats))
+        self._check(rc)
+        return idx, d2, stats
 
     def propose_radius(self, queries=None, method="learned", radius=10.0, k=16):
         """First-stage launch radius per query (input order) -> float32 array."""
@@ -536,6 +560,18 @@ def read_points(path):
     if rc != 0:
         raise ValueError(f"cannot read {path} (code {rc})")
     return out
+
+
+def read_results(path):
+    """Reads a result file ("RTNNRES1": u64 nq, u32 k, u32 flags, int32 idx[nq][k],
+    then float d2[nq][k] when flags bit 0) -> (idx, d2 or None)."""
+    with open(path, "rb") as f:
+        if f.read(8) != b"RTNNRES1":
+            raise ValueError(f"{path}: not a result file")
+        nq, k, flags = struct.unpack("<QII", f.read(16))
+        idx = np.fromfile(f, dtype="<i4", count=nq * k).reshape(nq, k)
+        d2 = np.fromfile(f, dtype="<f4", count=nq * k).reshape(nq, k) if flags & 1 else None
+    return idx, d2
 
 
 def read_point_sets(path):

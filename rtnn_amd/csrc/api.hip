@@ -60,6 +60,33 @@ struct DevBuf {
 
 }  // namespace
 
+// Pinned host staging memory (grown on demand, freed with the context).
+struct PinBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  hipError_t ensure(size_t bytes) {
+    if (bytes <= cap && p) return hipSuccess;
+    if (p) {
+      (void)hipDeviceSynchronize();  // copies may still read or write it
+      (void)hipHostFree(p);
+      p = nullptr;
+      cap = 0;
+    }
+    hipError_t e = hipHostMalloc(&p, std::max<size_t>(bytes, 256), hipHostMallocDefault);
+    if (e != hipSuccess) {
+      p = nullptr;
+      return e;
+    }
+    cap = std::max<size_t>(bytes, 256);
+    return hipSuccess;
+  }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
 struct rtnn_ctx_s {
   int device = 0;
   hipStream_t st = nullptr;
@@ -104,12 +131,31 @@ struct rtnn_ctx_s {
   DevBuf bcnt, boff, bpairs;
   uint32_t batch_tiles = 0;  // tiles of the last batched grid
 
+  // streamed search (rtnn_search_stream): two slots of device and pinned
+  // buffers, a copy stream and its events
+  DevBuf sq[2], sidx[2], sd2[2];
+  PinBuf pq[2], pidx[2], pd2[2];
+  hipStream_t st3 = nullptr;
+  hipEvent_t ev_up[2] = {}, ev_done[2] = {}, ev_down[2] = {};
+
   ~rtnn_ctx_s() {
     DevBuf* all[] = {&pts, &keys, &vals, &counts, &cell_start, &spts, &tmp, &grids,
                      &qin, &qkeys, &qvals, &qsorted, &oidx, &od2, &shortrows, &misc,
                      &tiles, &offs, &bboxes, &pgrid, &pcounts, &qradius, &tr_q, &tr_idx,
                      &tr_d2, &tr_t, &tr_f, &tr_out, &bcnt, &boff, &bpairs};
     for (DevBuf* b : all) b->release();
+    if (st3) (void)hipStreamSynchronize(st3);
+    for (int i = 0; i < 2; ++i) {
+      sq[i].release();
+      sidx[i].release();
+      sd2[i].release();
+      pq[i].release();
+      pidx[i].release();
+      pd2[i].release();
+      for (hipEvent_t e : {ev_up[i], ev_done[i], ev_down[i]})
+        if (e) (void)hipEventDestroy(e);
+    }
+    if (st3) (void)hipStreamDestroy(st3);
     mlp_release(mlp);
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e);
@@ -869,6 +915,134 @@ int rtnn_propose_radius(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_dev
   if (!out_on_device)
     CK(hipMemcpyAsync(out_radius, dst, 4 * nq, hipMemcpyDeviceToHost, ctx->st));
   CK(hipStreamSynchronize(ctx->st));
+  return RTNN_OK;
+}
+
+// ----------------------------------------------------- streamed search --
+
+// Host queries in, host results (or a result file) out, in chunks of
+// `chunk` queries through two slots: while chunk i is searched, chunk i-1's
+// rows travel to pinned memory on a copy stream and chunk i-2's are copied
+// from there to the caller's arrays or written to the file.  Each chunk is
+// an ordinary search of its queries (rows = positions in the chunk), so the
+// concatenated rows equal one search over all queries.
+int rtnn_search_stream(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int mode, int method,
+                       float r, uint32_t k, uint64_t chunk, int32_t* out_idx, float* out_d2,
+                       const char* out_path, rtnn_stats* stats) {
+  if (!ctx) return RTNN_ERR_INVALID;
+  if (!qxyz && nq > 0) return fail(ctx, RTNN_ERR_INVALID, "qxyz is NULL");
+  if (!out_path && !out_idx) return fail(ctx, RTNN_ERR_INVALID, "no output (out_idx or out_path)");
+  if (method != RTNN_METHOD_PLAIN && method != RTNN_METHOD_PARTITIONED &&
+      method != RTNN_METHOD_LEARNED)
+    return fail(ctx, RTNN_ERR_INVALID, "unknown method %d", method);
+  if (method != RTNN_METHOD_PLAIN && mode != RTNN_MODE_KNN)
+    return fail(ctx, RTNN_ERR_INVALID, "the partitioned and learned methods are KNN only");
+  int rc = check_search_args(ctx, mode, r, k, out_path ? reinterpret_cast<const void*>(out_path)
+                                                      : out_idx);
+  if (rc) return rc;
+  if (ctx->n == 0) return fail(ctx, RTNN_ERR_STATE, "rtnn_set_points has not been called");
+  if (chunk == 0) chunk = 1ull << 22;
+  chunk = std::min<uint64_t>(chunk, std::max<uint64_t>(nq, 1));
+  CK(hipSetDevice(ctx->device));
+  if (!ctx->st3) {
+    CK(hipStreamCreateWithFlags(&ctx->st3, hipStreamNonBlocking));
+    for (int i = 0; i < 2; ++i) {
+      CK(hipEventCreateWithFlags(&ctx->ev_up[i], hipEventDisableTiming));
+      CK(hipEventCreateWithFlags(&ctx->ev_done[i], hipEventDisableTiming));
+      CK(hipEventCreateWithFlags(&ctx->ev_down[i], hipEventDisableTiming));
+    }
+  }
+  FILE* f = nullptr;
+  const uint64_t head = 8 + 8 + 4 + 4;
+  if (out_path) {
+    f = std::fopen(out_path, "wb");
+    if (!f) return fail(ctx, RTNN_ERR_IO, "cannot open %s for writing", out_path);
+    const uint32_t flags = 1;  // d2 present
+    std::fwrite("RTNNRES1", 1, 8, f);
+    std::fwrite(&nq, 8, 1, f);
+    std::fwrite(&k, 4, 1, f);
+    std::fwrite(&flags, 4, 1, f);
+  }
+  const bool want_d2 = out_path || out_d2;
+  const uint64_t rowb = 4ull * k;
+  for (int i = 0; i < 2; ++i) {
+    rc = RTNN_OK;
+    if (ctx->sq[i].ensure(12 * chunk) != hipSuccess || ctx->sidx[i].ensure(rowb * chunk) ||
+        (want_d2 && ctx->sd2[i].ensure(rowb * chunk)) || ctx->pq[i].ensure(12 * chunk) ||
+        ctx->pidx[i].ensure(rowb * chunk) || (want_d2 && ctx->pd2[i].ensure(rowb * chunk))) {
+      if (f) std::fclose(f);
+      return fail(ctx, RTNN_ERR_OOM, "streaming buffers of %llu queries",
+                  (unsigned long long)chunk);
+    }
+  }
+  const uint64_t nchunks = (nq + chunk - 1) / chunk;
+  rtnn_stats sum{};
+  bool io_err = false;
+  // Rows of chunk c (slot c & 1) from pinned memory to the caller / file.
+  auto flush = [&](uint64_t c) -> hipError_t {
+    const int s = (int)(c & 1);
+    hipError_t e = hipEventSynchronize(ctx->ev_down[s]);
+    if (e != hipSuccess) return e;
+    const uint64_t c0 = c * chunk, n = std::min(chunk, nq - c0);
+    if (f) {
+      io_err |= std::fseek(f, (long)(head + c0 * rowb), SEEK_SET) != 0 ||
+                std::fwrite(ctx->pidx[s].p, 1, n * rowb, f) != n * rowb;
+      io_err |= std::fseek(f, (long)(head + nq * rowb + c0 * rowb), SEEK_SET) != 0 ||
+                std::fwrite(ctx->pd2[s].p, 1, n * rowb, f) != n * rowb;
+    } else {
+      std::memcpy(out_idx + c0 * k, ctx->pidx[s].p, n * rowb);
+      if (out_d2) std::memcpy(out_d2 + c0 * k, ctx->pd2[s].p, n * rowb);
+    }
+    return hipSuccess;
+  };
+  for (uint64_t c = 0; c < nchunks && rc == RTNN_OK; ++c) {
+    const int s = (int)(c & 1);
+    const uint64_t c0 = c * chunk, n = std::min(chunk, nq - c0);
+    if (c >= 2) CK(flush(c - 2));  // frees slot s's pinned rows
+    // queries: caller -> pinned (once its last upload is done) -> device
+    CK(hipEventSynchronize(ctx->ev_up[s]));
+    std::memcpy(ctx->pq[s].p, qxyz + 3 * c0, 12 * n);
+    CK(hipStreamWaitEvent(ctx->st, ctx->ev_down[s], 0));  // slot s's device rows are free
+    CK(hipMemcpyAsync(ctx->sq[s].p, ctx->pq[s].p, 12 * n, hipMemcpyHostToDevice, ctx->st));
+    CK(hipEventRecord(ctx->ev_up[s], ctx->st));
+    rtnn_stats st{};
+    int32_t* di = ctx->sidx[s].as<int32_t>();
+    float* dd = want_d2 ? ctx->sd2[s].as<float>() : nullptr;
+    if (method == RTNN_METHOD_PLAIN)
+      rc = search_impl(ctx, ctx->sq[s].as<float>(), n, 1, mode, r, k, di, dd, 1,
+                       stats ? &st : nullptr);
+    else
+      rc = proposal_search(ctx, method == RTNN_METHOD_LEARNED ? Proposal::kLearned
+                                                              : Proposal::kPartition,
+                           ctx->sq[s].as<float>(), n, 1, r, k, di, dd, 1,
+                           stats ? &st : nullptr);
+    if (rc) break;
+    CK(hipEventRecord(ctx->ev_done[s], ctx->st));
+    CK(hipStreamWaitEvent(ctx->st3, ctx->ev_done[s], 0));
+    CK(hipMemcpyAsync(ctx->pidx[s].p, di, n * rowb, hipMemcpyDeviceToHost, ctx->st3));
+    if (want_d2)
+      CK(hipMemcpyAsync(ctx->pd2[s].p, dd, n * rowb, hipMemcpyDeviceToHost, ctx->st3));
+    CK(hipEventRecord(ctx->ev_down[s], ctx->st3));
+    sum.ms_total += st.ms_total;
+    sum.ms_grid += st.ms_grid;
+    sum.ms_qsort += st.ms_qsort;
+    sum.ms_search += st.ms_search;
+    sum.ms_mlp += st.ms_mlp;
+    sum.ms_fallback += st.ms_fallback;
+    sum.fallback_queries += st.fallback_queries;
+    sum.search_launches += st.search_launches;
+    sum.lanes_per_query = st.lanes_per_query;
+  }
+  for (uint64_t c = nchunks >= 2 ? nchunks - 2 : 0; c < nchunks && rc == RTNN_OK; ++c)
+    CK(flush(c));
+  CK(hipStreamSynchronize(ctx->st3));
+  if (f && std::fclose(f) != 0) io_err = true;
+  if (rc) return rc;
+  if (io_err) return fail(ctx, RTNN_ERR_IO, "writing %s failed", out_path);
+  if (stats) {
+    *stats = sum;
+    fill_stats(ctx, stats);
+  }
   return RTNN_OK;
 }
 

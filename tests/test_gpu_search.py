@@ -393,3 +393,35 @@ def test_search_async_on_a_torch_stream(ctx):
     assert torch.equal(idx, want_i) and torch.equal(d2, want_d)
     wi, wd = oracle.grid_search(pts, pts, oracle.MODE_RADIUS, 1.0, 8, threads=8)
     assert np.array_equal(ri.cpu().numpy(), wi) and np.array_equal(rd.cpu().numpy(), wd)
+
+
+def test_search_stream_chunks_equal_one_search(ctx, tmp_path):
+    # Host queries searched in chunks (the last one short) with the copies of
+    # one chunk overlapping the next chunk's search: rows equal one search,
+    # into host arrays and into a result file.
+    rng = np.random.default_rng(31)
+    pts = rng.uniform(0, 30, (200000, 3)).astype(np.float32)
+    qs = rng.uniform(-1, 31, (150001, 3)).astype(np.float32)
+    ctx.set_points(pts)
+    for mode, r, k in (("knn", 1.2, 16), ("radius", 1.0, 24)):
+        wi, wd, _ = ctx.search(qs, mode=mode, radius=r, k=k)
+        si, sd, st = ctx.search_stream(qs, mode=mode, radius=r, k=k, chunk=40000)
+        assert np.array_equal(si, wi) and np.array_equal(sd, wd)
+        si, sd, _ = ctx.search_stream(qs, mode=mode, radius=r, k=k, chunk=40000, with_d2=False)
+        assert np.array_equal(si, wi) and sd is None
+        path = tmp_path / f"rows_{mode}.bin"
+        _, _, _ = ctx.search_stream(qs, mode=mode, radius=r, k=k, chunk=65536, path=str(path))
+        fi, fd = rtnn_amd.read_results(str(path))
+        assert np.array_equal(fi, wi) and np.array_equal(fd, wd)
+    # proposal methods, one chunk and several
+    wi, wd, _ = ctx.search(qs, mode="knn", radius=1.2, k=8)
+    from oracle import mlp_ref as M
+    w = np.random.default_rng(5).normal(0, 0.25, M.NP).astype(np.float32)
+    w[M.OB3] = 0.3
+    ctx.set_mlp_weights(np.concatenate([w, [0.25, 1.0]]).astype(np.float32))
+    for method in ("partitioned", "learned"):
+        for chunk in (0, 33333):
+            si, sd, _ = ctx.search_stream(qs, method=method, radius=1.2, k=8, chunk=chunk)
+            assert np.array_equal(si, wi) and np.array_equal(sd, wd)
+    with pytest.raises(rtnn_amd.RtnnError):
+        ctx.search_stream(qs, mode="radius", method="learned", radius=1.2, k=8)
