@@ -174,6 +174,15 @@ int rtnn_search_learned(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_dev
                         int32_t* out_idx, float* out_d2, int out_on_device,
                         rtnn_stats* stats);
 
+/* Device and pinned host bytes the library holds now and at its peak, over
+ * all contexts of the process (any pointer may be NULL).  Replaces the
+ * reference's memstatlib cudaMalloc/cudaFree interposer
+ * (src/memstatlib/memstatlib.cpp:10-32): with RTNN_MEMSTAT set in the
+ * environment every allocation and free is logged to stderr with the running
+ * total, as there. */
+int rtnn_mem_stats(uint64_t* device_bytes, uint64_t* device_peak, uint64_t* pinned_bytes,
+                   uint64_t* pinned_peak);
+
 /* Streamed search for results larger than one wants to hold at once (C4's
  * 39 GB of rows, C5's 7.7 GB): host queries qxyz[nq][3] are searched in
  * chunks of `chunk` queries (0: 4M) through two slots of device and pinned

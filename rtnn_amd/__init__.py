@@ -105,6 +105,7 @@ SIGNATURES = {
     "rtnn_propose_radius": (_I, [_VP, _VP, _U64, _I, _I, _F, _U32, _VP, _I]),
     "rtnn_search_stream": (_I, [_VP, _VP, _U64, _I, _I, _F, _U32, _U64, _VP, _VP,
                                 ctypes.c_char_p, ctypes.POINTER(Stats)]),
+    "rtnn_mem_stats": (_I, [ctypes.POINTER(_U64)] * 4),
     "rtnn_search_batched": (_I, [_VP, _VP, _VP, _U32, _I, _F, _U32, _VP, _VP,
                                  ctypes.POINTER(Stats)]),
     "rtnn_search_percloud": (_I, [_VP, _VP, _VP, _U32, _I, _F, _U32, _VP, _VP,
@@ -560,6 +561,15 @@ def read_points(path):
     if rc != 0:
         raise ValueError(f"cannot read {path} (code {rc})")
     return out
+
+
+def mem_stats():
+    """Device and pinned bytes the library holds now and at its peak (all
+    contexts) -> dict."""
+    v = [ctypes.c_uint64(0) for _ in range(4)]
+    lib().rtnn_mem_stats(*[ctypes.byref(x) for x in v])
+    return {"device_bytes": v[0].value, "device_peak": v[1].value,
+            "pinned_bytes": v[2].value, "pinned_peak": v[3].value}
 
 
 def read_results(path):

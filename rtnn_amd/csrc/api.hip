@@ -9,9 +9,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -22,11 +24,70 @@
 #include "internal.h"
 #include "mlp.h"
 
+// Allocation statistics of the library (all contexts of the process): the
+// reference's memstatlib (src/memstatlib/memstatlib.cpp:10-32) interposes
+// cudaMalloc/cudaFree to log each allocation and the running total; here
+// every device and pinned allocation of the library goes through these
+// functions, which keep the current and peak bytes and, with RTNN_MEMSTAT set
+// in the environment, log the same lines.
+namespace rtnn {
+namespace {
+std::atomic<uint64_t> g_dev_cur{0}, g_dev_peak{0}, g_pin_cur{0}, g_pin_peak{0};
+bool memstat_log() {
+  static const bool on = std::getenv("RTNN_MEMSTAT") != nullptr;
+  return on;
+}
+void bump_peak(std::atomic<uint64_t>& peak, uint64_t v) {
+  uint64_t p = peak.load();
+  while (v > p && !peak.compare_exchange_weak(p, v)) {
+  }
+}
+}  // namespace
+
+hipError_t dev_alloc(void** p, size_t bytes) {
+  const hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    if (memstat_log())
+      std::fprintf(stderr, "[MEM_STATS] hipMalloc of %.3f MB failed (%s), in use %.3f MB\n",
+                   bytes / 1048576.0, hipGetErrorString(e), g_dev_cur.load() / 1048576.0);
+    return e;
+  }
+  const uint64_t cur = g_dev_cur += bytes;
+  bump_peak(g_dev_peak, cur);
+  if (memstat_log())
+    std::fprintf(stderr, "[MEM_STATS] hipMalloc (%p): %.3f MB\n[MEM_STATS] %.3f MB\n", *p,
+                 bytes / 1048576.0, cur / 1048576.0);
+  return e;
+}
+void dev_free(void* p, size_t bytes) {
+  (void)hipFree(p);
+  const uint64_t cur = g_dev_cur -= bytes;
+  if (memstat_log())
+    std::fprintf(stderr, "[MEM_STATS] hipFree (%p): %.3f MB\n[MEM_STATS] %.3f MB\n", p,
+                 bytes / 1048576.0, cur / 1048576.0);
+}
+hipError_t pinned_alloc(void** p, size_t bytes) {
+  const hipError_t e = hipHostMalloc(p, bytes, hipHostMallocDefault);
+  if (e != hipSuccess) return e;
+  bump_peak(g_pin_peak, g_pin_cur += bytes);
+  return e;
+}
+void pinned_free(void* p, size_t bytes) {
+  (void)hipHostFree(p);
+  g_pin_cur -= bytes;
+}
+uint64_t mem_dev_cur() { return g_dev_cur.load(); }
+uint64_t mem_dev_peak() { return g_dev_peak.load(); }
+uint64_t mem_pin_cur() { return g_pin_cur.load(); }
+uint64_t mem_pin_peak() { return g_pin_peak.load(); }
+}  // namespace rtnn
+
 using namespace rtnn;
 
 namespace {
 
 thread_local std::string g_create_err;
+
 
 struct DevBuf {
   void* p = nullptr;
@@ -36,12 +97,12 @@ struct DevBuf {
     if (p) {
       // Queued work may still read the old buffer.
       (void)hipDeviceSynchronize();
-      (void)hipFree(p);
+      rtnn::dev_free(p, cap);
       p = nullptr;
       cap = 0;
     }
     size_t want = std::max<size_t>(bytes, 256);
-    hipError_t e = hipMalloc(&p, want);
+    hipError_t e = rtnn::dev_alloc(&p, want);
     if (e != hipSuccess) {
       p = nullptr;
       return e;
@@ -50,7 +111,7 @@ struct DevBuf {
     return hipSuccess;
   }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) rtnn::dev_free(p, cap);
     p = nullptr;
     cap = 0;
   }
@@ -68,11 +129,11 @@ struct PinBuf {
     if (bytes <= cap && p) return hipSuccess;
     if (p) {
       (void)hipDeviceSynchronize();  // copies may still read or write it
-      (void)hipHostFree(p);
+      rtnn::pinned_free(p, cap);
       p = nullptr;
       cap = 0;
     }
-    hipError_t e = hipHostMalloc(&p, std::max<size_t>(bytes, 256), hipHostMallocDefault);
+    hipError_t e = rtnn::pinned_alloc(&p, std::max<size_t>(bytes, 256));
     if (e != hipSuccess) {
       p = nullptr;
       return e;
@@ -81,7 +142,7 @@ struct PinBuf {
     return hipSuccess;
   }
   void release() {
-    if (p) (void)hipHostFree(p);
+    if (p) rtnn::pinned_free(p, cap);
     p = nullptr;
     cap = 0;
   }
@@ -498,6 +559,15 @@ void fill_stats(rtnn_ctx ctx, rtnn_stats* s) {
 extern "C" {
 
 const char* rtnn_version(void) { return "rtnn-mi355x 0.1 (gfx950)"; }
+
+int rtnn_mem_stats(uint64_t* device_bytes, uint64_t* device_peak, uint64_t* pinned_bytes,
+                   uint64_t* pinned_peak) {
+  if (device_bytes) *device_bytes = rtnn::mem_dev_cur();
+  if (device_peak) *device_peak = rtnn::mem_dev_peak();
+  if (pinned_bytes) *pinned_bytes = rtnn::mem_pin_cur();
+  if (pinned_peak) *pinned_peak = rtnn::mem_pin_peak();
+  return RTNN_OK;
+}
 
 const char* rtnn_create_error(void) { return g_create_err.c_str(); }
 

@@ -419,6 +419,12 @@ int main(int argc, char** argv) {
   }
   int code = 0;
   if (a.check && !sanity_check(a, P, Q, idx, k)) code = 1;
+  if (std::getenv("RTNN_MEMSTAT")) {
+    uint64_t dev = 0, peak = 0, pin = 0, pin_peak = 0;
+    rtnn_mem_stats(&dev, &peak, &pin, &pin_peak);
+    std::printf("[MEM_STATS] peak device %.3f MB, peak pinned host %.3f MB\n",
+                peak / 1048576.0, pin_peak / 1048576.0);
+  }
   rtnn_destroy(ctx);
   return code;
 }

@@ -92,6 +92,16 @@ hipError_t partition_hier(const float4* qproc, uint32_t nq, const GridDesc* leve
 // covers (k, group_lanes).
 hipError_t launch_search(const SearchArgs& a, int knn_mode, int group_lanes, hipStream_t st,
                          int num_cus, bool wide = true, int tiled = 0, bool deep = false);
+// Every device / pinned allocation of the library (tracked for
+// rtnn_mem_stats; logged with RTNN_MEMSTAT set).
+hipError_t dev_alloc(void** p, size_t bytes);
+void dev_free(void* p, size_t bytes);
+hipError_t pinned_alloc(void** p, size_t bytes);
+void pinned_free(void* p, size_t bytes);
+uint64_t mem_dev_cur();
+uint64_t mem_dev_peak();
+uint64_t mem_pin_cur();
+uint64_t mem_pin_peak();
 // Largest k supported for a group width (0 = unsupported width).
 uint32_t max_k_for_group(int group_lanes);
 // Boundary-pair report over a raster grid (a.r = r): count pass when offs is

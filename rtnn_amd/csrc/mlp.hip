@@ -513,17 +513,18 @@ __global__ __launch_bounds__(256) void k_sample_points(const float* __restrict__
 hipError_t mlp_alloc(MlpState& m, uint32_t max_grad_blocks) {
   hipError_t e;
   if (!m.params) {
-    if ((e = hipMalloc(&m.params, 4 * kMlpNP)) != hipSuccess) return e;
-    if ((e = hipMalloc(&m.adam_m, 4 * kMlpNP)) != hipSuccess) return e;
-    if ((e = hipMalloc(&m.adam_v, 4 * kMlpNP)) != hipSuccess) return e;
+    if ((e = dev_alloc((void**)&m.params, 4 * kMlpNP)) != hipSuccess) return e;
+    if ((e = dev_alloc((void**)&m.adam_m, 4 * kMlpNP)) != hipSuccess) return e;
+    if ((e = dev_alloc((void**)&m.adam_v, 4 * kMlpNP)) != hipSuccess) return e;
   }
   if (m.partial_blocks < max_grad_blocks) {
     if (m.partial) {
       (void)hipDeviceSynchronize();
-      (void)hipFree(m.partial);
+      dev_free(m.partial, 4ull * (kMlpNP + 1) * m.partial_blocks);
       m.partial = nullptr;
     }
-    if ((e = hipMalloc(&m.partial, 4ull * (kMlpNP + 1) * max_grad_blocks)) != hipSuccess) {
+    if ((e = dev_alloc((void**)&m.partial, 4ull * (kMlpNP + 1) * max_grad_blocks)) !=
+        hipSuccess) {
       m.partial_blocks = 0;
       return e;
     }
@@ -533,9 +534,10 @@ hipError_t mlp_alloc(MlpState& m, uint32_t max_grad_blocks) {
 }
 
 void mlp_release(MlpState& m) {
-  float* ptrs[] = {m.params, m.adam_m, m.adam_v, m.partial};
+  float* ptrs[] = {m.params, m.adam_m, m.adam_v};
   for (float* p : ptrs)
-    if (p) (void)hipFree(p);
+    if (p) dev_free(p, 4 * kMlpNP);
+  if (m.partial) dev_free(m.partial, 4ull * (kMlpNP + 1) * m.partial_blocks);
   m = MlpState{};
 }
 

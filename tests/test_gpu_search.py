@@ -425,3 +425,25 @@ def test_search_stream_chunks_equal_one_search(ctx, tmp_path):
             assert np.array_equal(si, wi) and np.array_equal(sd, wd)
     with pytest.raises(rtnn_amd.RtnnError):
         ctx.search_stream(qs, mode="radius", method="learned", radius=1.2, k=8)
+
+
+def test_mem_stats_track_the_library_allocations():
+    # Device bytes rise with a context's buffers and fall back when it closes;
+    # the peak covers the largest buffer set (memstatlib's running total).
+    import gc
+    base = rtnn_amd.mem_stats()
+    rng = np.random.default_rng(41)
+    pts = rng.uniform(0, 20, (100000, 3)).astype(np.float32)
+    with rtnn_amd.Context(0) as c:
+        c.set_points(pts)
+        c.search(pts[:20000], mode="knn", radius=1.0, k=16)
+        during = rtnn_amd.mem_stats()
+        assert during["device_bytes"] >= base["device_bytes"] + 12 * len(pts)
+        assert during["device_peak"] >= during["device_bytes"]
+        c.search_stream(pts[:5000], mode="knn", radius=1.0, k=16, chunk=2000)
+        assert rtnn_amd.mem_stats()["pinned_bytes"] > base["pinned_bytes"]
+    gc.collect()
+    after = rtnn_amd.mem_stats()
+    assert after["device_bytes"] == base["device_bytes"]
+    assert after["pinned_bytes"] == base["pinned_bytes"]
+    assert after["device_peak"] >= during["device_bytes"]
