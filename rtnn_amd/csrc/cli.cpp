@@ -15,7 +15,12 @@
 // scheduling flags, -cr sets the cell factor (cell edge = r / crRatio, as
 // sort.cpp:80) and -ac 1 runs the autotuner (measured instead of modelled,
 // util.cpp:551-676), -fq 1 reports the queries beyond the points' box + r
-// (their rows stay, empty); the others are accepted and do not change results.
+// (their rows stay, empty), -ab 0 -nb n gives the partitioned KNN n radius
+// classes (grid levels); the others are accepted and do not change results:
+// -a 1/2 (approximate partition radii) run exact, -gmu/-egs feed the
+// reference's GPU-memory model of crRatio, which the measured autotuner
+// replaces, and -mc scales its meta-grid order, which full Morton order
+// replaces.
 //
 // Output file (little endian): "RTNNRES1", u64 nq, u32 k, u32 flags (bit 0:
 // d2 present), then int32 idx[nq][k] (empty slot = -1), then float d2[nq][k].
@@ -53,6 +58,8 @@ struct Args {
   int qpw = 0;
   std::string boundary;  // --boundary: boundary-pair report file
   bool filter_queries = false;  // -fq 1: report queries beyond the points' box + r
+  bool autobatch = true;  // -ab: with -ab 0, -nb n fixes the partition's radius classes
+  int numbatch = -1;      // -nb: radius classes (hierarchical grid levels) of -p 1 KNN
 };
 
 struct Timer {
@@ -132,9 +139,11 @@ Args parse(int argc, char** argv) {
       }
     } else if (s == "--filterQueries" || s == "-fq") {
       a.filter_queries = std::atoi(need(i)) != 0;
+    } else if (s == "--autobatch" || s == "-ab") {
+      a.autobatch = std::atoi(need(i)) != 0;
+    } else if (s == "--numbatch" || s == "-nb") {
+      a.numbatch = std::atoi(need(i));
     } else if (s == "--msr" || s == "-m" || s == "--deferFree" || s == "-df" ||
-               s == "--numbatch" || s == "-nb" ||
-               s == "--autobatch" || s == "-ab" ||
                s == "--interleave" || s == "-i" || s == "--pointsort" || s == "-ps" ||
                s == "--querysort" || s == "-qs" ||
                s == "--gpumemused" || s == "-gmu" || s == "--gather" || s == "-g" ||
@@ -306,6 +315,13 @@ int main(int argc, char** argv) {
   rtnn_options opt{};
   opt.cell_factor = a.cell_factor > 0 ? a.cell_factor : a.cr_ratio;
   opt.queries_per_wave = a.qpw;
+  // -ab 0 -nb n: the reference launches the partition in n batches of radius
+  // classes (sort.cpp:326-345); here one launch walks n grid levels, one per
+  // radius class (2..8).
+  if (!a.autobatch && a.numbatch > 0) {
+    opt.grid_levels = std::min(std::max(a.numbatch, 2), 8);
+    std::printf("grid levels: %d (radius classes of the partition, -nb)\n", opt.grid_levels);
+  }
   check_rc(ctx, rtnn_set_options(ctx, &opt), "rtnn_set_options");
   {
     Timer t;

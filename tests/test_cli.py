@@ -61,7 +61,8 @@ def test_cli_matches_golden(golden, tmp_path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("extra", [[], ["-p", "0"], ["--qpw", "16"]])
+@pytest.mark.parametrize("extra", [[], ["-p", "0"], ["--qpw", "16"], ["-ab", "0", "-nb", "3"],
+                                   ["-a", "0", "-gmu", "100", "-mc", "2", "-egs", "10"]])
 def test_cli_knn_separate_queries(tmp_path, extra):
     import oracle
     rng = np.random.default_rng(3)
@@ -72,6 +73,8 @@ def test_cli_knn_separate_queries(tmp_path, extra):
     np.savetxt(qf, q, fmt="%.9g", delimiter=",")
     r = run("-f", str(pf), "-q", str(qf), "-sm", "knn", "-r", "3", "-c", "1", "-o", out, *extra)
     assert r.returncode == 0, r.stderr
+    if "-nb" in extra:  # the partition's radius classes became grid levels
+        assert "grid levels: 3" in r.stdout
     idx, d2 = read_result(out)
     k = idx.shape[1]
     wi, wd = oracle.bruteforce(p, q, oracle.MODE_KNN, 3.0, k, threads=4)
