@@ -89,7 +89,10 @@ def workload(name):
     if name == "c4":
         xyz, offs = rtnn_amd.gen_lattice_clouds(4, 4096, 1000, 200000, h=0.1, jit=0.02)
         return dict(pts=xyz, qs=None, offsets=offs, mode="radius", r=0.2, k=64,
-                    methods=["percloud", "batched", "graph"],
+                    methods=["percloud", "batched", "classes", "graph"],
+                    # cloud-size classes of the per-class tuning: <= 6000,
+                    # <= 35000 and larger points (thirds of the log range)
+                    class_bounds=[6000, 35000],
                     cfg={"workload": "batched range r=0.2 K=64: 4096 jittered cubic lattices "
                                      "(h=0.1, jitter 0.02h), sizes log-uniform in [1e3, 2e5], "
                                      "queries = points",
@@ -240,6 +243,9 @@ def main():
         elif m == "batched":
             fn = lambda: ctx.search_batched(dp, w["offsets"], mode=mode, radius=r, k=k,
                                             with_d2=False)
+        elif m == "classes":  # batched, one tuned configuration per cloud-size class
+            fn = lambda: ctx.search_batched(dp, w["offsets"], mode=mode, radius=r, k=k,
+                                            with_d2=False)
         elif m == "graph":  # the batched pipeline captured once (below), replayed per step
             fn = lambda: (graph.idx, graph.d2, graph.launch())
         else:
@@ -255,6 +261,21 @@ def main():
             fn = lambda: ctx.search_learned(dq, radius=r, k=k, out_device=True)
         ctx.set_options(**(fixed_opts or w.get("options", {})))
         tuned = None
+        if m == "classes":
+            bounds = w["class_bounds"]
+            if args.autotune:
+                t0 = time.perf_counter()
+                copts, cms = ctx.autotune_batched_classes(dp, w["offsets"], bounds, mode=mode,
+                                                          radius=r, k=k)
+                keep = ("cell_factor", "queries_per_wave", "group_buffer", "lds_tile")
+                tuned = {"class_bounds": bounds,
+                         "classes": [{kk: o[kk] for kk in keep} for o in copts],
+                         "class_search_ms": [round(v, 4) for v in cms],
+                         "seconds": round(time.perf_counter() - t0, 3)}
+                log(f"[bench] classes: autotuned {tuned}")
+            else:
+                o = fixed_opts or w.get("options", {})
+                ctx.set_size_classes(bounds, [o] * (len(bounds) + 1))
         if args.autotune and m in ("batched", "graph"):
             if batched_opts is None:
                 t0 = time.perf_counter()
@@ -264,6 +285,13 @@ def main():
             ctx.set_options(**batched_opts[0])
             tuned = dict(batched_opts[0], search_ms=batched_opts[1], seconds=batched_opts[2])
         if m == "graph":  # capture with the options in force (tuned or default)
+            # with the size classes when they timed faster than one configuration
+            use_cls = ("classes" in results and "batched" in results and
+                       results["classes"]["ms_per_step"] < results["batched"]["ms_per_step"])
+            if not use_cls:
+                ctx.set_size_classes([], [])
+            graph_cfg = "size classes" if use_cls else "one configuration"
+            log(f"[bench] graph: captured with {graph_cfg}")
             graph = ctx.graph_batched(dp, w["offsets"], mode=mode, radius=r, k=k, with_d2=False)
         if args.autotune and w.get("autotune", True) and m in ("plain", "partitioned", "learned"):
             t0 = time.perf_counter()
@@ -278,6 +306,8 @@ def main():
                              for kk, v in agg.items()})
         if tuned:
             results[m]["autotuned"] = tuned
+        if m == "graph":
+            results[m]["captured_with"] = graph_cfg
         outputs[m] = (out[0], out[1])
 
     # The same head method with host-resident queries and outputs (the CLI's
@@ -400,6 +430,8 @@ def main():
                                  "fallback",
                   "batched": "per-cloud bbox + one grid build for all clouds + one persistent "
                              "search launch",
+                  "classes": "per-cloud bbox + one grid build for all clouds + one "
+                             "persistent search launch per cloud-size class",
                   "graph": "one HIP graph replay: per-cloud bbox + device-side grid plan + "
                            "cell keys + radix sort + cell table + gather + one persistent "
                            "search launch",

@@ -218,6 +218,21 @@ int rtnn_propose_radius(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_dev
 int rtnn_search_batched(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets,
                         uint32_t nclouds, int mode, float r, uint32_t k,
                         int32_t* out_idx, float* out_d2, rtnn_stats* stats);
+/* Size classes of the batched search (and of graphs captured afterwards):
+ * class j holds the clouds of at most upper[j] points (the last class, j =
+ * nclasses - 1, every larger cloud; upper has nclasses - 1 increasing
+ * entries) and is searched with opts[j]'s cell factor, queries per wave,
+ * group buffer and LDS tile: one persistent launch per non-empty class over
+ * that class's clouds, all in one grid build.  nclasses = 0 drops the classes
+ * (the context's options apply to every cloud again).  No counterpart in the
+ * reference, which runs one configuration per launch (src/optixNSearch/
+ * main.cpp:91-148). */
+int rtnn_set_size_classes(rtnn_ctx ctx, uint32_t nclasses, const uint64_t* upper,
+                          const rtnn_options* opts);
+/* Current classes: *nclasses (0 if none); upper (nclasses - 1 entries) and
+ * opts (nclasses entries) are filled when not NULL. */
+int rtnn_get_size_classes(rtnn_ctx ctx, uint32_t* nclasses, uint64_t* upper,
+                          rtnn_options* opts);
 /* Same result, one full pipeline per cloud (the per-cloud-launch baseline). */
 int rtnn_search_percloud(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets,
                          uint32_t nclouds, int mode, float r, uint32_t k,
@@ -288,14 +303,25 @@ int rtnn_autotune(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_device, i
                   int mode, float r, uint32_t k, int32_t* out_idx, float* out_d2,
                   rtnn_options* best, double* best_ms);
 /* The same for rtnn_search_batched (and so for graphs captured afterwards):
- * cell edge r / {0.5, 1, 2} x queries per wave {1, 4, 16, 64} x the group
- * kernel's LDS use (wide buffer / compact buffer / compact buffer + a tile of
- * 512 candidates per wave), timed on this batch.  out_idx
- * (required) and out_d2 (nullable) are device scratch of n x k. */
+ * cell edge r / {0.5, 1, 2} x queries per wave {1, 4, 8, 16, 32, 64} x the
+ * group kernel's LDS use (wide buffer / compact buffer / compact buffer + a
+ * tile of 256, 512 or 1024 candidates per wave), timed on this batch; one
+ * configuration for every cloud (drops size classes).  out_idx (required) and
+ * out_d2 (nullable) are device scratch of n x k. */
 int rtnn_autotune_batched(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets,
                           uint32_t nclouds, int mode, float r, uint32_t k,
                           int32_t* out_idx, float* out_d2,
                           rtnn_options* best, double* best_ms);
+/* Per size class: splits the batch by cloud size at upper[0..nclasses-2]
+ * (as rtnn_set_size_classes), tunes each non-empty class on the sub-batch of
+ * its clouds as rtnn_autotune_batched does, and installs the classes with
+ * their winners (empty classes keep the context's options).  best (nclasses
+ * entries) and best_ms (each class's time on its sub-batch) are nullable. */
+int rtnn_autotune_batched_classes(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets,
+                                  uint32_t nclouds, int mode, float r, uint32_t k,
+                                  int32_t* out_idx, float* out_d2, uint32_t nclasses,
+                                  const uint64_t* upper, rtnn_options* best,
+                                  double* best_ms);
 
 /* ---- building blocks (exported for parity tests; device pointers) ---- */
 

@@ -54,6 +54,16 @@ class Options(ctypes.Structure):
                 ("lds_tile", ctypes.c_int)]
 
 
+def _options_struct(cell_factor=0.0, queries_per_wave=0, query_order=0, time_stages=0,
+                    max_cells=0, grid_levels=0, level_alpha=0.0, group_buffer=0, lds_tile=0):
+    return Options(cell_factor, queries_per_wave, query_order, time_stages, max_cells,
+                   grid_levels, level_alpha, group_buffer, lds_tile)
+
+
+def _options_dict(o):
+    return {name: getattr(o, name) for name, _ in Options._fields_}
+
+
 class Stats(ctypes.Structure):
     _fields_ = [("ms_total", ctypes.c_double),
                 ("ms_grid", ctypes.c_double),
@@ -122,6 +132,10 @@ SIGNATURES = {
                            ctypes.POINTER(Options), ctypes.POINTER(ctypes.c_double)]),
     "rtnn_autotune_batched": (_I, [_VP, _VP, _VP, _U32, _I, _F, _U32, _VP, _VP,
                                    ctypes.POINTER(Options), ctypes.POINTER(ctypes.c_double)]),
+    "rtnn_autotune_batched_classes": (_I, [_VP, _VP, _VP, _U32, _I, _F, _U32, _VP, _VP, _U32,
+                                           _VP, _VP, _VP]),
+    "rtnn_set_size_classes": (_I, [_VP, _U32, _VP, _VP]),
+    "rtnn_get_size_classes": (_I, [_VP, ctypes.POINTER(_U32), _VP, _VP]),
     "rtnn_radix_sort_pairs": (_I, [_VP, _VP, _VP, _U64, _U32]),
     "rtnn_exclusive_scan": (_I, [_VP, _VP, _VP, _U64]),
     "rtnn_morton3": (_I, [_VP, _VP, _VP, _U64]),
@@ -240,6 +254,31 @@ class Context:
 
     def synchronize(self):
         self._check(self._L.rtnn_synchronize(self._h))
+
+    def set_size_classes(self, upper, options):
+        """Batched-search size classes: class j holds clouds of at most upper[j]
+        points (the last class every larger one) and uses options[j] (a dict of
+        set_options keywords).  Empty lists drop the classes."""
+        n = len(options)
+        if n and len(upper) != n - 1:
+            raise ValueError("need len(options) - 1 class bounds")
+        up = np.ascontiguousarray(upper, dtype=np.uint64)
+        arr = (Options * max(n, 1))()
+        for j, o in enumerate(options):
+            arr[j] = _options_struct(**o)
+        self._check(self._L.rtnn_set_size_classes(
+            self._h, n, ctypes.c_void_p(up.ctypes.data) if len(up) else None, arr))
+
+    def size_classes(self):
+        """(upper bounds, list of option dicts) of the current size classes."""
+        n = _U32(0)
+        self._check(self._L.rtnn_get_size_classes(self._h, ctypes.byref(n), None, None))
+        up = np.zeros(max(n.value - 1, 1), dtype=np.uint64)
+        arr = (Options * max(n.value, 1))()
+        self._check(self._L.rtnn_get_size_classes(self._h, ctypes.byref(n),
+                                                  ctypes.c_void_p(up.ctypes.data), arr))
+        return ([int(u) for u in up[:max(n.value - 1, 0)]],
+                [_options_dict(arr[j]) for j in range(n.value)])
 
     def set_points(self, points):
         p = _as_points(points)
@@ -497,6 +536,27 @@ class Context:
         del idx
         return ({"cell_factor": best.cell_factor, "queries_per_wave": best.queries_per_wave,
                  "group_buffer": best.group_buffer, "lds_tile": best.lds_tile}, ms.value)
+
+    def autotune_batched_classes(self, xyz, offsets, upper, mode="radius", radius=0.2, k=64):
+        """Tunes every size class (bounds `upper`, as set_size_classes) on the
+        sub-batch of its clouds and installs the classes; returns (list of
+        option dicts, list of ms per class sub-batch)."""
+        import torch
+        if not (_is_torch(xyz) and xyz.is_cuda):
+            raise ValueError("batched autotune takes a CUDA tensor of points")
+        xyz = xyz.to(torch.float32).contiguous()
+        offs = np.ascontiguousarray(offsets, dtype=np.uint64)
+        up = np.ascontiguousarray(upper, dtype=np.uint64)
+        n = len(up) + 1
+        idx = torch.empty((int(offs[-1]), k), dtype=torch.int32, device=xyz.device)
+        best = (Options * n)()
+        ms = (ctypes.c_double * n)()
+        self._check(self._L.rtnn_autotune_batched_classes(
+            self._h, ctypes.c_void_p(xyz.data_ptr()), ctypes.c_void_p(offs.ctypes.data),
+            len(offs) - 1, _mode(mode), radius, k, ctypes.c_void_p(idx.data_ptr()), None, n,
+            ctypes.c_void_p(up.ctypes.data) if len(up) else None, best, ms))
+        del idx
+        return ([_options_dict(best[j]) for j in range(n)], [ms[j] for j in range(n)])
 
     # MLP building blocks (device tensors)
     def mlp_features(self, queries, radius):

@@ -192,6 +192,14 @@ struct rtnn_ctx_s {
   DevBuf bcnt, boff, bpairs;
   uint32_t batch_tiles = 0;  // tiles of the last batched grid
 
+  // batched size classes (rtnn_set_size_classes): class j holds the clouds of
+  // at most class_upper[j] points (the last class: any size) and is searched
+  // with class_opt[j]; class_tiles[j] .. [j+1] are its tiles in the last
+  // batched grid (tiles are class-major)
+  std::vector<uint64_t> class_upper;
+  std::vector<rtnn_options> class_opt;
+  std::vector<uint32_t> class_tiles;
+
   // streamed search (rtnn_search_stream): two slots of device and pinned
   // buffers, a copy stream and its events
   DevBuf sq[2], sidx[2], sd2[2];
@@ -627,8 +635,59 @@ int rtnn_destroy(rtnn_ctx ctx) {
 
 const char* rtnn_last_error(rtnn_ctx ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
+}  // extern "C"
+
+namespace {
+int check_options(rtnn_ctx ctx, const rtnn_options* opt);
+}  // namespace
+
+extern "C" {
+
 int rtnn_set_options(rtnn_ctx ctx, const rtnn_options* opt) {
   if (!ctx || !opt) return RTNN_ERR_INVALID;
+  const int rc = check_options(ctx, opt);
+  if (rc) return rc;
+  ctx->opt = *opt;
+  return RTNN_OK;
+}
+
+int rtnn_set_size_classes(rtnn_ctx ctx, uint32_t nclasses, const uint64_t* upper,
+                          const rtnn_options* opts) {
+  if (!ctx) return RTNN_ERR_INVALID;
+  if (nclasses == 0) {
+    ctx->class_upper.clear();
+    ctx->class_opt.clear();
+    return RTNN_OK;
+  }
+  if (nclasses > 16 || !opts || (nclasses > 1 && !upper))
+    return fail(ctx, RTNN_ERR_INVALID, "1..16 classes with %s", "bounds and options");
+  for (uint32_t j = 0; j + 1 < nclasses; ++j)
+    if (j > 0 && upper[j] <= upper[j - 1])
+      return fail(ctx, RTNN_ERR_INVALID, "class bounds must increase");
+  for (uint32_t j = 0; j < nclasses; ++j) {
+    const int rc = check_options(ctx, &opts[j]);
+    if (rc) return rc;
+  }
+  ctx->class_upper.assign(upper, upper + (nclasses - 1));
+  ctx->class_opt.assign(opts, opts + nclasses);
+  return RTNN_OK;
+}
+
+int rtnn_get_size_classes(rtnn_ctx ctx, uint32_t* nclasses, uint64_t* upper,
+                          rtnn_options* opts) {
+  if (!ctx || !nclasses) return RTNN_ERR_INVALID;
+  const uint32_t n = (uint32_t)ctx->class_opt.size();
+  if (upper && n > 1) std::memcpy(upper, ctx->class_upper.data(), 8 * (n - 1));
+  if (opts && n) std::memcpy(opts, ctx->class_opt.data(), sizeof(rtnn_options) * n);
+  *nclasses = n;
+  return RTNN_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+int check_options(rtnn_ctx ctx, const rtnn_options* opt) {
   if (opt->cell_factor < 0 || !std::isfinite(opt->cell_factor))
     return fail(ctx, RTNN_ERR_INVALID, "cell_factor must be >= 0");
   const int qpw = opt->queries_per_wave;
@@ -645,9 +704,12 @@ int rtnn_set_options(rtnn_ctx ctx, const rtnn_options* opt) {
     return fail(ctx, RTNN_ERR_INVALID, "lds_tile must be 0, 1 (= 512), 256, 512 or 1024");
   if (!(opt->level_alpha >= 0.0f) || !std::isfinite(opt->level_alpha))
     return fail(ctx, RTNN_ERR_INVALID, "level_alpha must be >= 0");
-  ctx->opt = *opt;
   return RTNN_OK;
 }
+
+}  // namespace
+
+extern "C" {
 
 int rtnn_synchronize(rtnn_ctx ctx) {
   if (!ctx) return RTNN_ERR_INVALID;
@@ -1302,8 +1364,28 @@ int check_offsets(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets, uint3
 // Builds one grid per cloud (edge `cell`, grown until the total cell count
 // fits the budget) over the concatenated clouds, the sorted points and the
 // query tiles (runs of <= kTileQ points of one cloud); ctx->batch_tiles is set.
+// Size class of a cloud of np points (rtnn_set_size_classes).
+int cloud_class(rtnn_ctx ctx, uint64_t np) {
+  int j = 0;
+  while (j < (int)ctx->class_upper.size() && np > ctx->class_upper[j]) ++j;
+  return j;
+}
+
+// Cell edge per cloud: its class's cell factor over r with size classes set,
+// else `cell` for every cloud.
+std::vector<float> batched_cells(rtnn_ctx ctx, const uint64_t* offsets, uint32_t nclouds,
+                                 int mode, float r, float cell) {
+  std::vector<float> v(nclouds, cell);
+  if (ctx->class_opt.empty()) return v;
+  for (uint32_t c = 0; c < nclouds; ++c) {
+    const rtnn_options& o = ctx->class_opt[cloud_class(ctx, offsets[c + 1] - offsets[c])];
+    v[c] = r / (o.cell_factor > 0 ? o.cell_factor : default_cell_factor(mode));
+  }
+  return v;
+}
+
 int build_batched(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets, uint32_t nclouds,
-                  float cell) {
+                  const std::vector<float>& cells) {
   const uint64_t n = offsets[nclouds];
   // Per-cloud bbox on the device, plan on the host.
   CK(ctx->offs.ensure(8 * (nclouds + 1)));
@@ -1324,10 +1406,11 @@ int build_batched(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets, uint3
   std::vector<GridDesc> gd(nclouds);
   uint64_t total = 0;
   const uint64_t cap = max_cells_for(ctx, n);
+  float gf = 1.0f;  // common growth of every cloud's edge
   for (int tries = 0;; ++tries) {
     total = 0;
     for (uint32_t c = 0; c < nclouds; ++c) {
-      plan_grid(&hb[6 * c], cell, gd[c]);
+      plan_grid(&hb[6 * c], cells[c] * gf, gd[c]);
       if (offsets[c + 1] == offsets[c]) gd[c].dim[0] = gd[c].dim[1] = gd[c].dim[2] = 1;
       gd[c].cell_base = (uint32_t)total;
       gd[c].point_base = (uint32_t)offsets[c];
@@ -1336,7 +1419,7 @@ int build_batched(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets, uint3
     }
     if (total <= cap && total < 0xfffffff0ull) break;
     if (tries > 200) return fail(ctx, RTNN_ERR_UNSUPPORTED, "grid planning failed");
-    cell *= 1.25f;
+    gf *= 1.25f;
   }
   ctx->hgrid = gd[0];
   ctx->ncells = total;
@@ -1361,16 +1444,67 @@ int build_batched(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets, uint3
   // Tiles: runs of <= kTileQ sorted queries inside one cloud.  Cell keys are
   // cloud-major, so cloud c's points occupy sorted positions
   // [offsets[c], offsets[c+1]).
+  // With size classes the tiles are class-major (class j's tiles are
+  // class_tiles[j] .. class_tiles[j+1]), one launch per class.
   std::vector<Tile> tl;
   tl.reserve(n / kTileQ + nclouds);
-  for (uint32_t c = 0; c < nclouds; ++c)
-    for (uint64_t q0 = offsets[c]; q0 < offsets[c + 1]; q0 += kTileQ)
-      tl.push_back(Tile{(uint32_t)q0, (uint32_t)std::min<uint64_t>(q0 + kTileQ, offsets[c + 1]),
-                        c, 0});
+  const int ncls = std::max<int>(1, (int)ctx->class_opt.size());
+  ctx->class_tiles.assign(1, 0);
+  for (int j = 0; j < ncls; ++j) {
+    for (uint32_t c = 0; c < nclouds; ++c) {
+      if (ncls > 1 && cloud_class(ctx, offsets[c + 1] - offsets[c]) != j) continue;
+      for (uint64_t q0 = offsets[c]; q0 < offsets[c + 1]; q0 += kTileQ)
+        tl.push_back(Tile{(uint32_t)q0,
+                          (uint32_t)std::min<uint64_t>(q0 + kTileQ, offsets[c + 1]), c, 0});
+    }
+    ctx->class_tiles.push_back((uint32_t)tl.size());
+  }
   CK(ctx->tiles.ensure(sizeof(Tile) * tl.size()));
   CK(hipMemcpyAsync(ctx->tiles.p, tl.data(), sizeof(Tile) * tl.size(), hipMemcpyHostToDevice,
                     ctx->st));
   ctx->batch_tiles = (uint32_t)tl.size();
+  return RTNN_OK;
+}
+
+// Lanes per query of options o for (mode, k).
+int opt_lanes(const rtnn_options& o, int mode, uint32_t k) {
+  const int qpw = o.queries_per_wave;
+  return qpw > 0 ? 64 / qpw : default_group_lanes(mode, k);
+}
+
+// The batched search over a's tiles: one persistent launch with options o, or
+// with size classes one per class over its tile range with its options.
+hipError_t launch_batched(const SearchArgs& a, int mode, uint32_t k, int num_cus,
+                          const rtnn_options& o, const std::vector<uint32_t>& ctiles,
+                          const std::vector<rtnn_options>& copt, hipStream_t st) {
+  const bool knn = mode == RTNN_MODE_KNN;
+  if (copt.empty())
+    return launch_search(a, knn, opt_lanes(o, mode, k), st, num_cus, o.group_buffer != 1,
+                         o.lds_tile == 1 ? 512 : o.lds_tile);
+  for (size_t j = 0; j < copt.size(); ++j) {
+    const uint32_t t0 = ctiles[j], t1 = ctiles[j + 1];
+    if (t0 == t1) continue;
+    SearchArgs b = a;
+    b.tiles = a.tiles + t0;
+    b.ntiles = t1 - t0;
+    const rtnn_options& c = copt[j];
+    const hipError_t e = launch_search(b, knn, opt_lanes(c, mode, k), st, num_cus,
+                                       c.group_buffer != 1, c.lds_tile == 1 ? 512 : c.lds_tile);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+// Fails unless every configuration the batched search would launch covers k.
+int check_batched_lanes(rtnn_ctx ctx, int mode, uint32_t k) {
+  std::vector<rtnn_options> all = ctx->class_opt;
+  if (all.empty()) all.push_back(ctx->opt);
+  for (const rtnn_options& o : all) {
+    const int gl = opt_lanes(o, mode, k);
+    if (gl < 0 || k > max_k_for_group(gl))
+      return fail(ctx, RTNN_ERR_UNSUPPORTED, "k=%u not supported by queries_per_wave=%d", k,
+                  o.queries_per_wave);
+  }
   return RTNN_OK;
 }
 
@@ -1438,20 +1572,21 @@ int rtnn_search_batched(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets,
   if (rc) return rc;
   const uint64_t n = offsets[nclouds];
   if (n == 0) return RTNN_OK;
-  const int gl = group_lanes_from_opt(ctx, mode, k);
-  if (gl < 0 || k > max_k_for_group(gl))
-    return fail(ctx, RTNN_ERR_UNSUPPORTED, "k=%u not supported by the selected kernel", k);
+  rc = check_batched_lanes(ctx, mode, k);
+  if (rc) return rc;
   CK(hipSetDevice(ctx->device));
   const bool timing = ctx->opt.time_stages != 0 || stats != nullptr;
   if (timing) CK(hipEventRecord(ctx->ev[0], ctx->st));
   const float f = ctx->opt.cell_factor > 0 ? ctx->opt.cell_factor : default_cell_factor(mode);
-  rc = build_batched(ctx, xyz, offsets, nclouds, r / f);
+  rc = build_batched(ctx, xyz, offsets, nclouds,
+                     batched_cells(ctx, offsets, nclouds, mode, r, r / f));
   if (rc) return rc;
   if (timing) CK(hipEventRecord(ctx->ev[1], ctx->st));
 
   OutBufs o{out_idx, out_d2};
   SearchArgs a = batched_args(ctx, n, r, k, o);
-  CK(launch_search(a, mode == RTNN_MODE_KNN, gl, ctx->st, ctx->num_cus, wide_buffers(ctx), tile_points(ctx)));
+  CK(launch_batched(a, mode, k, ctx->num_cus, ctx->opt, ctx->class_tiles, ctx->class_opt,
+                    ctx->st));
   if (timing) CK(hipEventRecord(ctx->ev[2], ctx->st));
   CK(hipStreamSynchronize(ctx->st));
   if (stats) {
@@ -1459,8 +1594,8 @@ int rtnn_search_batched(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets,
     stats->ms_grid = ev_ms(ctx->ev[0], ctx->ev[1]);
     stats->ms_search = ev_ms(ctx->ev[1], ctx->ev[2]);
     stats->ms_total = ev_ms(ctx->ev[0], ctx->ev[2]);
-    stats->search_launches = 1;
-    stats->lanes_per_query = (uint32_t)gl;
+    stats->search_launches = (uint32_t)std::max<size_t>(1, ctx->class_opt.size());
+    stats->lanes_per_query = (uint32_t)opt_lanes(ctx->opt, mode, k);
     fill_stats(ctx, stats);
   }
   return RTNN_OK;
@@ -1530,7 +1665,7 @@ int rtnn_boundary_pairs_batched(rtnn_ctx ctx, const float* xyz, const uint64_t* 
   const uint64_t n = offsets[nclouds];
   if (n == 0) return RTNN_OK;
   CK(hipSetDevice(ctx->device));
-  rc = build_batched(ctx, xyz, offsets, nclouds, r);
+  rc = build_batched(ctx, xyz, offsets, nclouds, std::vector<float>(nclouds, r));
   if (rc) return rc;
   OutBufs o{nullptr, nullptr};
   SearchArgs a = batched_args(ctx, n, r, 1, o);
@@ -1658,6 +1793,9 @@ int rtnn_autotune_batched(rtnn_ctx ctx, const float* xyz, const uint64_t* offset
   if (!ctx) return RTNN_ERR_INVALID;
   int rc = check_search_args(ctx, mode, r, k, out_idx);
   if (rc) return rc;
+  // One configuration for the whole batch: size classes are dropped.
+  ctx->class_upper.clear();
+  ctx->class_opt.clear();
   const rtnn_options base = ctx->opt;
   double best_t = 1e300;
   rtnn_options best_opt = base;
@@ -1689,6 +1827,78 @@ int rtnn_autotune_batched(rtnn_ctx ctx, const float* xyz, const uint64_t* offset
   ctx->opt = best_opt;
   if (best) *best = best_opt;
   if (best_ms) *best_ms = best_t;
+  return RTNN_OK;
+}
+
+int rtnn_autotune_batched_classes(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets,
+                                  uint32_t nclouds, int mode, float r, uint32_t k,
+                                  int32_t* out_idx, float* out_d2, uint32_t nclasses,
+                                  const uint64_t* upper, rtnn_options* best, double* best_ms) {
+  if (!ctx) return RTNN_ERR_INVALID;
+  int rc = check_search_args(ctx, mode, r, k, out_idx);
+  if (rc) return rc;
+  rc = check_offsets(ctx, xyz, offsets, nclouds);
+  if (rc) return rc;
+  if (nclasses == 0 || nclasses > 16 || (nclasses > 1 && !upper))
+    return fail(ctx, RTNN_ERR_INVALID, "1..16 classes with %s", "bounds");
+  for (uint32_t j = 1; j + 1 < nclasses; ++j)
+    if (upper[j] <= upper[j - 1]) return fail(ctx, RTNN_ERR_INVALID, "class bounds must increase");
+  CK(hipSetDevice(ctx->device));
+  // Every class is tuned on the sub-batch of its clouds (gathered into one
+  // buffer, in batch order) with a single configuration, then all classes
+  // are installed together.
+  ctx->class_upper.assign(upper, upper + (nclasses - 1));
+  ctx->class_opt.clear();
+  std::vector<int> cls(nclouds);
+  for (uint32_t c = 0; c < nclouds; ++c) cls[c] = cloud_class(ctx, offsets[c + 1] - offsets[c]);
+  const rtnn_options base = ctx->opt;
+  std::vector<rtnn_options> opts(nclasses, base);
+  std::vector<double> times(nclasses, 0.0);
+  DevBuf sub;
+  int out = RTNN_OK;
+  for (uint32_t j = 0; j < nclasses && out == RTNN_OK; ++j) {
+    std::vector<uint64_t> so(1, 0);
+    for (uint32_t c = 0; c < nclouds; ++c)
+      if (cls[c] == (int)j) so.push_back(so.back() + (offsets[c + 1] - offsets[c]));
+    const uint32_t nc = (uint32_t)so.size() - 1;
+    if (nc == 0 || so.back() == 0) continue;
+    const hipError_t he = sub.ensure(12 * so.back());
+    if (he != hipSuccess) {
+      out = hip_fail(ctx, he, "allocating a class's sub-batch");
+      break;
+    }
+    // Copy the class's clouds, merging runs of adjacent clouds.
+    uint64_t dst = 0;
+    for (uint32_t c = 0; c < nclouds;) {
+      if (cls[c] != (int)j) {
+        ++c;
+        continue;
+      }
+      uint32_t e = c + 1;
+      while (e < nclouds && cls[e] == (int)j) ++e;
+      const uint64_t np = offsets[e] - offsets[c];
+      const hipError_t ce = np ? hipMemcpyAsync(sub.as<float>() + 3 * dst, xyz + 3 * offsets[c],
+                                                12 * np, hipMemcpyDeviceToDevice, ctx->st)
+                               : hipSuccess;
+      if (ce != hipSuccess) {
+        out = hip_fail(ctx, ce, "gathering a class's clouds");
+        break;
+      }
+      dst += np;
+      c = e;
+    }
+    if (out != RTNN_OK) break;
+    ctx->opt = base;
+    out = rtnn_autotune_batched(ctx, sub.as<float>(), so.data(), nc, mode, r, k, out_idx, out_d2,
+                                &opts[j], &times[j]);
+  }
+  sub.release();
+  ctx->opt = base;
+  if (out != RTNN_OK) return out;
+  ctx->class_upper.assign(upper, upper + (nclasses - 1));
+  ctx->class_opt = opts;
+  if (best) std::memcpy(best, opts.data(), sizeof(rtnn_options) * nclasses);
+  if (best_ms) std::memcpy(best_ms, times.data(), sizeof(double) * nclasses);
   return RTNN_OK;
 }
 
@@ -1731,15 +1941,20 @@ struct rtnn_graph_s {
   hipGraphExec_t exec = nullptr;
   // Buffers owned by the graph, so calls on the context between replays
   // cannot move memory the graph's nodes point at.
-  DevBuf offs, bboxes, status, grids, keys, vals, cell_start, spts, tmp, tiles;
+  DevBuf offs, bboxes, status, grids, keys, vals, cell_start, spts, tmp, tiles, ccells;
   uint64_t n = 0;
   uint32_t nclouds = 0, ntiles = 0;
   uint64_t cap_cells = 0;
+  // Options at capture: the whole batch's, or one per size class with the
+  // class-major tile ranges class_tiles[j] .. [j+1].
+  rtnn_options opt;
+  std::vector<rtnn_options> class_opt;
+  std::vector<uint32_t> class_tiles;
   ~rtnn_graph_s() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     DevBuf* all[] = {&offs, &bboxes, &status, &grids, &keys, &vals, &cell_start, &spts, &tmp,
-                     &tiles};
+                     &tiles, &ccells};
     for (DevBuf* b : all) b->release();
   }
 };
@@ -1755,7 +1970,7 @@ namespace {
 // clear -> per-cloud bbox -> device-side grid plan -> cell keys -> radix sort
 // -> cell table -> gather -> one persistent search.  Kernels only: no host
 // synchronisation, allocation, memset or host copy, so it can be captured.
-hipError_t enqueue_batched(rtnn_graph g, const float* xyz, float cell, int mode, int gl, float r,
+hipError_t enqueue_batched(rtnn_graph g, const float* xyz, float cell, int mode, float r,
                            uint32_t k, int32_t* out_idx, float* out_d2) {
   hipStream_t st = g->ctx->st;
   uint32_t* status = g->status.as<uint32_t>();
@@ -1766,7 +1981,8 @@ hipError_t enqueue_batched(rtnn_graph g, const float* xyz, float cell, int mode,
   if ((e = fill_u32(status, 4, 0, st)) != hipSuccess) return e;
   if ((e = bbox_segmented(xyz, off, g->nclouds, g->bboxes.as<float>(), status, st)) != hipSuccess)
     return e;
-  if ((e = plan_batched(g->bboxes.as<float>(), off, g->nclouds, cell, g->cap_cells,
+  const float* cells = g->class_opt.empty() ? nullptr : g->ccells.as<float>();
+  if ((e = plan_batched(g->bboxes.as<float>(), off, g->nclouds, cell, cells, g->cap_cells,
                         g->grids.as<GridDesc>(), status, st)) != hipSuccess)
     return e;
   if ((e = cell_keys(xyz, g->n, g->grids.as<GridDesc>(), off, g->nclouds, keys, vals, nullptr,
@@ -1795,8 +2011,7 @@ hipError_t enqueue_batched(rtnn_graph g, const float* xyz, float cell, int mode,
   a.out_d2 = out_d2;
   a.local_index = 1;
   a.persistent = 1;
-  return launch_search(a, mode == RTNN_MODE_KNN, gl, st, g->ctx->num_cus, wide_buffers(g->ctx),
-                       tile_points(g->ctx));
+  return launch_batched(a, mode, k, g->ctx->num_cus, g->opt, g->class_tiles, g->class_opt, st);
 }
 
 }  // namespace
@@ -1815,21 +2030,24 @@ int rtnn_graph_batched(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets,
   if (rc) return rc;
   const uint64_t n = offsets[nclouds];
   if (n == 0) return fail(ctx, RTNN_ERR_INVALID, "the batch holds no points");
-  const int gl = group_lanes_from_opt(ctx, mode, k);
-  if (gl < 0 || k > max_k_for_group(gl))
-    return fail(ctx, RTNN_ERR_UNSUPPORTED, "k=%u not supported by the selected kernel", k);
+  rc = check_batched_lanes(ctx, mode, k);
+  if (rc) return rc;
   CK(hipSetDevice(ctx->device));
   const float f = ctx->opt.cell_factor > 0 ? ctx->opt.cell_factor : default_cell_factor(mode);
   const float cell = r / f;
+  const std::vector<float> cells = batched_cells(ctx, offsets, nclouds, mode, r, cell);
   // One host-planned build sizes the cell table and the tiles; replays plan
-  // on the device within that table (growing the edge if new data needs it).
-  rc = build_batched(ctx, xyz, offsets, nclouds, cell);
+  // on the device within that table (growing the edges if new data needs it).
+  rc = build_batched(ctx, xyz, offsets, nclouds, cells);
   if (rc) return rc;
   std::unique_ptr<rtnn_graph_s> g(new rtnn_graph_s);
   g->ctx = ctx;
   g->n = n;
   g->nclouds = nclouds;
   g->ntiles = ctx->batch_tiles;
+  g->opt = ctx->opt;
+  g->class_opt = ctx->class_opt;
+  g->class_tiles = ctx->class_tiles;
   g->cap_cells = std::min<uint64_t>(max_cells_for(ctx, n), ctx->ncells + ctx->ncells / 4 + 4096);
   g->cap_cells = std::max<uint64_t>(g->cap_cells, ctx->ncells);
   CK(g->offs.ensure(8 * (nclouds + 1)));
@@ -1842,17 +2060,20 @@ int rtnn_graph_batched(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets,
   CK(g->spts.ensure(16 * n));
   CK(g->tmp.ensure(4 * std::max(radix_tmp_words(n), cell_bounds_tmp_words(g->cap_cells))));
   CK(g->tiles.ensure(sizeof(Tile) * g->ntiles));
+  CK(g->ccells.ensure(4 * (uint64_t)nclouds));
+  CK(hipMemcpyAsync(g->ccells.p, cells.data(), 4 * (uint64_t)nclouds, hipMemcpyHostToDevice,
+                    ctx->st));
   CK(hipMemcpyAsync(g->offs.p, offsets, 8 * (nclouds + 1), hipMemcpyHostToDevice, ctx->st));
   CK(hipMemcpyAsync(g->tiles.p, ctx->tiles.p, sizeof(Tile) * g->ntiles, hipMemcpyDeviceToDevice,
                     ctx->st));
   // Eager run first: loads every kernel and checks the inputs.
-  CK(enqueue_batched(g.get(), xyz, cell, mode, gl, r, k, out_idx, out_d2));
+  CK(enqueue_batched(g.get(), xyz, cell, mode, r, k, out_idx, out_d2));
   uint32_t hs[4];
   CK(hipMemcpyAsync(hs, g->status.p, 16, hipMemcpyDeviceToHost, ctx->st));
   CK(hipStreamSynchronize(ctx->st));
   if (hs[0]) return fail(ctx, RTNN_ERR_INVALID, "%u non-finite point coordinates", hs[0]);
   CK(hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
-  const hipError_t e = enqueue_batched(g.get(), xyz, cell, mode, gl, r, k, out_idx, out_d2);
+  const hipError_t e = enqueue_batched(g.get(), xyz, cell, mode, r, k, out_idx, out_d2);
   hipGraph_t graph = nullptr;
   const hipError_t e2 = hipStreamEndCapture(ctx->st, &graph);
   g->graph = graph;
@@ -1878,7 +2099,7 @@ int rtnn_graph_launch(rtnn_graph g, int sync, rtnn_stats* stats) {
   if (stats) {
     std::memset(stats, 0, sizeof *stats);
     stats->ms_total = ev_ms(ctx->ev[6], ctx->ev[7]);
-    stats->search_launches = 1;
+    stats->search_launches = (uint32_t)std::max<size_t>(1, g->class_opt.size());
   }
   if (hs[0])
     return fail(ctx, RTNN_ERR_INVALID, "%u non-finite point coordinates (rows undefined)", hs[0]);

@@ -166,33 +166,38 @@ __device__ inline uint64_t block_sum_u64(uint64_t v, uint64_t* sh) {
 // cell (inv = 0): slow but exact, and status[1] is set.
 __global__ __launch_bounds__(kPlanThreads) void k_plan_batched(
     const float* __restrict__ bboxes, const uint64_t* __restrict__ off, uint32_t nclouds,
-    float cell, uint64_t cap, GridDesc* __restrict__ grids, uint32_t* __restrict__ status) {
+    float cell, const float* __restrict__ cells, uint64_t cap, GridDesc* __restrict__ grids,
+    uint32_t* __restrict__ status) {
   __shared__ uint64_t sh[kPlanThreads / kWave];
   __shared__ uint64_t s_scan[kPlanThreads];
-  float c = cell;
+  // Cloud i's edge: cells[i] (size classes) or cell, times the common growth.
+  float gf = 1.0f;
   bool ok = false;
   for (int tries = 0; tries <= 200 && !ok; ++tries) {
-    const float inv = 1.0f / c;
     uint64_t tot = 0;
     for (uint32_t i = threadIdx.x; i < nclouds; i += kPlanThreads) {
       int d[3];
-      plan_dims(bboxes + 6 * i, inv, off[i + 1] == off[i], d);
+      plan_dims(bboxes + 6 * i, 1.0f / ((cells ? cells[i] : cell) * gf), off[i + 1] == off[i], d);
       tot = min(tot + cells_sat(d[0], d[1], d[2]), (uint64_t)1e15);  // no wrap
     }
     tot = block_sum_u64(tot, sh);
     ok = tot <= cap && tot < 0xfffffff0ull;
-    if (!ok) c *= 1.25f;
+    if (!ok) gf *= 1.25f;
   }
-  const float inv = ok ? 1.0f / c : 0.0f;
   if (!ok && threadIdx.x == 0) status[1] = 1u;
   uint64_t carry = 0;
   for (uint32_t i0 = 0; i0 < nclouds; i0 += kPlanThreads) {
     const uint32_t i = i0 + threadIdx.x;
     int d[3] = {1, 1, 1};
     bool empty = true;
+    float c = FLT_MAX, inv = 0.0f;
     if (i < nclouds) {
       empty = off[i + 1] == off[i];
-      if (ok) plan_dims(bboxes + 6 * i, inv, empty, d);
+      if (ok) {
+        c = (cells ? cells[i] : cell) * gf;
+        inv = 1.0f / c;
+        plan_dims(bboxes + 6 * i, inv, empty, d);
+      }
     }
     const uint64_t mine = i < nclouds ? cells_sat(d[0], d[1], d[2]) : 0;
     // inclusive scan of this chunk (Hillis-Steele in LDS)
@@ -213,7 +218,7 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan_batched(
         g.dim[a] = d[a];
       }
       g.inv = inv;
-      g.cell = ok ? c : FLT_MAX;
+      g.cell = c;
       g.cell_base = (uint32_t)(carry + s_scan[threadIdx.x] - mine);
       g.point_base = (uint32_t)off[i];
       g.npoints = (uint32_t)(off[i + 1] - off[i]);
@@ -226,11 +231,11 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan_batched(
 }
 
 hipError_t plan_batched(const float* bboxes, const uint64_t* offsets, uint32_t nclouds,
-                        float cell, uint64_t cap, GridDesc* grids, uint32_t* status,
-                        hipStream_t st) {
+                        float cell, const float* cells, uint64_t cap, GridDesc* grids,
+                        uint32_t* status, hipStream_t st) {
   if (nclouds == 0) return hipSuccess;
   hipLaunchKernelGGL(k_plan_batched, dim3(1), dim3(kPlanThreads), 0, st, bboxes, offsets,
-                     nclouds, cell, cap, grids, status);
+                     nclouds, cell, cells, cap, grids, status);
   return hipGetLastError();
 }
 

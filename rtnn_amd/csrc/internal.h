@@ -27,11 +27,12 @@ hipError_t bbox(const float* xyz, uint64_t n, float* part, float* out, uint32_t*
 hipError_t bbox_segmented(const float* xyz, const uint64_t* offsets, uint32_t nclouds,
                           float* out, uint32_t* nonfinite, hipStream_t st);
 // Per-cloud grids planned on the device from the per-cloud bboxes: edge
-// `cell`, grown by 1.25x until the total cell count is <= cap (one cell per
-// cloud if none fits, status[1] = 1); status[2] = total cells.
+// cells[i] (device array, size classes) or `cell` when cells is null, every
+// edge grown by the same 1.25x steps until the total cell count is <= cap
+// (one cell per cloud if none fits, status[1] = 1); status[2] = total cells.
 hipError_t plan_batched(const float* bboxes, const uint64_t* offsets, uint32_t nclouds,
-                        float cell, uint64_t cap, GridDesc* grids, uint32_t* status,
-                        hipStream_t st);
+                        float cell, const float* cells, uint64_t cap, GridDesc* grids,
+                        uint32_t* status, hipStream_t st);
 // keys[i] = grid cell of point i, vals[i] = i, counts[key] += 1 (counts nullable).
 hipError_t cell_keys(const float* xyz, uint64_t n, const GridDesc* grids,
                      const uint64_t* offsets, uint32_t nclouds, uint32_t* keys,

@@ -374,6 +374,102 @@ def test_autotune_batched_keeps_result(ctx):
     assert torch.equal(got_i, want_i) and torch.equal(got_d, want_d)
 
 
+@pytest.fixture
+def fresh_ctx():
+    # Size classes stay on a context: these tests use one of their own.
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    c = rtnn_amd.Context(0)
+    yield c
+    c.close()
+
+
+CLASS_OPTS = [{"cell_factor": 0.5, "queries_per_wave": 1},
+              {"cell_factor": 1.0, "queries_per_wave": 16, "group_buffer": 1, "lds_tile": 256},
+              {"cell_factor": 2.0, "queries_per_wave": 4, "group_buffer": 2}]
+
+
+@pytest.mark.parametrize("mode,r,k", [("radius", 0.2, 64), ("knn", 0.25, 8)])
+def test_batched_size_classes(fresh_ctx, mode, r, k):
+    # Three classes with different cell edges, lanes and LDS use in one grid
+    # build: the rows equal the per-cloud pipeline's and the oracle's.
+    import torch
+    ctx = fresh_ctx
+    xyz, offs = rtnn_amd.gen_lattice_clouds(21, 40, 300, 9000)
+    offs = np.insert(offs, 7, offs[7])  # an empty cloud
+    t = torch.from_numpy(xyz).cuda()
+    want_i, want_d, _ = ctx.search_percloud(t, offs, mode=mode, radius=r, k=k)
+    ctx.set_size_classes([1500, 5000], CLASS_OPTS)
+    up, opts = ctx.size_classes()
+    assert up == [1500, 5000] and [o["queries_per_wave"] for o in opts] == [1, 16, 4]
+    sizes = np.diff(offs)
+    assert (sizes <= 1500).any() and ((sizes > 1500) & (sizes <= 5000)).any()
+    assert (sizes > 5000).any()
+    got_i, got_d, st = ctx.search_batched(t, offs, mode=mode, radius=r, k=k)
+    assert st.search_launches == 3
+    assert torch.equal(got_i, want_i) and torch.equal(got_d, want_d)
+    gi = got_i.cpu().numpy()
+    for c in (int(np.argmin(sizes + (sizes == 0) * 10**9)), int(np.argmax(sizes))):
+        a, b = int(offs[c]), int(offs[c + 1])
+        wi, wd = oracle.grid_search(xyz[a:b], xyz[a:b], MODES[mode], r, k, threads=8)
+        assert np.array_equal(gi[a:b], wi)
+    # the captured pipeline uses the same classes
+    g = ctx.graph_batched(t, offs, mode=mode, radius=r, k=k)
+    try:
+        for rep in range(3):
+            g.idx.fill_(-7)
+            st = g.launch()
+            assert st.search_launches == 3
+            assert torch.equal(g.idx, want_i), rep
+            assert torch.equal(g.d2, want_d), rep
+            ctx.search_batched(t, offs, mode=mode, radius=r, k=k)  # eager work between
+            torch.cuda.synchronize()
+    finally:
+        g.close()
+    ctx.set_size_classes([], [])
+    assert ctx.size_classes() == ([], [])
+    _, _, st = ctx.search_batched(t, offs, mode=mode, radius=r, k=k)
+    assert st.search_launches == 1
+
+
+def test_size_class_errors(fresh_ctx):
+    ctx = fresh_ctx
+    with pytest.raises(rtnn_amd.RtnnError, match="increase"):
+        ctx.set_size_classes([5000, 1500], CLASS_OPTS)
+    with pytest.raises(rtnn_amd.RtnnError):
+        ctx.set_size_classes([1500, 5000], [CLASS_OPTS[0], {"lds_tile": 300}, CLASS_OPTS[2]])
+    with pytest.raises(ValueError):
+        ctx.set_size_classes([1500], CLASS_OPTS)
+    assert ctx.size_classes() == ([], [])
+    # a class whose lanes cannot hold k fails the search, not silently
+    import torch
+    xyz, offs = rtnn_amd.gen_lattice_clouds(22, 6, 300, 3000)
+    t = torch.from_numpy(xyz).cuda()
+    ctx.set_size_classes([1000], [{"queries_per_wave": 32}, {}])
+    with pytest.raises(rtnn_amd.RtnnError, match="k=64"):
+        ctx.search_batched(t, offs, mode="radius", radius=0.2, k=64)
+
+
+def test_autotune_batched_classes(fresh_ctx):
+    import torch
+    ctx = fresh_ctx
+    xyz, offs = rtnn_amd.gen_lattice_clouds(23, 30, 300, 12000)
+    t = torch.from_numpy(xyz).cuda()
+    want_i, want_d, _ = ctx.search_percloud(t, offs, mode="radius", radius=0.2, k=64)
+    opts, ms = ctx.autotune_batched_classes(t, offs, [2000, 6000], mode="radius", radius=0.2,
+                                            k=64)
+    assert len(opts) == 3 and all(m > 0 for m in ms)
+    assert all(o["queries_per_wave"] in (1, 4, 8, 16, 32, 64) for o in opts)
+    assert ctx.size_classes() == ([2000, 6000], opts)
+    got_i, got_d, st = ctx.search_batched(t, offs, mode="radius", radius=0.2, k=64)
+    assert st.search_launches == 3
+    assert torch.equal(got_i, want_i) and torch.equal(got_d, want_d)
+    # the single-configuration tuner drops the classes
+    ctx.autotune_batched(t, offs, mode="radius", radius=0.2, k=64)
+    assert ctx.size_classes() == ([], [])
+
+
 def test_search_async_on_a_torch_stream(ctx):
     # Enqueued on a caller's stream; the context's next (synchronous) call is
     # ordered after it without any host synchronisation in between.
