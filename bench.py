@@ -34,6 +34,24 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def copy_rate_gbs(torch):
+    """Achievable HBM rate of a plain 1 GiB device copy (bytes read + written
+    per second), timed with events; reported beside the 8 TB/s spec peak."""
+    n = 1 << 28
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(5):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / 5
+    del src, dst
+    return 2 * 4 * n / (ms * 1e-3) / 1e9
+
+
 def pmc_traffic(workload_name, kernel):
     """HBM bytes per launch of `kernel` on this workload, from the committed
     rocprofv3 counter summary profiles/hbm_traffic.md (FETCH_SIZE and
@@ -379,6 +397,10 @@ def main():
             "kernel_ms": round(kms, 4), "algorithmic_bytes": algo, "bytes_definition": algo_def}
     if c27_note:
         roof["c27"] = c27_note
+    copy_gbs = copy_rate_gbs(torch)
+    roof["copy_rate_gbs"] = round(copy_gbs, 1)
+    roof["frac_of_copy_rate"] = round(achieved / copy_gbs, 5)
+    log(f"[bench] device copy rate {copy_gbs:.0f} GB/s (1 GiB copy)")
 
     # exactness on a sample + CPU baseline (oracle brute force on the sample)
     import oracle
@@ -422,6 +444,23 @@ def main():
                       f"{len(pts)} points ({dt:.1f} s)")
         cpu = {"value": round(ns / dt, 3), "unit": "queries/s", "cores": cores, "kind": "port",
                "sample": sample}
+        # the reference's check path runs on one thread: the same brute force
+        # on one core over a subsample of about the same CPU time
+        ns1 = max(1, ns // cores)
+        rows1 = sel[:ns1] if "offsets" not in w else None
+        t = time.perf_counter()
+        if rows1 is not None:
+            oracle.bruteforce(pts, pts if qs is None else qs, om, r, k, qsel=rows1, threads=1)
+        else:
+            c = int(clouds[0])
+            a, b = int(offs[c]), int(offs[c + 1])
+            rows1 = np.arange(min(ns1, b - a), dtype=np.uint64)
+            oracle.bruteforce(pts[a:b], pts[a:b], om, r, k, qsel=rows1, threads=1)
+        dt1 = time.perf_counter() - t
+        cpu["single_thread"] = {"value": round(len(rows1) / dt1, 3), "unit": "queries/s",
+                                "cores": 1, "sample": f"{len(rows1)} of the same queries "
+                                                      f"({dt1:.1f} s)"}
+        log(f"[bench] cpu baseline, one thread: {len(rows1) / dt1:.1f} q/s")
         log(f"[bench] cpu baseline {ns / dt:.1f} q/s on {cores} threads; mismatches {mism}")
 
     value = results[head]["queries_per_s"]

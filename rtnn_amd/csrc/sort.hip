@@ -6,14 +6,16 @@
 // sort.cpp:535-612 uses for the counting sort of particles into cells.
 //
 // Radix sort, one 8-bit digit per pass, three kernels per pass:
-//   rs_hist    : per 4096-key tile, a 256-bin digit histogram (LDS atomics),
-//                stored digit-major so one exclusive scan gives every tile's
-//                global base for every digit;
-//   scan       : exclusive scan of that table;
-//   rs_scatter : per tile, stable local ranks from wave64 ballot matching
-//                (8 ballots per key) plus per-wave digit counts, keys staged
-//                in LDS in digit order, then written out so consecutive
-//                lanes store consecutive addresses of one digit run.
+//   rs_hist    : per 4096-key tile, a digit histogram (LDS atomics), stored
+//                digit-major (row d = every tile's count of digit d);
+//   rs_rowscan : one block per digit scans its row (the tile's start within
+//                the digit) and writes the digit's total;
+//   rs_scatter : per tile, the global digit starts from the totals; each
+//                wave ranks a contiguous quarter of the tile by wave64 ballot
+//                matching (one ballot per digit bit) against its own running
+//                digit counts, a scan over the four waves orders the quarters,
+//                keys are staged in LDS in digit order and written out so
+//                consecutive lanes store consecutive addresses of a digit run.
 // Stability (equal keys keep input order) makes the sort deterministic.
 #include "common.h"
 #include "internal.h"
@@ -144,19 +146,33 @@ hipError_t exclusive_scan(const uint32_t* in, uint32_t* out, uint64_t n, uint32_
 
 // ----------------------------------------------------------- radix sort --
 
+// 8-bit digits: ceil(bits / 8) passes.  (9- and 10-bit digits, three passes
+// for 27- or 30-bit keys, measured slower on gfx950: 0.35 vs 0.31 ms for 10M
+// 27-bit keys, 7.9 vs 4.6 ms for 150M 28-bit keys; the scatter's digit runs
+// get shorter and its LDS use cuts the blocks per CU.)
+constexpr int kRsDigit = 8;
+
 __global__ __launch_bounds__(kRsThreads) void k_rs_hist(const uint32_t* __restrict__ keys,
-                                                        uint64_t n, int shift, uint32_t ntiles,
+                                                        uint64_t n, int shift, uint32_t lowmask,
+                                                        uint32_t ntiles,
                                                         uint32_t* __restrict__ hist) {
-  __shared__ uint32_t h[256];
-  h[threadIdx.x] = 0;
+  constexpr int NB = 1 << kRsDigit;
+  __shared__ uint32_t h[NB];
+  for (int i = threadIdx.x; i < NB; i += kRsThreads) h[i] = 0;
   __syncthreads();
-  uint64_t base = (uint64_t)blockIdx.x * kRsTile;
+  const uint64_t base = (uint64_t)blockIdx.x * kRsTile;
   const int lane = threadIdx.x & (kWave - 1);
-#pragma unroll 4
+  uint32_t rk[kRsItems];
+#pragma unroll
   for (int i = 0; i < kRsItems; ++i) {
-    uint64_t e = base + (uint64_t)i * kRsThreads + threadIdx.x;
+    const uint64_t e = base + (uint64_t)i * kRsThreads + threadIdx.x;
+    rk[i] = e < n ? keys[e] : 0u;
+  }
+#pragma unroll
+  for (int i = 0; i < kRsItems; ++i) {
+    const uint64_t e = base + (uint64_t)i * kRsThreads + threadIdx.x;
     const bool valid = e < n;
-    const uint32_t d = valid ? (keys[e] >> shift) & 255u : 0u;
+    const uint32_t d = valid ? ((rk[i] & lowmask) >> shift) & (NB - 1) : 0u;
     // A wave whose keys share one digit (common for nearly sorted input:
     // clouds laid out cell by cell) adds its count with one atomic instead
     // of 64 to the same LDS word.
@@ -171,7 +187,39 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_hist(const uint32_t* __restri
     }
   }
   __syncthreads();
-  hist[(uint64_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+  for (int i = threadIdx.x; i < NB; i += kRsThreads)
+    hist[(uint64_t)i * ntiles + blockIdx.x] = h[i];
+}
+
+// Block d: exclusive scan of row d of the digit-major tile histogram (tile t's
+// count of digit d at d * ntiles + t) into hscan, and the row's total into
+// totals[d].  One launch per pass instead of a device-wide scan of the table.
+__global__ __launch_bounds__(kScanThreads) void k_rs_rowscan(const uint32_t* __restrict__ hraw,
+                                                             uint32_t ntiles,
+                                                             uint32_t* __restrict__ hscan,
+                                                             uint32_t* __restrict__ totals) {
+  __shared__ uint32_t sh[kScanThreads / kWave + 1];
+  const uint64_t row = (uint64_t)blockIdx.x * ntiles;
+  uint32_t run = 0;
+  for (uint32_t b = 0; b < ntiles; b += kScanTile) {
+    const uint32_t first = b + threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
+    uint32_t s = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+      v[i] = first + i < ntiles ? hraw[row + first + i] : 0u;
+      s += v[i];
+    }
+    uint32_t tot;
+    uint32_t ex = block_excl_scan(s, sh, tot) + run;
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+      if (first + i < ntiles) hscan[row + first + i] = ex;
+      ex += v[i];
+    }
+    run += tot;
+  }
+  if (threadIdx.x == 0) totals[blockIdx.x] = run;
 }
 
 __device__ inline uint64_t lanemask_lt() {
@@ -182,70 +230,108 @@ __device__ inline uint64_t lanemask_lt() {
 __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(
     const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
     uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, uint64_t n, int shift,
-    uint32_t ntiles, const uint32_t* __restrict__ hist_raw,
-    const uint32_t* __restrict__ hist_scan) {
+    uint32_t lowmask, uint32_t ntiles, const uint32_t* __restrict__ hist_raw,
+    const uint32_t* __restrict__ hist_scan, const uint32_t* __restrict__ totals) {
+  constexpr int DB = kRsDigit;
+  constexpr int NB = 1 << DB;
+  constexpr int PT = NB / kRsThreads;               // digits per thread in the digit scans
+  constexpr int kWaveItems = kRsTile / kRsWaves;    // consecutive tile items of one wave
   __shared__ uint32_t s_keys[kRsTile];
   __shared__ uint32_t s_vals[kRsTile];
-  __shared__ uint32_t s_lbase[256];    // tile-local start of each digit
-  __shared__ uint32_t s_gbase[256];    // global start of this tile's digit run
-  __shared__ uint32_t s_run[256];      // keys of each digit placed so far
-  __shared__ uint32_t s_wcnt[kRsWaves][256];
-  __shared__ uint32_t s_wofs[kRsWaves][256];
+  __shared__ uint32_t s_lbase[NB];  // tile-local start of each digit
+  __shared__ uint32_t s_gbase[NB];  // global start of this tile's digit run
+  __shared__ uint32_t s_cnt[kRsWaves][NB];
   __shared__ uint32_t s_scan[kRsThreads / kWave + 1];
 
   const int t = threadIdx.x;
   const int w = t / kWave;
+  const int lane = t & (kWave - 1);
   const uint64_t base = (uint64_t)blockIdx.x * kRsTile;
   const uint32_t tile_n = (uint32_t)((n - base) < (uint64_t)kRsTile ? (n - base) : kRsTile);
 
   {
-    uint32_t c = hist_raw[(uint64_t)t * ntiles + blockIdx.x];
-    s_gbase[t] = hist_scan[(uint64_t)t * ntiles + blockIdx.x];
-    uint32_t tot;
-    // block_excl_scan expects kScanThreads == kRsThreads threads.
-    uint32_t lane_ex = block_excl_scan(c, s_scan, tot);
-    s_lbase[t] = lane_ex;
-    s_run[t] = 0;
+    // Thread t owns digits t*PT .. t*PT+PT-1: the tile-local digit starts
+    // scan this tile's counts; the global starts add the exclusive scan of
+    // the digit totals to the tile's prefix within its digit row.
+    uint32_t c[PT], tt[PT];
+    uint32_t cs = 0, ts = 0;
 #pragma unroll
-    for (int i = 0; i < kRsWaves; ++i) s_wcnt[i][t] = 0;
+    for (int j = 0; j < PT; ++j) {
+      const uint64_t d = (uint64_t)(t * PT + j);
+      c[j] = hist_raw[d * ntiles + blockIdx.x];
+      tt[j] = totals[d];
+      cs += c[j];
+      ts += tt[j];
+    }
+    uint32_t tot;
+    uint32_t lex = block_excl_scan(cs, s_scan, tot);
+    uint32_t gex = block_excl_scan(ts, s_scan, tot);
+#pragma unroll
+    for (int j = 0; j < PT; ++j) {
+      const int d = t * PT + j;
+      s_lbase[d] = lex;
+      lex += c[j];
+      s_gbase[d] = gex + hist_scan[(uint64_t)d * ntiles + blockIdx.x];
+      gex += tt[j];
+#pragma unroll
+      for (int i = 0; i < kRsWaves; ++i) s_cnt[i][d] = 0;
+    }
   }
   __syncthreads();
 
+  // Wave w ranks the tile's items [w*kWaveItems, (w+1)*kWaveItems) on its
+  // own, with running per-digit counts in its LDS row: no block barrier
+  // inside the loop.  All loads are issued before the first is used.
+  uint32_t rk[kRsItems], rv[kRsItems], rr[kRsItems];
+#pragma unroll
   for (int it = 0; it < kRsItems; ++it) {
-    const uint32_t j = (uint32_t)it * kRsThreads + t;
+    const uint32_t j = (uint32_t)(w * kWaveItems + it * kWave + lane);
+    rk[it] = j < tile_n ? kin[base + j] : 0u;
+    rv[it] = j < tile_n ? vin[base + j] : 0u;
+  }
+#pragma unroll
+  for (int it = 0; it < kRsItems; ++it) {
+    const uint32_t j = (uint32_t)(w * kWaveItems + it * kWave + lane);
     const bool valid = j < tile_n;
-    uint32_t key = 0, val = 0, d = 0;
-    if (valid) {
-      key = kin[base + j];
-      val = vin[base + j];
-      d = (key >> shift) & 255u;
-    }
+    const uint32_t d = valid ? ((rk[it] & lowmask) >> shift) & (NB - 1) : 0u;
+    // lanes of this wave with the same digit, from one ballot per digit bit
     uint64_t match = __ballot(valid);
 #pragma unroll
-    for (int b = 0; b < 8; ++b) {
+    for (int b = 0; b < DB; ++b) {
       const bool bit = (d >> b) & 1u;
       const uint64_t bb = __ballot(bit);
       match &= bit ? bb : ~bb;
     }
-    const uint32_t rank = (uint32_t)__popcll(match & lanemask_lt());
-    if (valid && rank == 0) s_wcnt[w][d] = (uint32_t)__popcll(match);
-    __syncthreads();
-    {
-      uint32_t run = s_run[t];
+    const uint32_t below = (uint32_t)__popcll(match & lanemask_lt());
+    const uint32_t cnt = (uint32_t)__popcll(match);
+    // Every lane reads its digit's running count before the group's last
+    // lane writes the new one (one wave: LDS operations in program order).
+    const uint32_t cur = valid ? s_cnt[w][d] : 0u;
+    rr[it] = cur + below;
+    if (valid && below == cnt - 1) s_cnt[w][d] = cur + cnt;
+  }
+  __syncthreads();
+  // Per digit: waves in order -> each wave's start inside the digit's run.
 #pragma unroll
-      for (int i = 0; i < kRsWaves; ++i) {
-        uint32_t c = s_wcnt[i][t];
-        s_wofs[i][t] = run;
-        run += c;
-        s_wcnt[i][t] = 0;
-      }
-      s_run[t] = run;
+  for (int j = 0; j < PT; ++j) {
+    const int dd = t * PT + j;
+    uint32_t run = 0;
+#pragma unroll
+    for (int i = 0; i < kRsWaves; ++i) {
+      const uint32_t c = s_cnt[i][dd];
+      s_cnt[i][dd] = run;
+      run += c;
     }
-    __syncthreads();
-    if (valid) {
-      const uint32_t lp = s_lbase[d] + s_wofs[w][d] + rank;
-      s_keys[lp] = key;
-      s_vals[lp] = val;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < kRsItems; ++it) {
+    const uint32_t j = (uint32_t)(w * kWaveItems + it * kWave + lane);
+    if (j < tile_n) {
+      const uint32_t d = ((rk[it] & lowmask) >> shift) & (NB - 1);
+      const uint32_t lp = s_lbase[d] + s_cnt[w][d] + rr[it];
+      s_keys[lp] = rk[it];
+      s_vals[lp] = rv[it];
     }
   }
   __syncthreads();
@@ -253,7 +339,7 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(
     const uint32_t j = (uint32_t)it * kRsThreads + t;
     if (j < tile_n) {
       const uint32_t key = s_keys[j];
-      const uint32_t d = (key >> shift) & 255u;
+      const uint32_t d = ((key & lowmask) >> shift) & (NB - 1);
       const uint64_t g = (uint64_t)s_gbase[d] + (j - s_lbase[d]);
       kout[g] = key;
       vout[g] = s_vals[j];
@@ -262,39 +348,46 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(
 }
 
 uint64_t radix_tmp_words(uint64_t n) {
-  uint64_t ntiles = (n + kRsTile - 1) / kRsTile;
-  uint64_t hist = 256 * ntiles;
-  // keys + vals ping-pong, raw hist, scanned hist (+1), scan partials
-  return 2 * n + hist + (hist + 1) + (uint64_t)scan_tmp_words(hist) + 64;
+  const uint64_t ntiles = (n + kRsTile - 1) / kRsTile;
+  const uint64_t hist = (1ull << kRsDigit) * ntiles;
+  // two key/value ping-pong buffers, raw and scanned tile histograms, totals
+  return 4 * n + 2 * hist + (1ull << kRsDigit) + 64;
 }
 
+
 // Stable sort of (keys, vals) on key bits [0, bits).  Result in keys/vals.
+// Passes after the first alternate between two scratch buffers and the last
+// writes keys/vals, so no copy follows unless there is a single pass.
 hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint64_t n, int bits,
                             uint32_t* tmp, hipStream_t st, int* passes_out) {
-  int passes = (bits + 7) / 8;
+  const int passes = bits <= 0 ? 0 : (bits + kRsDigit - 1) / kRsDigit;
   if (passes_out) *passes_out = passes;
   if (n <= 1 || passes == 0) return hipSuccess;
-  uint32_t ntiles = (uint32_t)((n + kRsTile - 1) / kRsTile);
-  uint64_t hsz = 256ull * ntiles;
-  uint32_t* k2 = tmp;
-  uint32_t* v2 = k2 + n;
-  uint32_t* hraw = v2 + n;
+  const uint32_t lowmask = bits >= 32 ? 0xffffffffu : ((1u << bits) - 1u);
+  const uint32_t ntiles = (uint32_t)((n + kRsTile - 1) / kRsTile);
+  const uint64_t hsz = (1ull << kRsDigit) * ntiles;
+  uint32_t* bk[2] = {tmp, tmp + 2 * n};
+  uint32_t* bv[2] = {tmp + n, tmp + 3 * n};
+  uint32_t* hraw = tmp + 4 * n;
   uint32_t* hscan = hraw + hsz;
-  uint32_t* stmp = hscan + hsz + 1;
-  uint32_t *ka = keys, *va = vals, *kb = k2, *vb = v2;
+  uint32_t* totals = hscan + hsz;
+  const uint32_t* ka = keys;
+  const uint32_t* va = vals;
   for (int p = 0; p < passes; ++p) {
-    int shift = 8 * p;
-    hipLaunchKernelGGL(k_rs_hist, dim3(ntiles), dim3(kRsThreads), 0, st, ka, n, shift, ntiles,
-                       hraw);
-    hipError_t e = exclusive_scan(hraw, hscan, hsz, stmp, st);
-    if (e != hipSuccess) return e;
+    const bool last = p == passes - 1 && passes > 1;
+    uint32_t* kb = last ? keys : bk[p & 1];
+    uint32_t* vb = last ? vals : bv[p & 1];
+    const int shift = kRsDigit * p;
+    hipLaunchKernelGGL(k_rs_hist, dim3(ntiles), dim3(kRsThreads), 0, st, ka, n, shift, lowmask,
+                       ntiles, hraw);
+    hipLaunchKernelGGL(k_rs_rowscan, dim3(1u << kRsDigit), dim3(kScanThreads), 0, st, hraw,
+                       ntiles, hscan, totals);
     hipLaunchKernelGGL(k_rs_scatter, dim3(ntiles), dim3(kRsThreads), 0, st, ka, va, kb, vb, n,
-                       shift, ntiles, hraw, hscan);
-    uint32_t* t;
-    t = ka; ka = kb; kb = t;
-    t = va; va = vb; vb = t;
+                       shift, lowmask, ntiles, hraw, hscan, totals);
+    ka = kb;
+    va = vb;
   }
-  if (ka != keys) {
+  if (passes == 1) {
     hipError_t e = hipMemcpyAsync(keys, ka, n * 4, hipMemcpyDeviceToDevice, st);
     if (e != hipSuccess) return e;
     e = hipMemcpyAsync(vals, va, n * 4, hipMemcpyDeviceToDevice, st);

@@ -27,11 +27,15 @@ def test_exclusive_scan(ctx, n):
 
 
 @pytest.mark.parametrize("n,bits", [(1, 8), (1000, 8), (4097, 13), (100000, 24),
-                                    (1 << 20, 32), (777777, 30)])
+                                    (1 << 20, 32), (777777, 30), (5000, 9), (70000, 10),
+                                    (300001, 17), (300001, 19), (250000, 20), (400000, 25),
+                                    (1 << 20, 27), (17_000_000, 20)])
 def test_radix_sort_stable(ctx, n, bits):
     rng = np.random.default_rng(bits * 1000 + n)
     hi = (1 << bits) if bits < 32 else (1 << 32)
-    keys = rng.integers(0, min(hi, 5000 if n > 1000 else hi), n, dtype=np.uint64).astype(np.uint32)
+    # few distinct keys (long equal runs: stability) or the full key range
+    top = hi if (n <= 1000 or bits in (9, 10, 17, 19, 20, 25, 27)) else min(hi, 5000)
+    keys = rng.integers(0, top, n, dtype=np.uint64).astype(np.uint32)
     if bits < 32:
         keys &= np.uint32((1 << bits) - 1)
     vals = np.arange(n, dtype=np.uint32)
@@ -42,15 +46,18 @@ def test_radix_sort_stable(ctx, n, bits):
     assert np.array_equal(tv.cpu().numpy().view(np.uint32), vals[order])
 
 
-def test_radix_sort_low_bits_only(ctx):
-    # Sorting on the low 8 bits must leave order among equal low bytes stable.
-    rng = np.random.default_rng(1)
-    keys = rng.integers(0, 1 << 31, 50000, dtype=np.uint64).astype(np.uint32)
+@pytest.mark.parametrize("bits", [8, 20, 25, 27])
+def test_radix_sort_low_bits_only(ctx, bits):
+    # Sorting on the low bits must ignore the bits above them (also where the
+    # passes' digits together span more bits) and keep ties stable.
+    rng = np.random.default_rng(bits)
+    keys = rng.integers(0, 1 << 32, 50000, dtype=np.uint64).astype(np.uint32)
     vals = np.arange(len(keys), dtype=np.uint32)
     tk, tv = _t(keys.view(np.int32)), _t(vals.view(np.int32))
-    ctx.radix_sort_pairs(tk, tv, 8)
-    order = np.argsort(keys & 255, kind="stable")
+    ctx.radix_sort_pairs(tk, tv, bits)
+    order = np.argsort(keys & np.uint32((1 << bits) - 1), kind="stable")
     assert np.array_equal(tv.cpu().numpy().view(np.uint32), vals[order])
+    assert np.array_equal(tk.cpu().numpy().view(np.uint32), keys[order])
 
 
 def test_morton3_matches_oracle(ctx):
