@@ -39,7 +39,8 @@ struct Tile {
 
 // Kernel argument block shared by the search kernels.
 struct SearchArgs {
-  const float4* spts;        // points sorted by cell: (x, y, z, bits(orig index))
+  const float4* spts;        // points sorted by cell: (x, y, z, bits(orig index)),
+                             // followed by kSptsPadBytes of readable padding
   const uint32_t* cell_start;  // global cell table, cells + 1 entries
   const GridDesc* grids;
   const float4* queries;     // processing order: (x, y, z, bits(output row))
@@ -78,6 +79,12 @@ struct SearchArgs {
 };
 
 constexpr int kTileQ = 256;  // queries per tile (= threads of a KNN block)
+
+// Sorted point buffers are allocated this much beyond their last point: the
+// group kernels load a batch of candidate slots without a per-slot bounds
+// select, and a slot past the end of its run reads at most 64 slots (1 KiB)
+// further on (its value is not used).
+constexpr uint64_t kSptsPadBytes = 4096;
 
 // Radius growth of a row that found m < k hits within R: the k-th distance
 // scales like (k / m)^(1/2) for points on surfaces and (k / m)^(1/3) in volume,

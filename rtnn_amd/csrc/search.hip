@@ -123,6 +123,7 @@ __device__ inline float query_radius(const SearchArgs& a, uint32_t qi, uint32_t 
 // runs (rows or cells) whose bounds each group lane loads per batch.
 constexpr int kUnroll = 4;
 constexpr int kGroupUnroll = 8;
+static_assert(64 * kGroupUnroll * 16 <= (int)kSptsPadBytes * 2, "padding of the sorted points");
 constexpr int kRunsPerLane = 1;
 
 // --------------------------------------------------------------------------
@@ -693,39 +694,58 @@ __global__ __launch_bounds__(kTileQ) void k_topk_group(const SearchArgs a) {
         for (uint32_t v0 = 0; v0 < total; v0 += G * kGroupUnroll) {
           // Positions of kGroupUnroll candidates per lane first, then their
           // loads as one straight-line batch with no branch in between, so
-          // all of them are in flight before the first test.  A slot past the
-          // end loads position 0 (valid) and is not a hit.
-          uint32_t pos[kGroupUnroll];
+          // all of them are in flight before the first test.
+          float4 pv[kGroupUnroll];
+          if constexpr (kRegRuns) {
+            // Candidate u of lane l is v = vl + G*u.  Run j (offset off_j,
+            // first point s_j) would hold v = vl at pb[j] = spts + s_j +
+            // (vl - off_j), 64-bit (vl - off_j may be negative); candidate u
+            // adds G*u points, a constant that goes into the load's offset
+            // field.  Its run is the last j with G*u >= off_j - vl, so the
+            // address read is s_j + (v - off_j) >= s_j.  A slot past the end
+            // reads at most G*kGroupUnroll points beyond some run's end (the
+            // buffer's padding covers it) and is not a hit.
+            const uint32_t vl = v0 + (uint32_t)l;
+            const float4* pb[RB];
+            int tj[RB];
 #pragma unroll
-          for (int u = 0; u < kGroupUnroll; ++u) {
-            const uint32_t v = v0 + (uint32_t)(u * G + l);
-            if constexpr (kRegRuns) {
-              uint32_t d = rt[0].y;
+            for (int j = 0; j < RB; ++j) {
+              tj[j] = (int)(rt[j].x - vl);
+              pb[j] = a.spts + (uint64_t)(rt[j].y + rt[j].x) - (int64_t)tj[j];
+            }
 #pragma unroll
-              for (int j = 1; j < RB; ++j) d = v >= rt[j].x ? rt[j].y : d;
-              pos[u] = v < total ? v + d : 0u;
-            } else {
+            for (int u = 0; u < kGroupUnroll; ++u) {
+              const float4* pp = pb[0];
+#pragma unroll
+              for (int j = 1; j < RB; ++j) pp = G * u >= tj[j] ? pb[j] : pp;
+              pv[u] = pp[G * u];
+            }
+          } else {
+            uint32_t pos[kGroupUnroll];
+#pragma unroll
+            for (int u = 0; u < kGroupUnroll; ++u) {
+              // A slot past the end loads position 0 and is not a hit.
+              const uint32_t v = v0 + (uint32_t)(u * G + l);
               pos[u] = 0;
               if (v < total) {
                 while (ro[rp + 1] <= v) ++rp;
                 pos[u] = rs[rp] + (v - ro[rp]);
               }
             }
-          }
-          float4 pv[kGroupUnroll];
 #pragma unroll
-          for (int u = 0; u < kGroupUnroll; ++u) pv[u] = a.spts[pos[u]];
+            for (int u = 0; u < kGroupUnroll; ++u) pv[u] = a.spts[pos[u]];
+          }
           // Hits of this lane as a bit mask, then one group scan of the
           // counts places them; the buffer keeps CAP - G free of this batch
           // (the entry condition of group_take).  A batch that does not fit
           // goes through group_take, one candidate at a time.
           uint64_t key[kGroupUnroll];
           uint32_t bits = 0;
+          const int tt = (int)(total - v0 - (uint32_t)l);  // valid while G*u < tt
 #pragma unroll
           for (int u = 0; u < kGroupUnroll; ++u) {
-            const uint32_t v = v0 + (uint32_t)(u * G + l);
             const float d2 = d2_of(qv.x, qv.y, qv.z, pv[u].x, pv[u].y, pv[u].z);
-            const bool ok = (KNN ? (d2 > 0.0f && d2 <= lim) : (d2 <= lim)) && v < total;
+            const bool ok = (KNN ? (d2 > 0.0f && d2 <= lim) : (d2 <= lim)) && G * u < tt;
             key[u] = pack_key(d2, __float_as_uint(pv[u].w));
             bits |= ok ? 1u << u : 0u;
           }
