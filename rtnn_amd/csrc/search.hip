@@ -493,12 +493,63 @@ __device__ inline void rank_chunk(const uint64_t* buf, uint32_t cnt, uint32_t ba
 }
 constexpr int kRankChunk = 8;
 
+// Number of buffer entries with d2 <= t, in every lane of the group.  Called
+// with the group's lanes all active (group_scan_fast's moves stay inside it).
+template <int G>
+__device__ inline uint32_t group_count_le(const uint64_t* buf, uint32_t cnt, float t, int l) {
+  uint32_t c = 0;
+  for (uint32_t e = (uint32_t)l; e < cnt; e += G) c += key_d2(buf[e]) <= t ? 1u : 0u;
+  return group_last<G>(group_scan_fast<G>(c, l), l);
+}
+
+// Cheap partial select: lowers the limit to a d2 threshold t with at least k
+// entries at d2 <= t and drops the entries above it (in place, order kept).
+// t comes from up to four interpolation steps between the counts at 0 and at
+// the limit (aiming at about k + k/4 entries); no (d2, index) ranking.  Every
+// entry that can be among the k nearest stays.  Returns with cnt >= k.
+template <int G>
+__device__ inline void group_shrink(uint64_t* buf, uint32_t& cnt, float& lim, uint32_t k,
+                                    int l, uint64_t gmask) {
+  const uint32_t target = k + (k >> 2) + 1;
+  float lo = 0.0f, hi = lim;
+  uint32_t clo = 0, chi = cnt;
+  for (int it = 0; it < 4 && chi > target; ++it) {
+    const float f = fminf(fmaxf((float)(target - clo) / (float)(chi - clo), 0.05f), 0.95f);
+    const float t = lo + (hi - lo) * f;
+    const uint32_t c = group_count_le<G>(buf, cnt, t, l);
+    if (c >= k) {
+      hi = t;
+      chi = c;
+    } else {
+      lo = t;
+      clo = c;
+    }
+  }
+  if (chi < cnt) {
+    uint32_t kept = 0;
+    for (uint32_t e0 = 0; e0 < cnt; e0 += G) {
+      const uint32_t e = e0 + (uint32_t)l;
+      const uint64_t key = e < cnt ? buf[e] : ~0ull;
+      const bool keep = e < cnt && key_d2(key) <= hi;
+      const uint64_t km = __ballot(keep) & gmask;
+      wave_sync();
+      if (keep) buf[kept + (uint32_t)__popcll(km & lanemask_lt64())] = key;
+      kept += (uint32_t)__popcll(km);
+      wave_sync();
+    }
+    cnt = kept;
+    lim = hi;
+  }
+}
+
 // One step of a group of G lanes: lane l tests candidate p (if valid) and
 // the group appends its hits to the LDS buffer (ballot + prefix popcount).
-// Entry: cnt <= CAP - G.  When the buffer nears CAP, the k smallest are kept:
-// the k-th smallest key T is found by ranking, the entries <= T are
-// compacted to the front in place (a round writes below the entries it has
-// read), and later hits must have d2 <= d2(T).
+// Entry: cnt <= CAP - G.  When the buffer nears CAP it is cut back: first by
+// a d2 threshold (group_shrink) where the buffer has room above k for it,
+// else, or when ties leave it too full, to the k smallest: the k-th smallest
+// key T is found by ranking, the entries <= T are compacted to the front in
+// place (a round writes below the entries it has read), and later hits must
+// have d2 <= d2(T).
 template <int G, int CAP, bool KNN>
 __device__ inline void group_take(const float4& p, bool valid, const float4& qv,
                                   uint64_t gmask, uint64_t* buf, uint32_t& cnt, float& lim,
@@ -513,7 +564,15 @@ __device__ inline void group_take(const float4& p, bool valid, const float4& qv,
   const uint64_t m = __ballot(hit) & gmask;
   if (hit) buf[cnt + (uint32_t)__popcll(m & lanemask_lt64())] = key;
   cnt += (uint32_t)__popcll(m);
-  if (cnt > (uint32_t)(CAP - G)) {
+  if (cnt <= (uint32_t)(CAP - G)) return;
+  // With room for it, a threshold step first; the exact select below runs
+  // when that leaves the buffer more than half way from k to full (ties).
+  if ((uint32_t)(CAP - G) >= k + (k >> 1)) {
+    wave_sync();
+    group_shrink<G>(buf, cnt, lim, k, l, gmask);
+    if (cnt <= (k + (uint32_t)(CAP - G)) / 2) return;
+  }
+  {
     wave_sync();
     for (uint32_t b0 = 0; b0 < cnt; b0 += G * kRankChunk) {
       uint64_t kv[kRankChunk];
