@@ -600,23 +600,41 @@ __device__ inline void group_take(const float4& p, bool valid, const float4& qv,
   }
 }
 
+// Entries each lane ranks per round in group_finish: enough that one round
+// (G * CH entries) covers the KB + KB/4 + 1 a threshold step leaves, up to 12,
+// else kRankChunk.
+template <int G, int KB>
+constexpr int finish_chunk() {
+  constexpr int t = KB + KB / 4 + 1;
+  constexpr int ch = (t + G - 1) / G;
+  return ch <= 12 ? ch : kRankChunk;
+}
+
 // Writes a group's row: the buffer's entries by rank (the k smallest), then
 // empty slots; appends the query to the short list if it has fewer than k
 // and was searched below r (a row searched at r is final: it holds every
-// point within r).
-template <int G>
-__device__ inline void group_finish(const SearchArgs& a, const uint64_t* buf, uint32_t cnt,
-                                    uint32_t k, int l, uint32_t row, uint32_t base,
+// point within r).  A buffer beyond one ranking round is first cut by a d2
+// threshold (all entries are within lim).
+template <int G, int KB>
+__device__ inline void group_finish(const SearchArgs& a, uint64_t* buf, uint32_t cnt,
+                                    float lim, uint32_t k, int l, uint32_t row, uint32_t base,
                                     uint32_t qs, float R) {
+  constexpr int CH = finish_chunk<G, KB>();
   wave_sync();
+  if (cnt > (uint32_t)(G * CH) && k + (k >> 2) + 1 <= (uint32_t)(G * CH)) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const uint64_t gmask = G == kWave ? ~0ull : (((1ull << G) - 1ull) << (lane - l));
+    group_shrink<G>(buf, cnt, lim, k, l, gmask);
+    wave_sync();
+  }
   int32_t* oi = a.out_idx + (uint64_t)row * k;
   float* od = a.out_d2 ? a.out_d2 + (uint64_t)row * k : nullptr;
-  for (uint32_t b0 = 0; b0 < cnt; b0 += G * kRankChunk) {
-    uint64_t kv[kRankChunk];
-    uint32_t rk[kRankChunk];
-    rank_chunk<G, kRankChunk>(buf, cnt, b0, l, kv, rk);
+  for (uint32_t b0 = 0; b0 < cnt; b0 += G * CH) {
+    uint64_t kv[CH];
+    uint32_t rk[CH];
+    rank_chunk<G, CH>(buf, cnt, b0, l, kv, rk);
 #pragma unroll
-    for (int i = 0; i < kRankChunk; ++i)
+    for (int i = 0; i < CH; ++i)
       if (rk[i] < k) {
         oi[rk[i]] = (int32_t)(key_idx(kv[i]) - base);
         if (od) od[rk[i]] = key_d2(kv[i]);
@@ -829,7 +847,7 @@ __global__ __launch_bounds__(kTileQ) void k_topk_group(const SearchArgs a) {
         wave_sync();
       }
 
-      group_finish<G>(a, buf, cnt, k, l, row, a.local_index ? g.point_base : 0u, qs, R);
+      group_finish<G, KB>(a, buf, cnt, lim, k, l, row, a.local_index ? g.point_base : 0u, qs, R);
       wave_sync();
     }
   }
@@ -987,7 +1005,8 @@ __global__ __launch_bounds__(kTileQ) void k_topk_tiled(const SearchArgs a) {
         }
       }
       if (active)
-        group_finish<G>(a, buf, cnt, k, l, row, a.local_index ? g.point_base : 0u, qs, R);
+        group_finish<G, KB>(a, buf, cnt, lim, k, l, row, a.local_index ? g.point_base : 0u, qs,
+                            R);
       wave_sync();  // the tile and buffers are reused by the next round
     }
   }
