@@ -550,7 +550,7 @@ __device__ inline void group_shrink(uint64_t* buf, uint32_t& cnt, float& lim, ui
 // key T is found by ranking, the entries <= T are compacted to the front in
 // place (a round writes below the entries it has read), and later hits must
 // have d2 <= d2(T).
-template <int G, int CAP, bool KNN>
+template <int G, int KB, int CAP, bool KNN>
 __device__ inline void group_take(const float4& p, bool valid, const float4& qv,
                                   uint64_t gmask, uint64_t* buf, uint32_t& cnt, float& lim,
                                   uint32_t k, int l, uint64_t* thr_slot) {
@@ -565,12 +565,16 @@ __device__ inline void group_take(const float4& p, bool valid, const float4& qv,
   if (hit) buf[cnt + (uint32_t)__popcll(m & lanemask_lt64())] = key;
   cnt += (uint32_t)__popcll(m);
   if (cnt <= (uint32_t)(CAP - G)) return;
-  // With room for it, a threshold step first; the exact select below runs
-  // when that leaves the buffer more than half way from k to full (ties).
-  if ((uint32_t)(CAP - G) >= k + (k >> 1)) {
-    wave_sync();
-    group_shrink<G>(buf, cnt, lim, k, l, gmask);
-    if (cnt <= (k + (uint32_t)(CAP - G)) / 2) return;
+  // Where the buffer has room above k for it (decided at compile time, so
+  // the other configurations keep their code as it was), a threshold step
+  // first; the exact select below runs when that leaves the buffer more than
+  // half way from k to full (ties).
+  if constexpr (CAP - G >= KB + KB / 2) {
+    if ((uint32_t)(CAP - G) >= k + (k >> 1)) {
+      wave_sync();
+      group_shrink<G>(buf, cnt, lim, k, l, gmask);
+      if (cnt <= (k + (uint32_t)(CAP - G)) / 2) return;
+    }
   }
   {
     wave_sync();
@@ -621,11 +625,13 @@ __device__ inline void group_finish(const SearchArgs& a, uint64_t* buf, uint32_t
                                     uint32_t qs, float R) {
   constexpr int CH = finish_chunk<G, KB>();
   wave_sync();
-  if (cnt > (uint32_t)(G * CH) && k + (k >> 2) + 1 <= (uint32_t)(G * CH)) {
-    const int lane = threadIdx.x & (kWave - 1);
-    const uint64_t gmask = G == kWave ? ~0ull : (((1ull << G) - 1ull) << (lane - l));
-    group_shrink<G>(buf, cnt, lim, k, l, gmask);
-    wave_sync();
+  if constexpr (G * CH >= KB + KB / 4 + 1) {
+    if (cnt > (uint32_t)(G * CH) && k + (k >> 2) + 1 <= (uint32_t)(G * CH)) {
+      const int lane = threadIdx.x & (kWave - 1);
+      const uint64_t gmask = G == kWave ? ~0ull : (((1ull << G) - 1ull) << (lane - l));
+      group_shrink<G>(buf, cnt, lim, k, l, gmask);
+      wave_sync();
+    }
   }
   int32_t* oi = a.out_idx + (uint64_t)row * k;
   float* od = a.out_d2 ? a.out_d2 + (uint64_t)row * k : nullptr;
@@ -839,7 +845,7 @@ __global__ __launch_bounds__(kTileQ) void k_topk_group(const SearchArgs a) {
 #pragma unroll
             for (int u = 0; u < kGroupUnroll; ++u) {
               const uint32_t v = v0 + (uint32_t)(u * G + l);
-              group_take<G, C::CAP, KNN>(pv[u], v < total, qv, gmask, buf, cnt, lim, k, l,
+              group_take<G, KB, C::CAP, KNN>(pv[u], v < total, qv, gmask, buf, cnt, lim, k, l,
                                          &s_thr[gq]);
             }
           }
@@ -970,7 +976,7 @@ __global__ __launch_bounds__(kTileQ) void k_topk_tiled(const SearchArgs a) {
         for (uint32_t v0 = 0; v0 < T; v0 += G) {
           const uint32_t v = v0 + (uint32_t)l;
           const float4 p = v < T ? tile[v] : make_float4(0.f, 0.f, 0.f, 0.f);
-          group_take<G, C::CAP, KNN>(p, v < T, qv, gmask, buf, cnt, lim, k, l, &s_thr[gq]);
+          group_take<G, KB, C::CAP, KNN>(p, v < T, qv, gmask, buf, cnt, lim, k, l, &s_thr[gq]);
         }
       } else if (active && mine) {
         const int nrows = ny * nz;
@@ -998,7 +1004,7 @@ __global__ __launch_bounds__(kTileQ) void k_topk_tiled(const SearchArgs a) {
               while (ro[rp + 1] <= v) ++rp;
               p = a.spts[rs[rp] + (v - ro[rp])];
             }
-            group_take<G, C::CAP, KNN>(p, v < total, qv, gmask, buf, cnt, lim, k, l,
+            group_take<G, KB, C::CAP, KNN>(p, v < total, qv, gmask, buf, cnt, lim, k, l,
                                        &s_thr[gq]);
           }
           wave_sync();
