@@ -416,9 +416,11 @@ def main():
             per = max(1, args.cpu_sample // len(clouds))
             t = time.perf_counter()
             ns = 0
+            cloud_sel = []
             for c in clouds:
                 a, b = int(offs[c]), int(offs[c + 1])
                 sel = rng.choice(b - a, min(per, b - a), replace=False).astype(np.uint64)
+                cloud_sel.append((a, b, sel))
                 wi, wd = oracle.bruteforce(pts[a:b], pts[a:b], om, r, k, qsel=sel, threads=cores)
                 rows = torch.from_numpy(sel.astype(np.int64) + a).cuda()
                 for m, (oi, od) in outputs.items():
@@ -446,21 +448,22 @@ def main():
                "sample": sample}
         # the reference's check path runs on one thread: the same brute force
         # on one core over a subsample of about the same CPU time
-        ns1 = max(1, ns // cores)
-        rows1 = sel[:ns1] if "offsets" not in w else None
         t = time.perf_counter()
-        if rows1 is not None:
-            oracle.bruteforce(pts, pts if qs is None else qs, om, r, k, qsel=rows1, threads=1)
-        else:
-            c = int(clouds[0])
-            a, b = int(offs[c]), int(offs[c + 1])
-            rows1 = np.arange(min(ns1, b - a), dtype=np.uint64)
-            oracle.bruteforce(pts[a:b], pts[a:b], om, r, k, qsel=rows1, threads=1)
+        if "offsets" not in w:
+            ns1 = max(1, ns // cores)
+            oracle.bruteforce(pts, pts if qs is None else qs, om, r, k, qsel=sel[:ns1],
+                              threads=1)
+        else:  # the same clouds, 1/cores of each cloud's sampled queries
+            ns1 = 0
+            for a, b, csel in cloud_sel:
+                sub = csel[:max(1, len(csel) // cores)]
+                oracle.bruteforce(pts[a:b], pts[a:b], om, r, k, qsel=sub, threads=1)
+                ns1 += len(sub)
         dt1 = time.perf_counter() - t
-        cpu["single_thread"] = {"value": round(len(rows1) / dt1, 3), "unit": "queries/s",
-                                "cores": 1, "sample": f"{len(rows1)} of the same queries "
+        cpu["single_thread"] = {"value": round(ns1 / dt1, 3), "unit": "queries/s",
+                                "cores": 1, "sample": f"{ns1} of the same queries "
                                                       f"({dt1:.1f} s)"}
-        log(f"[bench] cpu baseline, one thread: {len(rows1) / dt1:.1f} q/s")
+        log(f"[bench] cpu baseline, one thread: {ns1 / dt1:.1f} q/s")
         log(f"[bench] cpu baseline {ns / dt:.1f} q/s on {cores} threads; mismatches {mism}")
 
     value = results[head]["queries_per_s"]
