@@ -119,6 +119,26 @@ def test_group_buffer_overflow_path(ctx):
             check(ctx, pts, pts[:300], "radius", 2.0, 5)
 
 
+def test_lattice_ties_in_buffer_cuts(ctx):
+    # An exact integer lattice: whole shells of points at one d2 (6 at 1, 12 at
+    # 2, 8 at 3, 6 at 4, 24 at 5, ...).  A full buffer's d2-threshold step
+    # then cannot stop between tied entries, so it keeps whole shells and the
+    # exact select takes over; the rows must still be the (d2, index) order.
+    g = np.arange(0, 24, dtype=np.float32)
+    pts = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    pts = np.ascontiguousarray(pts[np.random.default_rng(5).permutation(len(pts))])
+    qs = np.concatenate([pts[:700], pts[:300] + np.float32(0.5)])
+    ctx.set_points(pts)
+    for buf in (2, 1):
+        for qpw in (1, 4, 8, 16, 32):
+            ctx.set_options(queries_per_wave=qpw, group_buffer=buf)
+            for k in (5, 16):
+                check(ctx, pts, qs, "knn", 2.6, k)
+                check(ctx, pts, qs, "radius", 2.6, k)
+            if MAX_K[64 // qpw] >= 32:
+                check(ctx, pts, qs[:400], "knn", 3.1, 32)
+
+
 def test_knn_separate_queries_gaussian_mixture(ctx):
     # Config-2 shape at 1/10 size: clusters + background, separate queries,
     # K = 8, r = 20; many background rows are short (sentinels).
