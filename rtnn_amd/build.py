@@ -1,6 +1,6 @@
 """Build recipe for the gfx950 library, its CLI and the tests' oracle.
 
-Outputs (in-tree, git-ignored, shipped with the tree to the GPU box):
+Outputs (in-tree and git-ignored; they travel with the source tree):
   rtnn_amd/librtnn_hip.so   C ABI of include/rtnn_hip.h and include/rtnn_data.h
   rtnn_amd/rtnn             optixNSearch-compatible command-line driver
 

@@ -196,7 +196,7 @@ float d2_of(const float* q, const float* p) {
 
 // Sanity check with the reference's rules (check.cpp:20-139): KNN compares 5
 // sampled queries against a brute force; radius checks every returned index is
-// within r and reports the average neighbour count.  This build also checks
+// within r and reports the average neighbour count.  This check also compares
 // the KNN indices and their (d2, index) order, and radius completeness for the
 // sampled queries.
 bool sanity_check(const Args& a, const std::vector<float>& P, const std::vector<float>& Q,

@@ -47,8 +47,9 @@ def test_golden_samplepc(golden, samplepc):
         assert np.array_equal(gi, idx) and np.array_equal(gd, d2)
 
 
-def test_golden_known_answers_from_survey(samplepc):
-    # Independent hand check of the survey's known answers at r=2 and r=10.
+def test_golden_known_answers_samplepc(samplepc):
+    # Hand-derived known answers for src/samplepc.txt at r=2 and r=10 (an
+    # FP32 brute force in file order, independent of the oracle).
     idx, d2 = oracle.bruteforce(samplepc, samplepc, oracle.MODE_RADIUS, 2.0, 4)
     assert idx[1].tolist() == [1, 5, 3, -1]
     assert idx[4].tolist() == [4, 2, -1, -1]
