@@ -162,6 +162,7 @@ struct rtnn_ctx_s {
 
   // grid
   DevBuf keys, vals, counts, cell_start, spts, tmp, grids;
+  DevBuf cboxes;  // hierarchical grids: bounding box per 64 sorted points
   GridDesc hgrid{};
   uint64_t ncells = 0;
   int sort_passes = 0;
@@ -211,7 +212,7 @@ struct rtnn_ctx_s {
     DevBuf* all[] = {&pts, &keys, &vals, &counts, &cell_start, &spts, &tmp, &grids,
                      &qin, &qkeys, &qvals, &qsorted, &oidx, &od2, &shortrows, &misc,
                      &tiles, &offs, &bboxes, &pgrid, &pcounts, &qradius, &tr_q, &tr_idx,
-                     &tr_d2, &tr_t, &tr_f, &tr_out, &bcnt, &boff, &bpairs};
+                     &tr_d2, &tr_t, &tr_f, &tr_out, &bcnt, &boff, &bpairs, &cboxes};
     for (DevBuf* b : all) b->release();
     if (st3) (void)hipStreamSynchronize(st3);
     for (int i = 0; i < 2; ++i) {
@@ -426,6 +427,8 @@ int build_hgrid(rtnn_ctx ctx, float r, int want_levels) {
                    ctx->st, 3 * l));
   CK(gather_points(ctx->pts.as<float>(), ctx->vals.as<uint32_t>(), n, ctx->spts.as<float4>(),
                    ctx->st));
+  CK(ctx->cboxes.ensure(32 * ((n + kChunkPts - 1) / kChunkPts) + 32));
+  CK(chunk_boxes(ctx->spts.as<float4>(), n, ctx->cboxes.as<float4>(), ctx->st));
   ctx->ncells = total;
   ctx->spts_current = true;
   ctx->hlevels = L;
@@ -549,6 +552,7 @@ SearchArgs base_args(rtnn_ctx ctx, const float4* qproc, uint64_t nq, float r, ui
     a.nlevels = ctx->hlevels;
     a.level_min = ctx->hlevel_min;
     a.level_alpha = ctx->opt.level_alpha > 0 ? ctx->opt.level_alpha : 1.0f;
+    a.chunk_box = ctx->cboxes.as<float4>();
   }
   return a;
 }
@@ -1432,6 +1436,7 @@ int build_batched(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets, uint3
   CK(ctx->spts.ensure(16 * n + kSptsPadBytes));
   CK(ctx->tmp.ensure(4 * std::max(radix_tmp_words(n), cell_bounds_tmp_words(total))));
   ctx->spts_current = false;  // spts now holds the clouds, not ctx->pts
+  ctx->hlevels = 0;           // and grids/cell_start the clouds' raster grids
   CK(cell_keys(xyz, n, ctx->grids.as<GridDesc>(), ctx->offs.as<uint64_t>(), nclouds,
                ctx->keys.as<uint32_t>(), ctx->vals.as<uint32_t>(), nullptr, ctx->st));
   CK(radix_sort_pairs(ctx->keys.as<uint32_t>(), ctx->vals.as<uint32_t>(), n,

@@ -76,7 +76,13 @@ struct SearchArgs {
   int nlevels;
   int level_min;
   float level_alpha;
+  // nullable: bounding box of every kChunkPts consecutive sorted points
+  // (chunk c: min at [2c], max at [2c+1]); the octree walk skips the chunks of
+  // a large leaf that lie beyond its limit.
+  const float4* chunk_box;
 };
+
+constexpr uint32_t kChunkPts = 64;
 
 constexpr int kTileQ = 256;  // queries per tile (= threads of a KNN block)
 

@@ -52,6 +52,8 @@ hipError_t count_sorted(const float4* spts, uint64_t n, const GridDesc* g, uint3
 // spts[j] = (xyz[vals[j]], bits(vals[j])).
 hipError_t gather_points(const float* xyz, const uint32_t* vals, uint64_t n, float4* spts,
                          hipStream_t st);
+// Bounding box of every kChunkPts consecutive points of spts (2 float4 each).
+hipError_t chunk_boxes(const float4* spts, uint64_t n, float4* box, hipStream_t st);
 // Morton key (10 bits per axis) of each query's cell in grid g, shifted right
 // by `shift` bits per axis; vals[i] = i.
 hipError_t query_morton_keys(const float* q, uint64_t nq, const GridDesc* g, int shift,

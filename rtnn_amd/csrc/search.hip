@@ -341,16 +341,36 @@ __global__ __launch_bounds__(kTileQ) void k_topk_tree(const SearchArgs a) {
                   if (l > L0 && e - s0 > kTreeLeaf) {
                     down = true;
                   } else {
-                    for (uint32_t j = s0; j < e; ++j) {
-                      const float4 p = a.spts[j];
-                      const float d2 = d2_of(qv.x, qv.y, qv.z, p.x, p.y, p.z);
-                      if (d2 > 0.0f && d2 <= lim) {
-                        const uint64_t key = pack_key(d2, __float_as_uint(p.w));
-                        if (key < list[KC - 1]) {
-                          topk_insert<KC>(list, key);
-                          if (list[KC - 1] != ~0ull) lim = fminf(lim, key_d2(list[KC - 1]));
+                    // A leaf beyond kTreeLeaf points (a finest-level cell)
+                    // goes chunk by chunk: a chunk whose box lies beyond the
+                    // limit (with the cells' FP32 margin) holds no hit.
+                    const bool chunked = a.chunk_box && e - s0 > kTreeLeaf;
+                    for (uint32_t c0 = s0; c0 < e;) {
+                      uint32_t c1 = e;
+                      if (chunked) {
+                        const uint32_t ch = c0 / kChunkPts;
+                        c1 = min(e, (ch + 1) * kChunkPts);
+                        const float4 blo = a.chunk_box[2 * ch], bhi = a.chunk_box[2 * ch + 1];
+                        const float bx = fmaxf(fmaxf(blo.x - qv.x, qv.x - bhi.x), 0.0f);
+                        const float by = fmaxf(fmaxf(blo.y - qv.y, qv.y - bhi.y), 0.0f);
+                        const float bz = fmaxf(fmaxf(blo.z - qv.z, qv.z - bhi.z), 0.0f);
+                        if ((bx * bx + by * by) + bz * bz > lim * 1.001f) {
+                          c0 = c1;
+                          continue;
                         }
                       }
+                      for (uint32_t j = c0; j < c1; ++j) {
+                        const float4 p = a.spts[j];
+                        const float d2 = d2_of(qv.x, qv.y, qv.z, p.x, p.y, p.z);
+                        if (d2 > 0.0f && d2 <= lim) {
+                          const uint64_t key = pack_key(d2, __float_as_uint(p.w));
+                          if (key < list[KC - 1]) {
+                            topk_insert<KC>(list, key);
+                            if (list[KC - 1] != ~0ull) lim = fminf(lim, key_d2(list[KC - 1]));
+                          }
+                        }
+                      }
+                      c0 = c1;
                     }
                   }
                 }

@@ -410,6 +410,29 @@ CLASS_OPTS = [{"cell_factor": 0.5, "queries_per_wave": 1},
               {"cell_factor": 2.0, "queries_per_wave": 4, "group_buffer": 2}]
 
 
+def test_batched_after_a_hierarchical_grid(fresh_ctx):
+    # A batched search on a context whose last build was a hierarchical Morton
+    # grid (partitioned KNN) must use the clouds' raster grids.
+    import torch
+    ctx = fresh_ctx
+    pts = rtnn_amd.gen_gaussian_mixture(2, 20, 50_000)
+    ctx.set_options(grid_levels=5, level_alpha=0.5)
+    ctx.set_points(pts)
+    ctx.search_partitioned(None, radius=20.0, k=8)
+    ctx.set_options()
+    xyz, offs = rtnn_amd.gen_lattice_clouds(9, 12, 500, 4000)
+    t = torch.from_numpy(xyz).cuda()
+    bi, bd, _ = ctx.search_batched(t, offs, mode="radius", radius=0.2, k=64)
+    pi, pd, _ = ctx.search_percloud(t, offs, mode="radius", radius=0.2, k=64)
+    assert torch.equal(bi, pi) and torch.equal(bd, pd)
+    # and the hierarchical search after it rebuilds its own grid
+    wi, wd = oracle.grid_search(pts, pts[:500], oracle.MODE_KNN, 20.0, 8, threads=8)
+    ctx.set_options(grid_levels=5, level_alpha=0.5)
+    ctx.set_points(pts)
+    gi, gd, _ = ctx.search_partitioned(pts[:500], radius=20.0, k=8)
+    assert np.array_equal(np.asarray(gi), wi) and np.array_equal(np.asarray(gd), wd)
+
+
 @pytest.mark.parametrize("mode,r,k", [("radius", 0.2, 64), ("knn", 0.25, 8)])
 def test_batched_size_classes(fresh_ctx, mode, r, k):
     # Three classes with different cell edges, lanes and LDS use in one grid
