@@ -464,37 +464,41 @@ __global__ __launch_bounds__(256) void k_gather(const float* __restrict__ xyz,
                         __uint_as_float(i));
 }
 
-// One wave per chunk of kChunkPts sorted points: the chunk's bounding box
-// (min at box[2c], max at box[2c+1]) by a wave reduction.
+// kWave / kChunkPts chunks of kChunkPts sorted points per wave: each chunk's
+// bounding box (min at box[2c], max at box[2c+1]) by a reduction over its
+// lanes (xor shuffles below kChunkPts stay inside the chunk).
 __global__ __launch_bounds__(256) void k_chunk_boxes(const float4* __restrict__ spts, uint64_t n,
                                                      float4* __restrict__ box) {
-  const uint64_t c = (uint64_t)blockIdx.x * (256 / kWave) + threadIdx.x / kWave;
-  const uint64_t nch = (n + kChunkPts - 1) / kChunkPts;
-  if (c >= nch) return;  // whole waves
+  constexpr uint32_t kPerWave = kWave / kChunkPts;
   const uint32_t lane = threadIdx.x & (kWave - 1);
-  const uint64_t j = c * kChunkPts + lane;
-  const float4 p = spts[j < n ? j : c * kChunkPts];
+  const uint64_t c = ((uint64_t)blockIdx.x * (256 / kWave) + threadIdx.x / kWave) * kPerWave +
+                     lane / kChunkPts;
+  const uint64_t nch = (n + kChunkPts - 1) / kChunkPts;
+  const bool valid = c < nch;
+  const uint64_t j = c * kChunkPts + lane % kChunkPts;
+  const float4 p = spts[!valid ? 0 : (j < n ? j : c * kChunkPts)];
   float mn[3] = {p.x, p.y, p.z}, mx[3] = {p.x, p.y, p.z};
 #pragma unroll
-  for (int o = kWave / 2; o > 0; o >>= 1) {
+  for (int o = kChunkPts / 2; o > 0; o >>= 1) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       mn[a] = fminf(mn[a], __shfl_xor(mn[a], o, kWave));
       mx[a] = fmaxf(mx[a], __shfl_xor(mx[a], o, kWave));
     }
   }
-  if (lane == 0) {
+  if (valid && lane % kChunkPts == 0) {
     box[2 * c] = make_float4(mn[0], mn[1], mn[2], 0.0f);
     box[2 * c + 1] = make_float4(mx[0], mx[1], mx[2], 0.0f);
   }
 }
-static_assert(kChunkPts == kWave, "one wave per chunk");
+static_assert(kWave % kChunkPts == 0, "whole chunks per wave");
 
 hipError_t chunk_boxes(const float4* spts, uint64_t n, float4* box, hipStream_t st) {
   const uint64_t nch = (n + kChunkPts - 1) / kChunkPts;
   if (nch == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_chunk_boxes, dim3((uint32_t)((nch + 3) / 4)), dim3(256), 0, st, spts, n,
-                     box);
+  const uint64_t waves = (nch + kWave / kChunkPts - 1) / (kWave / kChunkPts);
+  hipLaunchKernelGGL(k_chunk_boxes, dim3((uint32_t)((waves + 3) / 4)), dim3(256), 0, st, spts,
+                     n, box);
   return hipGetLastError();
 }
 

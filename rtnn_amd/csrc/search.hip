@@ -315,17 +315,27 @@ __global__ __launch_bounds__(kTileQ) void k_topk_tree(const SearchArgs a) {
     cell_range(g0, qv.y, R, 1, lo1, hi1);
     cell_range(g0, qv.z, R, 2, lo2, hi2);
     // the query's level-0 cell (clamped): its octant at every level
-    const uint32_t qm = morton3((uint32_t)min(max((int)floorf((qv.x - g0.mn[0]) * g0.inv), 0), g0.dim[0] - 1),
-                                (uint32_t)min(max((int)floorf((qv.y - g0.mn[1]) * g0.inv), 0), g0.dim[1] - 1),
-                                (uint32_t)min(max((int)floorf((qv.z - g0.mn[2]) * g0.inv), 0), g0.dim[2] - 1));
+    const int qc0 = min(max((int)floorf((qv.x - g0.mn[0]) * g0.inv), 0), g0.dim[0] - 1);
+    const int qc1 = min(max((int)floorf((qv.y - g0.mn[1]) * g0.inv), 0), g0.dim[1] - 1);
+    const int qc2 = min(max((int)floorf((qv.z - g0.mn[2]) * g0.inv), 0), g0.dim[2] - 1);
+    const uint32_t qm = morton3((uint32_t)qc0, (uint32_t)qc1, (uint32_t)qc2);
     const float mag = fabsf(g0.mn[0]) + fabsf(g0.mn[1]) + fabsf(g0.mn[2]) + fabsf(qv.x) +
                       fabsf(qv.y) + fabsf(qv.z) + (float)g0.dim[0] * g0.cell;
     if (hi0 >= lo0 && hi1 >= lo1 && hi2 >= lo2) {
       const int tl0 = lo0 >> T, th0 = hi0 >> T, tl1 = lo1 >> T, th1 = hi1 >> T;
       const int tl2 = lo2 >> T, th2 = hi2 >> T;
-      for (int tz = tl2; tz <= th2; ++tz)
-        for (int ty = tl1; ty <= th1; ++ty)
-          for (int tx = tl0; tx <= th0; ++tx) {
+      // Top-level cells in rings of growing Chebyshev distance around the
+      // query's own top cell, so the nearest points come first and the
+      // limit shrinks before the far cells are tested (each cell of the
+      // range is on exactly one ring; the result does not depend on order).
+      const int qt0 = qc0 >> T, qt1 = qc1 >> T, qt2 = qc2 >> T;
+      const int nring = max(max(max(qt0 - tl0, th0 - qt0), max(qt1 - tl1, th1 - qt1)),
+                            max(qt2 - tl2, th2 - qt2));
+      for (int ring = 0; ring <= nring; ++ring)
+      for (int tz = max(tl2, qt2 - ring); tz <= min(th2, qt2 + ring); ++tz)
+        for (int ty = max(tl1, qt1 - ring); ty <= min(th1, qt1 + ring); ++ty)
+          for (int tx = max(tl0, qt0 - ring); tx <= min(th0, qt0 + ring); ++tx) {
+            if (max(max(abs(tx - qt0), abs(ty - qt1)), abs(tz - qt2)) != ring) continue;
             int l = T;
             uint32_t m = morton3((uint32_t)tx, (uint32_t)ty, (uint32_t)tz);
             for (;;) {
@@ -341,10 +351,10 @@ __global__ __launch_bounds__(kTileQ) void k_topk_tree(const SearchArgs a) {
                   if (l > L0 && e - s0 > kTreeLeaf) {
                     down = true;
                   } else {
-                    // A leaf beyond kTreeLeaf points (a finest-level cell)
-                    // goes chunk by chunk: a chunk whose box lies beyond the
-                    // limit (with the cells' FP32 margin) holds no hit.
-                    const bool chunked = a.chunk_box && e - s0 > kTreeLeaf;
+                    // A leaf of more than one chunk goes chunk by chunk: a
+                    // chunk whose box lies beyond the limit (with the cells'
+                    // FP32 margin) holds no hit.
+                    const bool chunked = a.chunk_box && e - s0 > kChunkPts;
                     for (uint32_t c0 = s0; c0 < e;) {
                       uint32_t c1 = e;
                       if (chunked) {
