@@ -391,6 +391,10 @@ def main():
         algo_def = "range bytes per query 12 + 12*C27(q) + 4*K"
     achieved = algo / (kms * 1e-3) / 1e9
     kname = "k_topk_thread" if results[kroof]["lanes_per_query"] == 1 else "k_topk_group"
+    levels = (fixed_opts or w.get("options") or results[kroof].get("autotuned") or {}).get(
+        "grid_levels", 0)
+    if kname == "k_topk_thread" and mode == "knn" and levels:
+        kname = "k_topk_tree"  # one lane per query on a hierarchical grid: the octree walk
     traffic = pmc_traffic(args.workload, kname)
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "kernel": kname,

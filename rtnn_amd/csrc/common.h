@@ -82,7 +82,7 @@ struct SearchArgs {
   const float4* chunk_box;
 };
 
-constexpr uint32_t kChunkPts = 64;
+constexpr uint32_t kChunkPts = 32;
 
 constexpr int kTileQ = 256;  // queries per tile (= threads of a KNN block)
 

@@ -255,14 +255,17 @@ __global__ __launch_bounds__(kTileQ) void k_topk_thread(const SearchArgs a) {
 // query first (child j visits octant j ^ the query's octant), without a
 // stack: the position among siblings is recovered from the key.  A cell
 // whose padded box lies farther than the current limit is skipped; a cell of
-// the finest built level, or holding at most kTreeLeaf points, is scanned
-// (64: the full-size Levy-flight KNN ran 17.8 s per step with 16, 16.5 with
-// 64, 16.0 with 256).
+// the finest built level, or holding at most kTreeLeaf points, is scanned,
+// chunk by chunk: the bounding box of every kChunkPts sorted points skips the
+// chunks beyond the limit (on the full-size Levy-flight KNN, learned radius,
+// 7.35 s per step with leaves of 64 points, 7.16 with 128, 7.06 with 256).
+// Top-level cells are taken in rings of growing distance around the query's
+// own, so the limit shrinks early.
 // The limit starts at the float below R^2 and, once k keys are held, drops
 // to the k-th key's d2, so a dense cluster beyond the k nearest is cut off
 // at its coarsest cell.  The result is the same as the cube walk's: only
 // cells that cannot hold a hit (d2 <= limit) are skipped.
-constexpr uint32_t kTreeLeaf = 64;
+constexpr uint32_t kTreeLeaf = 256;
 
 __device__ inline uint32_t compact3(uint32_t x) {
   x &= 0x09249249u;
