@@ -100,7 +100,8 @@ def workload(name):
                     autotune=False,  # each candidate search at r = 100 takes seconds
                     # the octree walk of a 7-level hierarchical grid (fixed
                     # configuration, not tuned)
-                    options={"grid_levels": 7, "queries_per_wave": 0},
+                    # leaves of up to 65536 points, scanned chunk by chunk
+                    options={"grid_levels": 7, "queries_per_wave": 0, "tree_leaf": 65536},
                     cfg={"workload": "KNN K=32 r=100: 30M Levy-flight points (Pareto alpha 1.5, "
                                      "min step 0.01, periodic [0,1000]^3), 30M uniform queries",
                          "points": 30_000_000, "queries": 30_000_000, "k": 32, "radius": 100.0})

@@ -70,6 +70,11 @@ typedef struct {
                              wave stages the points of its queries' union block in
                              LDS (up to that many per wave) and scans them there
                              (1 = 512; 0 = off); implies compact buffers        */
+  int   tree_leaf;        /* hierarchical-grid KNN, one lane per query (the octree
+                             walk): a cell of at most this many points is scanned
+                             (chunk by chunk, 32 points per bounding box) instead
+                             of descended into; 0 = 256.  Large values suit very
+                             clustered points (Levy flights: 65536)            */
 } rtnn_options;
 
 /* Filled by search entry points when a non-NULL pointer is passed. */

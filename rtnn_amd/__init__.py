@@ -51,13 +51,15 @@ class Options(ctypes.Structure):
                 ("grid_levels", ctypes.c_int),
                 ("level_alpha", ctypes.c_float),
                 ("group_buffer", ctypes.c_int),
-                ("lds_tile", ctypes.c_int)]
+                ("lds_tile", ctypes.c_int),
+                ("tree_leaf", ctypes.c_int)]
 
 
 def _options_struct(cell_factor=0.0, queries_per_wave=0, query_order=0, time_stages=0,
-                    max_cells=0, grid_levels=0, level_alpha=0.0, group_buffer=0, lds_tile=0):
+                    max_cells=0, grid_levels=0, level_alpha=0.0, group_buffer=0, lds_tile=0,
+                    tree_leaf=0):
     return Options(cell_factor, queries_per_wave, query_order, time_stages, max_cells,
-                   grid_levels, level_alpha, group_buffer, lds_tile)
+                   grid_levels, level_alpha, group_buffer, lds_tile, tree_leaf)
 
 
 def _options_dict(o):
@@ -247,9 +249,9 @@ class Context:
 
     def set_options(self, cell_factor=0.0, queries_per_wave=0, query_order=0,
                     time_stages=0, max_cells=0, grid_levels=0, level_alpha=0.0,
-                    group_buffer=0, lds_tile=0):
+                    group_buffer=0, lds_tile=0, tree_leaf=0):
         o = Options(cell_factor, queries_per_wave, query_order, time_stages, max_cells,
-                    grid_levels, level_alpha, group_buffer, lds_tile)
+                    grid_levels, level_alpha, group_buffer, lds_tile, tree_leaf)
         self._check(self._L.rtnn_set_options(self._h, ctypes.byref(o)))
 
     def synchronize(self):
@@ -516,7 +518,8 @@ class Context:
                                           ctypes.byref(ms)))
         return ({"cell_factor": best.cell_factor, "queries_per_wave": best.queries_per_wave,
                  "grid_levels": best.grid_levels, "level_alpha": best.level_alpha,
-                 "group_buffer": best.group_buffer, "lds_tile": best.lds_tile}, ms.value)
+                 "group_buffer": best.group_buffer, "lds_tile": best.lds_tile,
+                 "tree_leaf": best.tree_leaf}, ms.value)
 
     def autotune_batched(self, xyz, offsets, mode="radius", radius=0.2, k=64):
         """Times the batched-search configurations on this batch and keeps the

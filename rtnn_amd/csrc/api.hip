@@ -553,6 +553,7 @@ SearchArgs base_args(rtnn_ctx ctx, const float4* qproc, uint64_t nq, float r, ui
     a.level_min = ctx->hlevel_min;
     a.level_alpha = ctx->opt.level_alpha > 0 ? ctx->opt.level_alpha : 1.0f;
     a.chunk_box = ctx->cboxes.as<float4>();
+    a.tree_leaf = ctx->opt.tree_leaf > 0 ? (uint32_t)ctx->opt.tree_leaf : kTreeLeafDefault;
   }
   return a;
 }
@@ -708,6 +709,7 @@ int check_options(rtnn_ctx ctx, const rtnn_options* opt) {
     return fail(ctx, RTNN_ERR_INVALID, "lds_tile must be 0, 1 (= 512), 256, 512 or 1024");
   if (!(opt->level_alpha >= 0.0f) || !std::isfinite(opt->level_alpha))
     return fail(ctx, RTNN_ERR_INVALID, "level_alpha must be >= 0");
+  if (opt->tree_leaf < 0) return fail(ctx, RTNN_ERR_INVALID, "tree_leaf must be >= 0");
   return RTNN_OK;
 }
 
@@ -1735,24 +1737,28 @@ int rtnn_autotune(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_device, i
     float cf;
     int qpw, levels;
     float alpha;
-    int buffer, tile;
+    int buffer, tile, leaf;
   };
   std::vector<Cand> cands;
   for (int i = 0; i < ncf; ++i)
     for (int j = 0; j < nqpw; ++j) {
       const bool group = qpws[j] > 0 && qpws[j] < 64;
-      cands.push_back({cfs[i], qpws[j], plain ? 0 : 1, 0.0f, group ? 2 : 0, 0});
+      cands.push_back({cfs[i], qpws[j], plain ? 0 : 1, 0.0f, group ? 2 : 0, 0, 0});
       if (group) {
-        cands.push_back({cfs[i], qpws[j], plain ? 0 : 1, 0.0f, 1, 0});
+        cands.push_back({cfs[i], qpws[j], plain ? 0 : 1, 0.0f, 1, 0, 0});
         if (qpws[j] != 32)  // 2-lane groups have no tiled kernel
           for (int tp : {256, 512, 1024})
-            cands.push_back({cfs[i], qpws[j], plain ? 0 : 1, 0.0f, 1, tp});
+            cands.push_back({cfs[i], qpws[j], plain ? 0 : 1, 0.0f, 1, tp, 0});
       }
     }
+  // Hierarchical grids: the octree walk (one lane per query) with leaves of
+  // 256 or 65536 points, and 4-lane groups on the level rule.
   if (!plain)
     for (int lv : {3, 5})
-      for (float al : {1.0f, 0.5f})
-        for (int qpw : {0, 16}) cands.push_back({0.0f, qpw, lv, al, 0, 0});
+      for (float al : {1.0f, 0.5f}) {
+        for (int leaf : {0, 65536}) cands.push_back({0.0f, 0, lv, al, 0, 0, leaf});
+        cands.push_back({0.0f, 16, lv, al, 0, 0, 0});
+      }
   for (const Cand& cand : cands) {
     {
       const int qpw = cand.qpw;
@@ -1765,6 +1771,7 @@ int rtnn_autotune(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_device, i
       ctx->opt.level_alpha = cand.alpha;
       ctx->opt.group_buffer = cand.buffer;
       ctx->opt.lds_tile = cand.tile;
+      ctx->opt.tree_leaf = cand.leaf;
       double ms = 0.0;
       const int rc = time_candidate(
           [&](rtnn_stats* st) {

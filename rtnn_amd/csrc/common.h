@@ -80,7 +80,10 @@ struct SearchArgs {
   // (chunk c: min at [2c], max at [2c+1]); the octree walk skips the chunks of
   // a large leaf that lie beyond its limit.
   const float4* chunk_box;
+  uint32_t tree_leaf;  // octree walk: cells of at most this many points are leaves
 };
+
+constexpr uint32_t kTreeLeafDefault = 256;
 
 constexpr uint32_t kChunkPts = 32;
 

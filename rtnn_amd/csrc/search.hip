@@ -255,17 +255,18 @@ __global__ __launch_bounds__(kTileQ) void k_topk_thread(const SearchArgs a) {
 // query first (child j visits octant j ^ the query's octant), without a
 // stack: the position among siblings is recovered from the key.  A cell
 // whose padded box lies farther than the current limit is skipped; a cell of
-// the finest built level, or holding at most kTreeLeaf points, is scanned,
+// the finest built level, or holding at most a.tree_leaf points, is scanned,
 // chunk by chunk: the bounding box of every kChunkPts sorted points skips the
-// chunks beyond the limit (on the full-size Levy-flight KNN, learned radius,
-// 7.35 s per step with leaves of 64 points, 7.16 with 128, 7.06 with 256).
+// chunks beyond the limit.  The leaf size is an option: on the full-size
+// Levy-flight KNN (learned radius) a step took 6.9 s with leaves of up to 256
+// points, 4.1 s with 4096, 1.3 s with 65536; the Gaussian-mixture KNN (1M
+// points) is fastest at 256 (2.9 ms, 3.5 ms with 65536).
 // Top-level cells are taken in rings of growing distance around the query's
 // own, so the limit shrinks early.
 // The limit starts at the float below R^2 and, once k keys are held, drops
 // to the k-th key's d2, so a dense cluster beyond the k nearest is cut off
 // at its coarsest cell.  The result is the same as the cube walk's: only
 // cells that cannot hold a hit (d2 <= limit) are skipped.
-constexpr uint32_t kTreeLeaf = 256;
 
 __device__ inline uint32_t compact3(uint32_t x) {
   x &= 0x09249249u;
@@ -351,7 +352,7 @@ __global__ __launch_bounds__(kTileQ) void k_topk_tree(const SearchArgs a) {
                 const uint32_t cb = s_base[l] + m;
                 const uint32_t s0 = a.cell_start[cb], e = a.cell_start[cb + 1];
                 if (e > s0) {
-                  if (l > L0 && e - s0 > kTreeLeaf) {
+                  if (l > L0 && e - s0 > a.tree_leaf) {
                     down = true;
                   } else {
                     // A leaf of more than one chunk goes chunk by chunk: a

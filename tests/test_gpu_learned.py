@@ -157,6 +157,27 @@ def test_hierarchical_grid_exact(ctx, levels, alpha, qpw):
         ctx.set_options()
 
 
+@pytest.mark.parametrize("leaf", [1, 64, 1 << 20])
+def test_octree_leaf_sizes_exact(ctx, leaf):
+    # The octree walk with tiny leaves (descend to the finest level), default
+    # and huge ones (a top-level cell scanned chunk by chunk) equals the plain
+    # KNN, also on Levy-flight clusters.
+    pts = rtnn_amd.gen_gaussian_mixture(2, 20, 100000)
+    qs = rtnn_amd.gen_gaussian_mixture(2, 21, 20000)
+    ctx.set_points(pts)
+    ctx.set_options(grid_levels=5, level_alpha=1.0, queries_per_wave=0, tree_leaf=leaf)
+    try:
+        for k in (1, 8, 32):
+            _plain_vs(ctx, qs, 20.0, k, ctx.search_partitioned)
+        lv = rtnn_amd.gen_levy(5, 300000, alpha=1.5, min_step=0.01, box=100.0)
+        lq = rtnn_amd.gen_uniform(51, 20000, 0.0, 100.0)
+        ctx.set_points(lv)
+        ctx.set_options(grid_levels=7, queries_per_wave=0, tree_leaf=leaf)
+        _plain_vs(ctx, lq, 10.0, 32, ctx.search_partitioned)
+    finally:
+        ctx.set_options()
+
+
 def test_autotune_picks_a_valid_config(ctx):
     rng = np.random.default_rng(21)
     pts = rng.uniform(0, 40, (50000, 3)).astype(np.float32)
