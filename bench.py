@@ -325,6 +325,8 @@ def main():
                              for kk, v in agg.items()})
         if tuned:
             results[m]["autotuned"] = tuned
+        elif fixed_opts or w.get("options"):
+            results[m]["options"] = dict(fixed_opts or w.get("options"))
         if m == "graph":
             results[m]["captured_with"] = graph_cfg
         outputs[m] = (out[0], out[1])
