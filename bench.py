@@ -214,6 +214,8 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    # one stderr line per autotuner candidate (a long sweep shows progress)
+    os.environ.setdefault("RTNN_AUTOTUNE_LOG", "1")
     import torch
     import rtnn_amd
 
@@ -412,7 +414,11 @@ def main():
     # exactness on a sample + CPU baseline (oracle brute force on the sample)
     import oracle
     om = oracle.MODE_KNN if mode == "knn" else oracle.MODE_RADIUS
-    cores = min(16, os.cpu_count() or 1)
+    # Host cores: every core this process may run on, up to the 16 a one-GPU
+    # box shares out to a GPU's process (nproc there counts the whole host).
+    nproc = os.cpu_count() or 1
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else nproc
+    cores = max(1, min(16, avail))
     cpu, mism = None, None
     if not args.no_cpu:
         rng = np.random.default_rng(1234)
@@ -452,7 +458,7 @@ def main():
             sample = (f"{ns} random queries of this workload, brute force over all "
                       f"{len(pts)} points ({dt:.1f} s)")
         cpu = {"value": round(ns / dt, 3), "unit": "queries/s", "cores": cores, "kind": "port",
-               "sample": sample}
+               "sample": sample, "host_nproc": nproc, "host_cpus_available": avail}
         # the reference's check path runs on one thread: the same brute force
         # on one core over a subsample of about the same CPU time
         t = time.perf_counter()

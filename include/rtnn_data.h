@@ -16,7 +16,8 @@ extern "C" {
 /* Reads a point file: one point per line, "x,y,z", parsed as doubles and
  * rounded to float (util.cpp:101-135: getline into a 1024-byte buffer, then
  * sscanf("%lf,%lf,%lf")).  Two calls: with xyz == NULL it only counts lines
- * into *n; then with a buffer of 3 * *n floats it fills it.  Differences from
+ * into *n; then with a buffer of 3 * *n floats (and *n as returned) it fills
+ * it, returning -4 if the file now holds more points than *n.  Differences from
  * the reference, on purpose: a line with fewer than three numbers is an error
  * (-1, *n = 1-based line number) instead of leaving uninitialised values,
  * blank lines are skipped, and an unreadable file is an error (-2) instead of

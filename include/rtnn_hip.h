@@ -91,6 +91,7 @@ typedef struct {
   uint32_t sort_passes;   /* LSD radix passes of the point sort                     */
   uint32_t search_launches;
   uint32_t lanes_per_query; /* range kernel group width actually used               */
+  double ms_copy;         /* result copy to host arrays (host outputs only)          */
 } rtnn_stats;
 
 /* ---- context ---------------------------------------------------------- */
@@ -255,7 +256,10 @@ int rtnn_search_percloud(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets
  * data.  The cell table is sized from the data at capture (+25 %); a replay
  * on data that needs more cells uses a coarser edge (still exact).  Replays
  * run on the context's stream.  rtnn_destroy also destroys the context's
- * remaining graphs (their handles must not be used afterwards). */
+ * remaining graphs.  The library keeps the set of live graph handles:
+ * rtnn_graph_launch and rtnn_graph_destroy on a handle already freed (by
+ * rtnn_graph_destroy or with its context) return RTNN_ERR_INVALID without
+ * touching it, so graph and context may be destroyed in either order. */
 typedef struct rtnn_graph_s* rtnn_graph;
 int rtnn_graph_batched(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets,
                        uint32_t nclouds, int mode, float r, uint32_t k,

@@ -78,7 +78,8 @@ class Stats(ctypes.Structure):
                 ("cell_size", ctypes.c_float),
                 ("sort_passes", ctypes.c_uint32),
                 ("search_launches", ctypes.c_uint32),
-                ("lanes_per_query", ctypes.c_uint32)]
+                ("lanes_per_query", ctypes.c_uint32),
+                ("ms_copy", ctypes.c_double)]
 
     def as_dict(self):
         d = {f: getattr(self, f) for f, _ in self._fields_}
@@ -621,9 +622,11 @@ def read_points(path):
     rc = L.rtnn_read_points(path.encode(), ctypes.c_void_p(out.ctypes.data), ctypes.byref(n))
     if rc == -1:
         raise ValueError(f"{path}: line {n.value} does not hold three numbers")
+    if rc == -4:
+        raise ValueError(f"{path} grew while it was read")
     if rc != 0:
         raise ValueError(f"cannot read {path} (code {rc})")
-    return out
+    return out[:n.value]
 
 
 def mem_stats():

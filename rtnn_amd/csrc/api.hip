@@ -16,6 +16,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -44,8 +46,21 @@ void bump_peak(std::atomic<uint64_t>& peak, uint64_t v) {
 }
 }  // namespace
 
+// Simulated allocation failure, forced from the host for tests of the OOM
+// paths: with RTNN_FAIL_ALLOC_BYTES=b in the environment every device or
+// pinned allocation of at least b bytes fails as out of memory without
+// reaching the runtime.  Read at each allocation, so a caller may set and
+// clear it between calls.
+static bool forced_alloc_failure(size_t bytes) {
+  const char* v = std::getenv("RTNN_FAIL_ALLOC_BYTES");
+  if (!v || !*v) return false;
+  char* end = nullptr;
+  const unsigned long long b = std::strtoull(v, &end, 10);
+  return end != v && bytes >= b;
+}
+
 hipError_t dev_alloc(void** p, size_t bytes) {
-  const hipError_t e = hipMalloc(p, bytes);
+  const hipError_t e = forced_alloc_failure(bytes) ? hipErrorOutOfMemory : hipMalloc(p, bytes);
   if (e != hipSuccess) {
     if (memstat_log())
       std::fprintf(stderr, "[MEM_STATS] hipMalloc of %.3f MB failed (%s), in use %.3f MB\n",
@@ -67,7 +82,8 @@ void dev_free(void* p, size_t bytes) {
                  bytes / 1048576.0, cur / 1048576.0);
 }
 hipError_t pinned_alloc(void** p, size_t bytes) {
-  const hipError_t e = hipHostMalloc(p, bytes, hipHostMallocDefault);
+  const hipError_t e = forced_alloc_failure(bytes) ? hipErrorOutOfMemory
+                                                   : hipHostMalloc(p, bytes, hipHostMallocDefault);
   if (e != hipSuccess) return e;
   bump_peak(g_pin_peak, g_pin_cur += bytes);
   return e;
@@ -162,7 +178,7 @@ struct rtnn_ctx_s {
 
   // grid
   DevBuf keys, vals, counts, cell_start, spts, tmp, grids;
-  DevBuf cboxes;  // hierarchical grids: bounding box per 64 sorted points
+  DevBuf cboxes;  // hierarchical grids: bounding box per kChunkPts sorted points
   GridDesc hgrid{};
   uint64_t ncells = 0;
   int sort_passes = 0;
@@ -796,6 +812,7 @@ static int search_impl(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_devi
     stats->ms_grid = ev_ms(ctx->ev[0], ctx->ev[1]);
     stats->ms_qsort = ev_ms(ctx->ev[1], ctx->ev[2]);
     stats->ms_search = ev_ms(ctx->ev[2], ctx->ev[3]);
+    stats->ms_copy = out_on_device ? 0.0 : ev_ms(ctx->ev[3], ctx->ev[4]);
     stats->ms_total = ev_ms(ctx->ev[0], ctx->ev[4]);
     stats->search_launches = 1;
     stats->lanes_per_query = (uint32_t)gl;
@@ -876,8 +893,15 @@ int compute_proposals(rtnn_ctx ctx, Proposal kind, const float* qxyz, uint64_t n
                                            : (kind == Proposal::kLearned ? 2.0f : 4.0f);
   const int levels = ctx->opt.grid_levels > 0 ? ctx->opt.grid_levels
                                               : (kind == Proposal::kLearned ? 1 : 5);
-  const bool hier = levels >= 2;
+  bool hier = levels >= 2;
   int rc = hier ? build_hgrid(ctx, r, levels) : build_grid(ctx, r / f);
+  if (rc == RTNN_ERR_UNSUPPORTED && hier) {
+    // The Morton grid needs the box within 1024 top-level cells of edge r per
+    // axis; a wider box (or a smaller r) takes the raster grid, which grows
+    // its cell edge to fit the cell budget.
+    hier = false;
+    rc = build_grid(ctx, r / f);
+  }
   if (rc) return rc;
   const float4* qproc = nullptr;
   rc = prepare_queries(ctx, qxyz, nq, on_device, &qproc);
@@ -930,7 +954,7 @@ int proposal_search(rtnn_ctx ctx, Proposal kind, const float* qxyz, uint64_t nq,
   if (ctx->n == 0) return fail(ctx, RTNN_ERR_STATE, "rtnn_set_points has not been called");
   if (kind == Proposal::kLearned && !ctx->mlp.ready)
     return fail(ctx, RTNN_ERR_STATE, "the radius MLP has no weights (train or load it)");
-  if (k > 64) return fail(ctx, RTNN_ERR_UNSUPPORTED, "proposal search supports k <= 64");
+  if (k > 1024) return fail(ctx, RTNN_ERR_UNSUPPORTED, "k=%u exceeds the supported maximum (1024)", k);
   if (!qxyz) nq = ctx->n;
   if (nq == 0) return RTNN_OK;
   if (nq >= 0xffffffffull) return fail(ctx, RTNN_ERR_UNSUPPORTED, "more than 2^32-2 queries");
@@ -983,8 +1007,10 @@ int proposal_search(rtnn_ctx ctx, Proposal kind, const float* qxyz, uint64_t nq,
     }
     // On the hierarchical grid the octree walk (one lane per query) prunes
     // by its shrinking limit, which pays most for these wide searches.
-    CK(launch_search(s, 1, ctx->hlevels > 0 ? 1 : kFallbackLanes, ctx->st, ctx->num_cus, true,
-                     false, true));
+    // Beyond k = 64 (the register top-k and the 4-lane buffers) the rows go
+    // to 16- or 64-lane groups.
+    const int fl = k > 256 ? 64 : k > 64 ? 16 : ctx->hlevels > 0 ? 1 : kFallbackLanes;
+    CK(launch_search(s, 1, fl, ctx->st, ctx->num_cus, true, false, true));
   }
   CK(hipEventRecord(ctx->ev[4], ctx->st));
   rc = copy_out(ctx, nq, k, out_idx, out_d2, out_on_device, o);
@@ -999,6 +1025,7 @@ int proposal_search(rtnn_ctx ctx, Proposal kind, const float* qxyz, uint64_t nq,
     stats->ms_mlp = ev_ms(ctx->ev[1], ctx->ev[2]);
     stats->ms_search = ev_ms(ctx->ev[2], ctx->ev[3]);
     stats->ms_fallback = ev_ms(ctx->ev[3], ctx->ev[4]);
+    stats->ms_copy = out_on_device ? 0.0 : ev_ms(ctx->ev[4], ctx->ev[5]);
     stats->ms_total = ev_ms(ctx->ev[0], ctx->ev[5]);
     stats->fallback_queries = nshort;
     stats->search_launches = 2 + kGrowStages;
@@ -1090,11 +1117,14 @@ int rtnn_search_stream(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int mode, i
       CK(hipEventCreateWithFlags(&ctx->ev_down[i], hipEventDisableTiming));
     }
   }
+  // The result file is closed on every return (an early HIP error included).
+  std::unique_ptr<FILE, int (*)(FILE*)> fguard(nullptr, &std::fclose);
   FILE* f = nullptr;
   const uint64_t head = 8 + 8 + 4 + 4;
   if (out_path) {
     f = std::fopen(out_path, "wb");
     if (!f) return fail(ctx, RTNN_ERR_IO, "cannot open %s for writing", out_path);
+    fguard.reset(f);
     const uint32_t flags = 1;  // d2 present
     std::fwrite("RTNNRES1", 1, 8, f);
     std::fwrite(&nq, 8, 1, f);
@@ -1108,7 +1138,6 @@ int rtnn_search_stream(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int mode, i
     if (ctx->sq[i].ensure(12 * chunk) != hipSuccess || ctx->sidx[i].ensure(rowb * chunk) ||
         (want_d2 && ctx->sd2[i].ensure(rowb * chunk)) || ctx->pq[i].ensure(12 * chunk) ||
         ctx->pidx[i].ensure(rowb * chunk) || (want_d2 && ctx->pd2[i].ensure(rowb * chunk))) {
-      if (f) std::fclose(f);
       return fail(ctx, RTNN_ERR_OOM, "streaming buffers of %llu queries",
                   (unsigned long long)chunk);
     }
@@ -1174,7 +1203,7 @@ int rtnn_search_stream(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int mode, i
   for (uint64_t c = nchunks >= 2 ? nchunks - 2 : 0; c < nchunks && rc == RTNN_OK; ++c)
     CK(flush(c));
   CK(hipStreamSynchronize(ctx->st3));
-  if (f && std::fclose(f) != 0) io_err = true;
+  if (f && std::fclose(fguard.release()) != 0) io_err = true;
   if (rc) return rc;
   if (io_err) return fail(ctx, RTNN_ERR_IO, "writing %s failed", out_path);
   if (stats) {
@@ -1570,6 +1599,26 @@ int check_boundary_args(rtnn_ctx ctx, float r, double tol, const void* out_pairs
 
 namespace {
 
+// With RTNN_AUTOTUNE_LOG set in the environment the autotuners print one
+// stderr line per candidate as it is timed (a sweep over a large workload
+// then shows its progress).  Read at each call, so a caller may set it late.
+void tune_log(const char* fmt, ...) {
+  const char* v = std::getenv("RTNN_AUTOTUNE_LOG");
+  if (!v || !*v || std::strcmp(v, "0") == 0) return;
+  va_list ap;
+  va_start(ap, fmt);
+  std::fprintf(stderr, "[rtnn] ");
+  std::vfprintf(stderr, fmt, ap);
+  std::fprintf(stderr, "\n");
+  std::fflush(stderr);
+  va_end(ap);
+}
+
+const char* method_name(int method) {
+  return method == RTNN_METHOD_LEARNED ? "learned"
+         : method == RTNN_METHOD_PARTITIONED ? "partitioned" : "plain";
+}
+
 // Times one autotuner candidate: a first run (sizes buffers, loads code),
 // then enough runs for about 4 ms of work (1 to 8); the fastest counts, so a
 // short workload's choice is not decided by one noisy sample.
@@ -1782,15 +1831,28 @@ int rtnn_autotune(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_device, i
                                    qxyz, nq, on_device, r, k, out_idx, out_d2, 1, st);
           },
           ms);
+      if (rc == RTNN_ERR_UNSUPPORTED) {  // a grid or kernel this workload cannot use
+        tune_log("autotune %s: candidate %zu/%zu unsupported (%s)", method_name(method),
+                 (size_t)(&cand - cands.data()) + 1, cands.size(), ctx->err.c_str());
+        continue;
+      }
       if (rc != RTNN_OK) {
         ctx->opt = base;
         return rc;
       }
+      tune_log("autotune %s: candidate %zu/%zu cell_factor %g qpw %d levels %d alpha %g "
+               "buffer %d tile %d leaf %d: %.4f ms", method_name(method),
+               (size_t)(&cand - cands.data()) + 1, cands.size(), cand.cf, cand.qpw, cand.levels,
+               cand.alpha, cand.buffer, cand.tile, cand.leaf, ms);
       if (ms < best_t) {
         best_t = ms;
         best_opt = ctx->opt;
       }
     }
+  }
+  if (best_t == 1e300) {
+    ctx->opt = base;
+    return fail(ctx, RTNN_ERR_UNSUPPORTED, "no configuration supports this workload");
   }
   ctx->opt = best_opt;
   if (method == RTNN_METHOD_LEARNED) {
@@ -1811,6 +1873,7 @@ int rtnn_autotune(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_device, i
         ctx->mlp.margin = m0;
         return rc;
       }
+      tune_log("autotune learned: radius margin %g: %.4f ms", ctx->mlp.margin, ms);
       if (ms < best_t) {
         best_t = ms;
         best_m = ctx->mlp.margin;
@@ -1859,6 +1922,7 @@ int rtnn_autotune_batched(rtnn_ctx ctx, const float* xyz, const uint64_t* offset
           ctx->opt = base;
           return rc;
         }
+        tune_log("autotune batched: cell_factor %g qpw %d lds %d: %.4f ms", cf, qpw, lds, ms);
         if (ms < best_t) {
           best_t = ms;
           best_opt = ctx->opt;
@@ -1999,7 +2063,23 @@ struct rtnn_graph_s {
   }
 };
 
+// Live graph handles of the process.  rtnn_destroy frees a context's graphs,
+// so a handle may outlive its graph; every graph call checks the handle here
+// first and rejects a freed one (RTNN_ERR_INVALID) instead of touching it.
+namespace {
+std::mutex g_graphs_mu;
+std::set<rtnn_graph_s*> g_live_graphs;
+bool graph_live(rtnn_graph_s* g) {
+  std::lock_guard<std::mutex> lk(g_graphs_mu);
+  return g_live_graphs.count(g) != 0;
+}
+}  // namespace
+
 static void destroy_graphs(rtnn_ctx ctx) {
+  {
+    std::lock_guard<std::mutex> lk(g_graphs_mu);
+    for (rtnn_graph_s* g : ctx->graphs) g_live_graphs.erase(g);
+  }
   for (rtnn_graph_s* g : ctx->graphs) delete g;
   ctx->graphs.clear();
 }
@@ -2122,11 +2202,15 @@ int rtnn_graph_batched(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets,
   CK(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
   *out = g.release();
   ctx->graphs.push_back(*out);
+  {
+    std::lock_guard<std::mutex> lk(g_graphs_mu);
+    g_live_graphs.insert(*out);
+  }
   return RTNN_OK;
 }
 
 int rtnn_graph_launch(rtnn_graph g, int sync, rtnn_stats* stats) {
-  if (!g) return RTNN_ERR_INVALID;
+  if (!g || !graph_live(g)) return RTNN_ERR_INVALID;
   rtnn_ctx ctx = g->ctx;
   CK(hipSetDevice(ctx->device));
   if (stats) CK(hipEventRecord(ctx->ev[6], ctx->st));
@@ -2148,6 +2232,11 @@ int rtnn_graph_launch(rtnn_graph g, int sync, rtnn_stats* stats) {
 
 int rtnn_graph_destroy(rtnn_graph g) {
   if (!g) return RTNN_OK;
+  {
+    // Already freed (by an earlier destroy, or with its context): rejected.
+    std::lock_guard<std::mutex> lk(g_graphs_mu);
+    if (!g_live_graphs.erase(g)) return RTNN_ERR_INVALID;
+  }
   rtnn_ctx ctx = g->ctx;
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->st);

@@ -64,12 +64,14 @@ int rtnn_read_points(const char* path, float* xyz, uint64_t* n) {
   // then each is parsed as three comma-separated doubles.
   std::vector<char> line(1 << 16);
   uint64_t count = 0;
+  const uint64_t cap = *n;  // second call: the buffer holds cap points
   int rc = 0;
   while (std::fgets(line.data(), (int)line.size(), f)) {
     const char* c = line.data();
     while (*c == ' ' || *c == '\t' || *c == '\r' || *c == '\n') ++c;
     if (*c == '\0') continue;  // blank line: skipped (the reference would parse garbage)
     if (xyz) {
+      if (count >= cap) { rc = -4; break; }  // the file grew since the counting call
       double v[3];
       char* p = line.data();
       int got = 0;

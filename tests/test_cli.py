@@ -121,3 +121,29 @@ def test_cli_filter_queries_reports_far_queries(tmp_path):
     assert "Filter queries: 102 (" in r.stdout
     idx, _ = read_result(out)
     assert (idx[300:] == -1).all() and (idx[:300, 0] >= 0).all()
+
+
+@pytest.mark.gpu
+def test_cli_knn_default_path_on_a_wide_box(tmp_path):
+    # The default KNN path (-p 1: the density partition, hierarchical Morton
+    # grid) on a box 2000 r long, wider than the Morton grid's 1024 top-level
+    # cells per axis: it falls back to the raster grid and stays exact.  The
+    # reference's own printed scopes appear.
+    import oracle
+    rng = np.random.default_rng(6)
+    p = np.concatenate([rng.uniform([0, 0, 0], [2000, 2, 2], (6000, 3)),
+                        rng.normal([500, 1, 1], 0.2, (800, 3))]).astype(np.float32)
+    q = rng.uniform([0, 0, 0], [2000, 2, 2], (900, 3)).astype(np.float32)
+    pf, qf, out = tmp_path / "p.txt", tmp_path / "q.txt", str(tmp_path / "o.bin")
+    np.savetxt(pf, p, fmt="%.9g", delimiter=",")
+    np.savetxt(qf, q, fmt="%.9g", delimiter=",")
+    p = np.loadtxt(pf, delimiter=",", dtype=np.float64).astype(np.float32)
+    q = np.loadtxt(qf, delimiter=",", dtype=np.float64).astype(np.float32)
+    r = run("-f", str(pf), "-q", str(qf), "-sm", "knn", "-r", "1", "-c", "1", "-o", out)
+    assert r.returncode == 0, r.stderr
+    for scope in ("sort points", "sort and/or partition queries", "search compute",
+                  "result copy D2H", "total search time"):
+        assert f"time {scope}: " in r.stdout, scope
+    idx, d2 = read_result(out)
+    wi, wd = oracle.bruteforce(p, q, oracle.MODE_KNN, 1.0, idx.shape[1], threads=4)
+    assert np.array_equal(idx, wi) and np.array_equal(d2, wd)

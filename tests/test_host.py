@@ -53,6 +53,29 @@ def test_read_points_matches_reference_parse(tmp_path):
     assert rtnn_amd.read_points(str(empty)).shape == (0, 3)
 
 
+def test_read_points_file_grown_between_calls(tmp_path):
+    # The counting call sizes the caller's buffer; a file that grows before
+    # the filling call must not write past that buffer (-4 instead).
+    import ctypes
+    L = rtnn_amd.lib()
+    p = tmp_path / "grow.txt"
+    p.write_text("1,2,3\n4,5,6\n")
+    n = ctypes.c_uint64(0)
+    assert L.rtnn_read_points(str(p).encode(), None, ctypes.byref(n)) == 0 and n.value == 2
+    with open(p, "a") as f:
+        f.write("7,8,9\n10,11,12\n")
+    buf = np.full(3 * 2 + 16, -7.0, dtype=np.float32)  # 2 points + a guard
+    rc = L.rtnn_read_points(str(p).encode(), ctypes.c_void_p(buf.ctypes.data), ctypes.byref(n))
+    assert rc == -4
+    assert buf[:6].tolist() == [1, 2, 3, 4, 5, 6]
+    assert (buf[6:] == -7.0).all()
+    # a file that shrank: the second call reports the points it filled
+    p.write_text("1,2,3\n")
+    n = ctypes.c_uint64(2)
+    assert L.rtnn_read_points(str(p).encode(), ctypes.c_void_p(buf.ctypes.data),
+                              ctypes.byref(n)) == 0 and n.value == 1
+
+
 def test_read_point_sets(tmp_path):
     # Each line: 3 * m numbers = point i of m sets; every number parsed
     # straight to float (stof), unlike the single-set reader's double.
