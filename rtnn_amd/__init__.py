@@ -166,6 +166,14 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"{LIB_PATH} is missing: build it with rtnn_amd/build.py")
+    # torch bundles a HIP runtime of its own under the same soname
+    # (libamdhip64.so.7).  Loaded first, it is the one the library binds to, so
+    # the process holds one runtime; loaded after the system one, torch gets a
+    # second runtime and the two cannot both open the GPU.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(L, name)

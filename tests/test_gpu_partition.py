@@ -134,8 +134,9 @@ def _queries(pts, seed):
     return np.concatenate([interior, uniform, corners, far]).astype(F)
 
 
-@pytest.mark.parametrize("r,k", [(4.0, 8), (4.0, 32), (2.5, 16), (12.0, 4)])
-def test_partition_radius_raster_matches_calc_search_size(ctx, r, k):
+@pytest.mark.parametrize("r,k,capped", [(4.0, 8, True), (4.0, 32, True), (2.5, 16, True),
+                                         (12.0, 4, False)])
+def test_partition_radius_raster_matches_calc_search_size(ctx, r, k, capped):
     pts = _cloud(11)
     qs = _queries(pts, 12)
     ctx.set_points(pts)
@@ -144,9 +145,10 @@ def test_partition_radius_raster_matches_calc_search_size(ctx, r, k):
     want = raster_expected(pts, qs, r, k)
     bad = np.nonzero(got.view(np.uint32) != want.view(np.uint32))[0]
     assert len(bad) == 0, (len(bad), bad[:5], got[bad[:5]], want[bad[:5]])
-    # the cases are all exercised: megacells of several sizes, the width cap
-    # (radius r) and corner/far queries
-    assert len(np.unique(want)) >= 3 and (want == F(r)).any() and (want < F(r)).any()
+    # the cases are exercised: megacells of several sizes and, where the cloud
+    # is sparse enough for it, the width cap (radius r)
+    assert len(np.unique(want)) >= 3 and (want < F(r)).any()
+    assert (want == F(r)).any() == capped
 
 
 @pytest.mark.parametrize("levels,r,k", [(3, 4.0, 8), (5, 8.0, 16), (4, 6.0, 1)])
