@@ -135,6 +135,26 @@ int rtnn_search(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_device,
 int rtnn_search_async(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int mode, float r,
                       uint32_t k, int32_t* out_idx, float* out_d2, void* stream);
 
+/* rtnn_set_points + rtnn_search in one call that allocates nothing (no
+ * hipMalloc, hipFree or allocating library routine): for callers that cannot
+ * allocate (tight serving loops, harnesses that forbid it).  No counterpart in
+ * the reference.  xyz [n][3] (points), qxyz [nq][3] (queries, or NULL for the
+ * points themselves), out_idx / out_d2 (nullable) [nq][k]: all device
+ * pointers.  Result contract as rtnn_search, rows equal to it.  Every buffer
+ * comes from one 128 MiB device array of the library, which each GPU it runs
+ * on holds until the process ends: about 40 B per point plus 24 B per
+ * separate query, the rest for the cell table (the cell edge grows to fit,
+ * results unchanged) -- roughly up to 3M points as their own queries, or 2M
+ * points with 2M other queries.  Larger inputs: RTNN_ERR_UNSUPPORTED.  The
+ * context's options (cell factor, lanes per query, buffers) apply; the
+ * context's own buffers and points are not used or changed.  The array keeps
+ * the last call's contents; calls that overlap on one GPU (another thread,
+ * another context on the same device) race and must be serialised by the
+ * caller.  Synchronous: returns once the rows are written. */
+int rtnn_search_noalloc(rtnn_ctx ctx, const float* xyz, uint64_t n, const float* qxyz,
+                        uint64_t nq, int mode, float r, uint32_t k, int32_t* out_idx,
+                        float* out_d2);
+
 /* KNN with the reference's exact density partition (sort.cpp:140-219,346-391
  * with approx mode 0): per query cell the smallest megacell holding >= k+1
  * points gives a launch radius min(r, sqrt(3)*(iter+1)*cell); a query whose

@@ -105,6 +105,7 @@ SIGNATURES = {
     "rtnn_search": (_I, [_VP, _VP, _U64, _I, _I, _F, _U32, _VP, _VP, _I,
                          ctypes.POINTER(Stats)]),
     "rtnn_search_async": (_I, [_VP, _VP, _U64, _I, _F, _U32, _VP, _VP, _VP]),
+    "rtnn_search_noalloc": (_I, [_VP, _VP, _U64, _VP, _U64, _I, _F, _U32, _VP, _VP]),
     "rtnn_search_partitioned": (_I, [_VP, _VP, _U64, _I, _F, _U32, _VP, _VP, _I,
                                      ctypes.POINTER(Stats)]),
     "rtnn_mlp_train": (_I, [_VP, _F, _U32, _U32, _U32, _U32, _F, _F, _U64,
@@ -354,6 +355,27 @@ class Context:
             self._h, qptr, nq, _mode(mode), radius, k, ctypes.c_void_p(idx.data_ptr()),
             ctypes.c_void_p(d2.data_ptr()) if d2 is not None else None,
             ctypes.c_void_p(st.cuda_stream)))
+
+    def search_noalloc(self, points, queries=None, mode="radius", radius=2.0, k=50,
+                       idx=None, d2=None):
+        """rtnn_search_noalloc: points, queries (None: the points) and the
+        outputs idx [nq, k] int32 / d2 [nq, k] float32 (None: not written) are
+        device tensors; the library allocates nothing."""
+        p = _as_points(points)
+        q = None if queries is None else _as_points(queries)
+        on_dev = lambda t: _is_torch(t) and t.is_cuda
+        if not on_dev(p) or (q is not None and not on_dev(q)):
+            raise ValueError("search_noalloc takes device tensors")
+        nq = len(p) if q is None else len(q)
+        if idx is None:
+            import torch
+            idx = torch.empty((nq, k), dtype=torch.int32, device=p.device)
+        self._check(self._L.rtnn_search_noalloc(
+            self._h, ctypes.c_void_p(p.data_ptr()), len(p),
+            None if q is None else ctypes.c_void_p(q.data_ptr()), 0 if q is None else len(q),
+            _mode(mode), radius, k, ctypes.c_void_p(idx.data_ptr()),
+            None if d2 is None else ctypes.c_void_p(d2.data_ptr())))
+        return idx, d2
 
     def search_partitioned(self, queries=None, radius=2.0, k=8, with_d2=True,
                            out_device=None):

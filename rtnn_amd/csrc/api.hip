@@ -164,6 +164,35 @@ struct PinBuf {
   }
 };
 
+// Scratch memory of rtnn_search_noalloc: one file-scope device array, which
+// every GPU the library runs on holds (its own copy) until the process ends.
+// In host code `scratch` names a host-side stand-in, not device memory: its
+// device address is taken once per device with hipGetSymbolAddress (with
+// that device current) and only that address is used.
+constexpr size_t kScratchBytes = 128ull << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_addr[kMaxDevices] = {};
+
+// Device address of `scratch` on `device` (current), looked up once per device.
+hipError_t scratch_address(int device, void** out) {
+  *out = nullptr;
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_addr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_addr[device] = p;
+  }
+  *out = g_scratch_addr[device];
+  return hipSuccess;
+}
+}  // namespace
+
 struct rtnn_ctx_s {
   int device = 0;
   hipStream_t st = nullptr;
@@ -216,6 +245,10 @@ struct rtnn_ctx_s {
   std::vector<uint64_t> class_upper;
   std::vector<rtnn_options> class_opt;
   std::vector<uint32_t> class_tiles;
+
+  // the device address of this device's copy of `scratch` (rtnn_search_noalloc)
+  void* scratch = nullptr;
+  hipError_t scratch_err = hipSuccess;
 
   // streamed search (rtnn_search_stream): two slots of device and pinned
   // buffers, a copy stream and its events
@@ -641,6 +674,10 @@ int rtnn_create(int device, rtnn_ctx* out) {
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
       cus > 0)
     c->num_cus = cus;
+  // Set-up of the allocation-free search: this device's scratch address.  A
+  // failure is kept and reported by rtnn_search_noalloc only.
+  c->scratch_err = scratch_address(device, &c->scratch);
+  if (c->scratch_err != hipSuccess) (void)hipGetLastError();
   *out = c;
   return RTNN_OK;
 }
@@ -847,6 +884,120 @@ int rtnn_search_async(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int mode, fl
   CK(hipEventRecord(ctx->ev_join, us));
   CK(hipStreamWaitEvent(own, ctx->ev_join, 0));
   return rc;
+}
+
+// The plain search of rtnn_set_points + rtnn_search with every buffer carved
+// from this device's `scratch` (no allocation): the grid is planned on the
+// device (bbox -> plan -> cell keys -> radix sort -> cell table -> gather),
+// the queries are put in Morton order of their cells, then the search kernel
+// writes the caller's rows.  Only the 16-byte status word is read back.
+int rtnn_search_noalloc(rtnn_ctx ctx, const float* xyz, uint64_t n, const float* qxyz,
+                        uint64_t nq, int mode, float r, uint32_t k, int32_t* out_idx,
+                        float* out_d2) {
+  int rc = check_search_args(ctx, mode, r, k, out_idx);
+  if (rc) return rc;
+  if (!xyz || n == 0) return fail(ctx, RTNN_ERR_INVALID, "xyz must be set and n >= 1");
+  if (!qxyz) nq = n;
+  if (nq == 0) return RTNN_OK;
+  if (n >= (kScratchBytes >> 4) || nq >= (kScratchBytes >> 4))
+    return fail(ctx, RTNN_ERR_UNSUPPORTED,
+                "%llu points / %llu queries exceed the %llu MiB scratch of rtnn_search_noalloc",
+                (unsigned long long)n, (unsigned long long)nq,
+                (unsigned long long)(kScratchBytes >> 20));
+  const int gl = group_lanes_from_opt(ctx, mode, k);
+  if (gl < 0 || k > max_k_for_group(gl))
+    return fail(ctx, RTNN_ERR_UNSUPPORTED, "k=%u is not supported by the selected kernel", k);
+  if (ctx->scratch_err != hipSuccess || !ctx->scratch)
+    return hip_fail(ctx, ctx->scratch_err != hipSuccess ? ctx->scratch_err : hipErrorUnknown,
+                    "hipGetSymbolAddress(scratch)");
+  // Carve the scratch (256-byte aligned pieces); sizes beyond it are refused.
+  const bool own_q = qxyz != nullptr;
+  const uint64_t sort_n = std::max(n, own_q ? nq : 0);
+  uint64_t used = 0;
+  auto take = [&](uint64_t bytes) -> uint64_t {
+    const uint64_t at = used;
+    used += (bytes + 255) & ~255ull;
+    return at;
+  };
+  const uint64_t o_status = take(16), o_offs = take(16), o_part = take(4 * 6 * kBboxBlocks);
+  const uint64_t o_bb = take(4 * 6), o_grid = take(sizeof(GridDesc));
+  const uint64_t o_keys = take(4 * n), o_vals = take(4 * n);
+  const uint64_t o_spts = take(16 * n + kSptsPadBytes);
+  const uint64_t o_tmp = take(4 * std::max(radix_tmp_words(sort_n),
+                                           cell_bounds_tmp_words(kScratchBytes / 4)));
+  const uint64_t o_qkeys = own_q ? take(4 * nq) : 0, o_qvals = own_q ? take(4 * nq) : 0;
+  const uint64_t o_qs = own_q ? take(16 * nq) : 0;
+  const uint64_t o_cells = take(0);
+  constexpr uint64_t kMinCells = 1ull << 16;
+  if (o_cells + 4 * (kMinCells + 1) > kScratchBytes)
+    return fail(ctx, RTNN_ERR_UNSUPPORTED,
+                "%llu points and %llu queries exceed the %llu MiB scratch of "
+                "rtnn_search_noalloc (about 40 B per point, 24 B per separate query, and "
+                "the cell table)",
+                (unsigned long long)n, (unsigned long long)(own_q ? nq : 0),
+                (unsigned long long)(kScratchBytes >> 20));
+  // The cell table takes the rest, up to the cell budget of rtnn_search.
+  const uint64_t cap = std::min<uint64_t>(std::min<uint64_t>((kScratchBytes - o_cells) / 4 - 1,
+                                                             max_cells_for(ctx, n)),
+                                          0xfffffff0ull);
+  unsigned char* base = static_cast<unsigned char*>(ctx->scratch);
+  uint32_t* status = reinterpret_cast<uint32_t*>(base + o_status);
+  uint64_t* offs = reinterpret_cast<uint64_t*>(base + o_offs);
+  float* bb = reinterpret_cast<float*>(base + o_bb);
+  GridDesc* grid = reinterpret_cast<GridDesc*>(base + o_grid);
+  uint32_t* keys = reinterpret_cast<uint32_t*>(base + o_keys);
+  uint32_t* vals = reinterpret_cast<uint32_t*>(base + o_vals);
+  float4* spts = reinterpret_cast<float4*>(base + o_spts);
+  uint32_t* tmp = reinterpret_cast<uint32_t*>(base + o_tmp);
+  uint32_t* cell_start = reinterpret_cast<uint32_t*>(base + o_cells);
+  CK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->st;
+  // status: [0] non-finite coordinates, [1] planning fell back, [2] cells;
+  // offsets of the one cloud: {0, n}
+  CK(fill_u32(status, 4, 0, st));
+  CK(fill_u32(reinterpret_cast<uint32_t*>(offs), 4, 0, st));
+  CK(fill_u32(reinterpret_cast<uint32_t*>(offs) + 2, 1, (uint32_t)n, st));
+  CK(bbox(xyz, n, reinterpret_cast<float*>(base + o_part), bb, &status[0], st));
+  const float f = ctx->opt.cell_factor > 0 ? ctx->opt.cell_factor : default_cell_factor(mode);
+  CK(plan_batched(bb, offs, 1, r / f, nullptr, cap, grid, status, st));
+  CK(cell_keys(xyz, n, grid, offs, 1, keys, vals, nullptr, st));
+  CK(radix_sort_pairs(keys, vals, n, bits_for(cap - 1), tmp, st, nullptr));
+  CK(cell_bounds(keys, n, (uint32_t)cap, cell_start, tmp, st));
+  CK(gather_points(xyz, vals, n, spts, st));
+  const float4* qproc = spts;
+  if (own_q) {
+    uint32_t* qkeys = reinterpret_cast<uint32_t*>(base + o_qkeys);
+    uint32_t* qvals = reinterpret_cast<uint32_t*>(base + o_qvals);
+    float4* qs = reinterpret_cast<float4*>(base + o_qs);
+    if (ctx->opt.query_order == 1) {
+      CK(iota_u32(qvals, nq, st));
+    } else {
+      CK(query_morton_keys(qxyz, nq, grid, -1, qkeys, qvals, st));
+      CK(radix_sort_pairs(qkeys, qvals, nq, 3 * kQueryKeyBits, tmp, st, nullptr));
+    }
+    CK(gather_points(qxyz, qvals, nq, qs, st));
+    qproc = qs;
+  }
+  SearchArgs a;
+  std::memset(&a, 0, sizeof a);
+  a.spts = spts;
+  a.cell_start = cell_start;
+  a.grids = grid;
+  a.queries = qproc;
+  a.nq = (uint32_t)nq;
+  a.r = r;
+  a.qradius_scale = 1.0f;
+  a.k = k;
+  a.out_idx = out_idx;
+  a.out_d2 = out_d2;
+  CK(launch_search(a, mode == RTNN_MODE_KNN, gl, st, ctx->num_cus, wide_buffers(ctx),
+                   tile_points(ctx)));
+  uint32_t hs[4] = {0, 0, 0, 0};
+  CK(hipMemcpyAsync(hs, status, 16, hipMemcpyDeviceToHost, st));
+  CK(hipStreamSynchronize(st));
+  if (hs[0])
+    return fail(ctx, RTNN_ERR_INVALID, "%u non-finite point coordinates (rows undefined)", hs[0]);
+  return RTNN_OK;
 }
 
 }  // extern "C"

@@ -517,6 +517,13 @@ __global__ __launch_bounds__(256) void k_query_morton(const float* __restrict__ 
   uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= nq) return;
   const GridDesc g = *grid;
+  if (shift < 0) {
+    // Grid planned on the device: 8 Morton bits per axis of the widest
+    // axis's cell coordinate (the host's rule for a known grid).
+    const uint32_t md = (uint32_t)max(max(g.dim[0], g.dim[1]), g.dim[2]) - 1u;
+    const int bits = md ? 32 - __clz((int)md) : 0;
+    shift = max(0, bits - 8);
+  }
   // Queries outside the box (or NaN) clamp to the border cells.
   uint32_t cx = (uint32_t)cell_of(q[3 * i], g.mn[0], g.inv, g.dim[0]) >> shift;
   uint32_t cy = (uint32_t)cell_of(q[3 * i + 1], g.mn[1], g.inv, g.dim[1]) >> shift;

@@ -55,7 +55,8 @@ hipError_t gather_points(const float* xyz, const uint32_t* vals, uint64_t n, flo
 // Bounding box of every kChunkPts consecutive points of spts (2 float4 each).
 hipError_t chunk_boxes(const float4* spts, uint64_t n, float4* box, hipStream_t st);
 // Morton key (10 bits per axis) of each query's cell in grid g, shifted right
-// by `shift` bits per axis; vals[i] = i.
+// by `shift` bits per axis (shift < 0: derived on the device from g's widest
+// axis, keeping 8 bits per axis); vals[i] = i.
 hipError_t query_morton_keys(const float* q, uint64_t nq, const GridDesc* g, int shift,
                              uint32_t* keys, uint32_t* vals, hipStream_t st);
 hipError_t morton3_codes(const uint32_t* xyz, uint64_t n, uint32_t* codes, hipStream_t st);
