@@ -65,7 +65,12 @@ typedef struct {
                              level whose cell edge is >= R * level_alpha (0: 1.0) */
   int   group_buffer;     /* group kernel's LDS hit buffer per query: 0 auto / 2
                              wide (2k + G keys), 1 compact (k + 2G keys: less LDS,
-                             more waves per CU, more selects on dense rows)     */
+                             more waves per CU, more selects on dense rows);
+                             3 pooled (k <= 64, raster grids, not batched): the
+                             candidates of queries_per_wave = 16, 32 or 64
+                             queries per wave are tested as one pool, hits kept
+                             in 2k-key buffers (a query with more is searched
+                             again by the group kernel)                        */
   int   lds_tile;         /* group kernel on a raster grid: 256, 512 or 1024 = each
                              wave stages the points of its queries' union block in
                              LDS (up to that many per wave) and scans them there

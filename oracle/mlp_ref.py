@@ -3,8 +3,12 @@
 The MLP is not part of the reference repository; it is specified by the task:
 30 -> 64 -> 64 -> 1, GELU, BF16 weights with FP32 accumulation, pinball loss.
 This module restates the exact network the GPU computes, in float64 numpy:
-  * weights and each layer's input rounded to BF16 (round to nearest even);
-  * GELU in the tanh form (Hendrycks & Gimpel 2016);
+  * weights and the inputs of the two MFMA layers rounded to BF16 (round to
+    nearest even); the last layer (64 -> 1, VALU on the GPU) takes the FP32
+    activations;
+  * GELU in the sigmoid form of Hendrycks & Gimpel (2016, section 2),
+    x sigma(1.702 x), which the GPU evaluates with one exp2 and one
+    reciprocal (`gelu_erf` is the exact x Phi(x), for its error bound);
   * features: log1p(count) * count_scale of the 27 edge-r cells around the
     query's cell ((dx,dy,dz) = (f%3-1, f//3%3-1, f//9-1)), then the query's
     offset inside its cell;
@@ -16,8 +20,7 @@ import numpy as np
 
 H, F, FP = 64, 30, 32
 OW1, OB1, OW2, OB2, OW3, OB3, NP = 0, 1920, 1984, 6080, 6144, 6208, 6209
-C = np.sqrt(2.0 / np.pi)
-A = 0.044715
+S = 1.702
 
 
 def bf16(x):
@@ -28,13 +31,24 @@ def bf16(x):
     return (u.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
 
 
+def _sigma(x):
+    return 0.5 * (1.0 + np.tanh(0.5 * x))  # 1 / (1 + exp(-x)) without overflow
+
+
 def gelu(x):
-    return 0.5 * x * (1.0 + np.tanh(C * (x + A * x ** 3)))
+    return x * _sigma(S * x)
 
 
 def gelu_grad(x):
-    t = np.tanh(C * (x + A * x ** 3))
-    return 0.5 * (1 + t) + 0.5 * x * (1 - t * t) * C * (1 + 3 * A * x * x)
+    s = _sigma(S * x)
+    return s + S * x * s * (1.0 - s)
+
+
+def gelu_erf(x):
+    """The exact GELU, x Phi(x) = 0.5 x (1 + erf(x / sqrt 2))."""
+    from math import erf, sqrt
+    x = np.asarray(x, np.float64)
+    return 0.5 * x * (1.0 + np.vectorize(lambda v: erf(v / sqrt(2.0)))(x))
 
 
 def unpack(p):
@@ -49,7 +63,7 @@ def forward(p, feats, keep=False):
     z1 = x @ bf16(W1).T + b1
     h1 = bf16(gelu(z1))
     z2 = h1 @ bf16(W2).T + b2
-    h2 = bf16(gelu(z2))
+    h2 = gelu(z2)
     o = h2 @ bf16(w3) + float(b3)
     if keep:
         return o, (x, z1, h1, z2, h2)

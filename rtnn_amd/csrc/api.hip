@@ -227,6 +227,7 @@ struct rtnn_ctx_s {
 
   // radius proposals (learned / partitioned)
   DevBuf pgrid, pcounts, qradius;
+  DevBuf ovf;  // pooled kernel: counter + rows whose hit buffers overflowed
   GridDesc hpgrid{};
   DevBuf tr_q, tr_idx, tr_d2, tr_t, tr_f, tr_out;
   MlpState mlp;
@@ -261,7 +262,7 @@ struct rtnn_ctx_s {
     DevBuf* all[] = {&pts, &keys, &vals, &counts, &cell_start, &spts, &tmp, &grids,
                      &qin, &qkeys, &qvals, &qsorted, &oidx, &od2, &shortrows, &misc,
                      &tiles, &offs, &bboxes, &pgrid, &pcounts, &qradius, &tr_q, &tr_idx,
-                     &tr_d2, &tr_t, &tr_f, &tr_out, &bcnt, &boff, &bpairs, &cboxes};
+                     &tr_d2, &tr_t, &tr_f, &tr_out, &bcnt, &boff, &bpairs, &cboxes, &ovf};
     for (DevBuf* b : all) b->release();
     if (st3) (void)hipStreamSynchronize(st3);
     for (int i = 0; i < 2; ++i) {
@@ -359,6 +360,13 @@ int default_group_lanes(int mode, uint32_t k) {
 
 // Group kernel LDS buffer: compact (k + 2G keys per query) only on request.
 bool wide_buffers(rtnn_ctx ctx) { return ctx->opt.group_buffer != 1; }
+// Pooled kernel (group_buffer 3): queries_per_wave 16, 32 or 64 queries per
+// wave share one candidate pool; k <= 64 and a raster grid (else the group
+// kernel of queries_per_wave runs, with wide buffers).
+bool pool_requested(rtnn_ctx ctx, uint32_t k) {
+  const int q = ctx->opt.queries_per_wave;
+  return ctx->opt.group_buffer == 3 && k <= 64 && (q == 16 || q == 32 || q == 64);
+}
 // Points per wave of the LDS candidate tile (0: no tile; option 1 = 512).
 int tile_points(rtnn_ctx ctx) { return ctx->opt.lds_tile == 1 ? 512 : ctx->opt.lds_tile; }
 
@@ -607,6 +615,15 @@ SearchArgs base_args(rtnn_ctx ctx, const float4* qproc, uint64_t nq, float r, ui
   return a;
 }
 
+// The pooled kernel's overflow list (nq rows) and its counter, cleared.
+int arm_overflow(rtnn_ctx ctx, uint64_t nq, SearchArgs& a) {
+  CK(ctx->ovf.ensure(4 * (nq + 64)));
+  a.ovf_count = ctx->ovf.as<uint32_t>();
+  a.ovf_rows = ctx->ovf.as<uint32_t>() + 64;
+  CK(fill_u32(a.ovf_count, 1, 0, ctx->st));
+  return RTNN_OK;
+}
+
 void fill_stats(rtnn_ctx ctx, rtnn_stats* s) {
   if (!s) return;
   for (int a = 0; a < 3; ++a) s->grid_dims[a] = (uint32_t)ctx->hgrid.dim[a];
@@ -755,8 +772,8 @@ int check_options(rtnn_ctx ctx, const rtnn_options* opt) {
     return fail(ctx, RTNN_ERR_INVALID, "query_order must be 0, 1 or 2");
   if (opt->grid_levels < 0 || opt->grid_levels > 8)
     return fail(ctx, RTNN_ERR_INVALID, "grid_levels must be in [0, 8]");
-  if (opt->group_buffer < 0 || opt->group_buffer > 2)
-    return fail(ctx, RTNN_ERR_INVALID, "group_buffer must be 0, 1 or 2");
+  if (opt->group_buffer < 0 || opt->group_buffer > 3)
+    return fail(ctx, RTNN_ERR_INVALID, "group_buffer must be 0, 1, 2 or 3");
   if (opt->lds_tile != 0 && opt->lds_tile != 1 && opt->lds_tile != 256 && opt->lds_tile != 512 &&
       opt->lds_tile != 1024)
     return fail(ctx, RTNN_ERR_INVALID, "lds_tile must be 0, 1 (= 512), 256, 512 or 1024");
@@ -816,7 +833,8 @@ static int search_impl(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_devi
   if (nq == 0) return RTNN_OK;
   if (nq >= 0xffffffffull) return fail(ctx, RTNN_ERR_UNSUPPORTED, "more than 2^32-2 queries");
   const int gl = group_lanes_from_opt(ctx, mode, k);
-  if (gl < 0 || k > max_k_for_group(gl)) {
+  const bool pooled = pool_requested(ctx, k);
+  if (!pooled && (gl < 0 || k > max_k_for_group(gl))) {
     if (ctx->opt.queries_per_wave > 0 && k <= 1024)
       return fail(ctx, RTNN_ERR_UNSUPPORTED, "k=%u is too large for queries_per_wave=%d", k,
                   ctx->opt.queries_per_wave);
@@ -837,7 +855,12 @@ static int search_impl(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_devi
   rc = output_targets(ctx, nq, k, out_idx, out_d2, out_on_device, o);
   if (rc) return rc;
   SearchArgs a = base_args(ctx, qproc, nq, r, k, o);
-  CK(launch_search(a, mode == RTNN_MODE_KNN, gl, ctx->st, ctx->num_cus, wide_buffers(ctx), tile_points(ctx)));
+  if (pooled) {
+    rc = arm_overflow(ctx, nq, a);
+    if (rc) return rc;
+  }
+  CK(launch_search(a, mode == RTNN_MODE_KNN, gl, ctx->st, ctx->num_cus, wide_buffers(ctx),
+                   tile_points(ctx), false, pooled ? ctx->opt.queries_per_wave : 0));
   if (timing) CK(hipEventRecord(ctx->ev[3], ctx->st));
   rc = copy_out(ctx, nq, k, out_idx, out_d2, out_on_device, o);
   if (rc) return rc;
@@ -851,8 +874,8 @@ static int search_impl(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_devi
     stats->ms_search = ev_ms(ctx->ev[2], ctx->ev[3]);
     stats->ms_copy = out_on_device ? 0.0 : ev_ms(ctx->ev[3], ctx->ev[4]);
     stats->ms_total = ev_ms(ctx->ev[0], ctx->ev[4]);
-    stats->search_launches = 1;
-    stats->lanes_per_query = (uint32_t)gl;
+    stats->search_launches = pooled ? 2 : 1;
+    stats->lanes_per_query = pooled ? (uint32_t)(64 / ctx->opt.queries_per_wave) : (uint32_t)gl;
     fill_stats(ctx, stats);
   }
   return RTNN_OK;
@@ -905,7 +928,8 @@ int rtnn_search_noalloc(rtnn_ctx ctx, const float* xyz, uint64_t n, const float*
                 (unsigned long long)n, (unsigned long long)nq,
                 (unsigned long long)(kScratchBytes >> 20));
   const int gl = group_lanes_from_opt(ctx, mode, k);
-  if (gl < 0 || k > max_k_for_group(gl))
+  const bool pooled = pool_requested(ctx, k);
+  if (!pooled && (gl < 0 || k > max_k_for_group(gl)))
     return fail(ctx, RTNN_ERR_UNSUPPORTED, "k=%u is not supported by the selected kernel", k);
   if (ctx->scratch_err != hipSuccess || !ctx->scratch)
     return hip_fail(ctx, ctx->scratch_err != hipSuccess ? ctx->scratch_err : hipErrorUnknown,
@@ -927,6 +951,7 @@ int rtnn_search_noalloc(rtnn_ctx ctx, const float* xyz, uint64_t n, const float*
                                            cell_bounds_tmp_words(kScratchBytes / 4)));
   const uint64_t o_qkeys = own_q ? take(4 * nq) : 0, o_qvals = own_q ? take(4 * nq) : 0;
   const uint64_t o_qs = own_q ? take(16 * nq) : 0;
+  const uint64_t o_ovf = pooled ? take(4 * (nq + 64)) : 0;
   const uint64_t o_cells = take(0);
   constexpr uint64_t kMinCells = 1ull << 16;
   if (o_cells + 4 * (kMinCells + 1) > kScratchBytes)
@@ -990,8 +1015,13 @@ int rtnn_search_noalloc(rtnn_ctx ctx, const float* xyz, uint64_t n, const float*
   a.k = k;
   a.out_idx = out_idx;
   a.out_d2 = out_d2;
+  if (pooled) {
+    a.ovf_count = reinterpret_cast<uint32_t*>(base + o_ovf);
+    a.ovf_rows = a.ovf_count + 64;
+    CK(fill_u32(a.ovf_count, 1, 0, st));
+  }
   CK(launch_search(a, mode == RTNN_MODE_KNN, gl, st, ctx->num_cus, wide_buffers(ctx),
-                   tile_points(ctx)));
+                   tile_points(ctx), false, pooled ? ctx->opt.queries_per_wave : 0));
   uint32_t hs[4] = {0, 0, 0, 0};
   CK(hipMemcpyAsync(hs, status, 16, hipMemcpyDeviceToHost, st));
   CK(hipStreamSynchronize(st));
@@ -1126,7 +1156,8 @@ int proposal_search(rtnn_ctx ctx, Proposal kind, const float* qxyz, uint64_t nq,
   uint32_t* counters = lists[0] + 4 * nq;
   CK(fill_u32(counters, kGrowStages + 1, 0, ctx->st));
   const int gl = group_lanes_from_opt(ctx, RTNN_MODE_KNN, k);
-  if (gl < 0 || k > max_k_for_group(gl))
+  const bool pooled = pool_requested(ctx, k) && ctx->hlevels == 0;
+  if (!pooled && (gl < 0 || k > max_k_for_group(gl)))
     return fail(ctx, RTNN_ERR_UNSUPPORTED, "k=%u not supported by the selected kernel", k);
   // Stage 1: every query at its proposed radius r'.
   SearchArgs a = base_args(ctx, qproc, nq, r, k, o);
@@ -1134,7 +1165,12 @@ int proposal_search(rtnn_ctx ctx, Proposal kind, const float* qxyz, uint64_t nq,
   a.short_rows = lists[0];
   a.short_count = &counters[0];
   a.short_hits = hits[0];
-  CK(launch_search(a, 1, gl, ctx->st, ctx->num_cus, wide_buffers(ctx), tile_points(ctx)));
+  if (pooled) {
+    rc = arm_overflow(ctx, nq, a);
+    if (rc) return rc;
+  }
+  CK(launch_search(a, 1, gl, ctx->st, ctx->num_cus, wide_buffers(ctx), tile_points(ctx), false,
+                   pooled ? ctx->opt.queries_per_wave : 0));
   CK(hipEventRecord(ctx->ev[3], ctx->st));
   // Growth stages: a row short at radius R (m < k hits) is searched again at
   // min(r, R * hit_scale(m, k)), hit_scale in [1.15, 2]; the radius it used is
@@ -1179,8 +1215,8 @@ int proposal_search(rtnn_ctx ctx, Proposal kind, const float* qxyz, uint64_t nq,
     stats->ms_copy = out_on_device ? 0.0 : ev_ms(ctx->ev[4], ctx->ev[5]);
     stats->ms_total = ev_ms(ctx->ev[0], ctx->ev[5]);
     stats->fallback_queries = nshort;
-    stats->search_launches = 2 + kGrowStages;
-    stats->lanes_per_query = (uint32_t)gl;
+    stats->search_launches = 2 + kGrowStages + (pooled ? 1 : 0);
+    stats->lanes_per_query = pooled ? (uint32_t)(64 / ctx->opt.queries_per_wave) : (uint32_t)gl;
     fill_stats(ctx, stats);
   }
   return RTNN_OK;
@@ -1951,6 +1987,11 @@ int rtnn_autotune(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_device, i
             cands.push_back({cfs[i], qpws[j], plain ? 0 : 1, 0.0f, 1, tp, 0});
       }
     }
+  // The pooled kernel (k <= 64): 16, 32 or 64 queries per wave share one
+  // candidate pool (raster grids).
+  if (k <= 64)
+    for (int i = 0; i < ncf; ++i)
+      for (int qpw : {16, 32, 64}) cands.push_back({cfs[i], qpw, plain ? 0 : 1, 0.0f, 3, 0, 0});
   // Hierarchical grids: the octree walk (one lane per query) with leaves of
   // 256 or 65536 points, and 4-lane groups on the level rule.
   if (!plain)
@@ -1962,7 +2003,7 @@ int rtnn_autotune(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_device, i
   for (const Cand& cand : cands) {
     {
       const int qpw = cand.qpw;
-      if (qpw > 0 && k > max_k_for_group(64 / qpw)) continue;
+      if (cand.buffer != 3 && qpw > 0 && k > max_k_for_group(64 / qpw)) continue;
       if (qpw == 0 && k > 64) continue;
       ctx->opt = base;
       ctx->opt.cell_factor = cand.cf;

@@ -81,6 +81,11 @@ struct SearchArgs {
   // a large leaf that lie beyond its limit.
   const float4* chunk_box;
   uint32_t tree_leaf;  // octree walk: cells of at most this many points are leaves
+  // Pooled kernel (k_topk_pool): a query with more hits than its LDS buffer
+  // holds appends its processing position here (count in ovf_count); the
+  // group kernel then searches those rows again.
+  uint32_t* ovf_rows;
+  uint32_t* ovf_count;
 };
 
 constexpr uint32_t kTreeLeafDefault = 256;

@@ -94,8 +94,14 @@ hipError_t partition_hier(const float4* qproc, uint32_t nq, const GridDesc* leve
 // k <= 16 as for k = 32 (fewer selects where rows have many hits: the
 // fallback stages).  Returns hipErrorInvalidValue when no kernel variant
 // covers (k, group_lanes).
+// pool = 16, 32 or 64 selects the pooled kernel (k_topk_pool: the candidates
+// of that many queries per wave tested as one pool; k <= 64, one raster grid,
+// no tile list) followed by the 4-lane group kernel over the rows whose LDS
+// buffers overflowed (a.ovf_rows / a.ovf_count: device list and counter,
+// counter zero on entry).
 hipError_t launch_search(const SearchArgs& a, int knn_mode, int group_lanes, hipStream_t st,
-                         int num_cus, bool wide = true, int tiled = 0, bool deep = false);
+                         int num_cus, bool wide = true, int tiled = 0, bool deep = false,
+                         int pool = 0);
 // Every device / pinned allocation of the library (tracked for
 // rtnn_mem_stats; logged with RTNN_MEMSTAT set).
 hipError_t dev_alloc(void** p, size_t bytes);

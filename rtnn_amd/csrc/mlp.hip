@@ -16,11 +16,13 @@
 // Per-block gradient partials are summed in a fixed order, so training is
 // deterministic.
 //
-// GELU is the tanh form of Hendrycks & Gimpel (2016),
-//   gelu(x) = 0.5 x (1 + tanh(sqrt(2/pi) (x + 0.044715 x^3))),
-// evaluated as x / (1 + exp(-2u)) through the hardware exp2 and reciprocal
-// (v_exp_f32, v_rcp_f32; about 1 ulp each, far below the BF16 rounding of the
-// activations); its derivative through tanh(u) = 1 - 2 / (1 + exp(2u)).
+// GELU is the sigmoid form of Hendrycks & Gimpel (2016, "Gaussian Error
+// Linear Units", section 2), gelu(x) = x sigma(1.702 x), which stays within
+// 0.021 of x Phi(x) for every x: x / (1 + exp2(-1.702 log2(e) x)) is one
+// multiply, the hardware exp2 (v_exp_f32), an add, the hardware reciprocal
+// (v_rcp_f32; about 1 ulp each, far below the BF16 rounding of the
+// activations) and a multiply.  Training differentiates the same function.
+// The last layer (64 -> 1, VALU) takes the FP32 activations.
 #include <math.h>
 
 #include "../../include/rtnn_hip.h"
@@ -48,29 +50,29 @@ __host__ __device__ inline uint16_t f2bf(float f) {
 __device__ inline float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 __device__ inline float round_bf(float f) { return bf2f(f2bf(f)); }
 
-constexpr float kGeluC = 0.7978845608028654f;  // sqrt(2 / pi)
-constexpr float kGeluA = 0.044715f;
+constexpr float kGeluS = 1.702f;                          // sigma slope
+constexpr float kGeluE = -1.702f * 1.4426950408889634f;   // -1.702 log2(e)
 
-__device__ inline float tanh_fast(float u) {
-  return 1.0f - 2.0f * __builtin_amdgcn_rcpf(1.0f + __expf(2.0f * u));
+__device__ inline float gelu_sigma(float x) {
+  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(kGeluE * x));
 }
-// 0.5 x (1 + tanh(u)) = x / (1 + exp(-2u)): one hardware exp2 and one
-// reciprocal, u expanded so exp2 takes x * (k1 + k2 x^2) directly
-// (k1 = -2 log2(e) c, k2 = k1 a).  exp2 overflowing to +inf gives 0 (the
-// limit for x -> -inf).
-constexpr float kGeluK1 = -2.0f * 1.4426950408889634f * kGeluC;
-constexpr float kGeluK2 = kGeluK1 * kGeluA;
-__device__ inline float gelu(float x) {
-  const float e = __builtin_amdgcn_exp2f(x * fmaf(kGeluK2, x * x, kGeluK1));
-  return x * __builtin_amdgcn_rcpf(1.0f + e);
-}
+// exp2 overflowing to +inf (x -> -inf) gives sigma = 0 and gelu = -0.
+__device__ inline float gelu(float x) { return x * gelu_sigma(x); }
 __device__ inline float gelu_grad(float x) {
-  float u = kGeluC * (x + kGeluA * x * x * x);
-  float t = tanh_fast(u);
-  return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * kGeluC * (1.0f + 3.0f * kGeluA * x * x);
+  const float s = gelu_sigma(x);
+  return s + kGeluS * x * s * (1.0f - s);
 }
 
 // ------------------------------------------------------------ features --
+
+constexpr float kLn2 = 0.6931471805599453f;
+
+// log1p(count) * count_scale as log2(1 + count) * (count_scale ln 2): one
+// hardware log2 (v_log_f32) and one multiply per count feature.
+__device__ inline float count_feature(uint32_t c, float scale_ln2) {
+  return __builtin_amdgcn_logf(1.0f + (float)c) * scale_ln2;
+}
+
 
 // Feature f of a query: f < 27 -> log1p(count of coarse cell c + (dx,dy,dz)),
 // (dx,dy,dz) = (f%3-1, f/3%3-1, f/9-1), times count_scale; 27..29 -> offset of
@@ -113,7 +115,7 @@ __device__ inline float feature(const GridDesc& g, const uint32_t* __restrict__ 
       c = counts[((uint32_t)z * (uint32_t)g.dim[1] + (uint32_t)y) * (uint32_t)g.dim[0] +
                  (uint32_t)x];
     }
-    return __logf(1.0f + (float)c) * count_scale;
+    return count_feature(c, count_scale * kLn2);
   }
   if (f == 27) return qc.ox;
   if (f == 28) return qc.oy;
@@ -132,9 +134,52 @@ struct FeatureLoads {
   uint32_t valid;  // bit t: feature t is a count inside the grid
 };
 
+// What the 8 features of a lane are, fixed by its lane group: bit t of cnt
+// says feature t is a cell count (f < 27), off[t] is that cell's raster
+// offset from the query's cell, and features 27, 28, 29 (the query's offset
+// in its cell; all in the second fragment) are w{x,y,z}[t - 4] = 1.
+struct FeatureLane {
+  int off[8];
+  uint32_t cnt;
+  float wx[4], wy[4], wz[4];
+};
+
+__device__ inline void feature_lane(const GridDesc& g, int grp, FeatureLane& fn) {
+  fn.cnt = 0;
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    const int f = 16 * (t / 4) + 4 * grp + (t % 4);
+    const int dx = f % 3 - 1, dy = (f / 3) % 3 - 1, dz = f / 9 - 1;
+    fn.off[t] = f < 27 ? (dz * g.dim[1] + dy) * g.dim[0] + dx : 0;
+    fn.cnt |= f < 27 ? (1u << t) : 0u;
+    if (t >= 4) {
+      fn.wx[t - 4] = f == 27 ? 1.0f : 0.0f;
+      fn.wy[t - 4] = f == 28 ? 1.0f : 0.0f;
+      fn.wz[t - 4] = f == 29 ? 1.0f : 0.0f;
+    }
+  }
+}
+
 template <bool MORTON>
 __device__ inline void feature_loads(const GridDesc& g, const uint32_t* __restrict__ counts,
-                                     const QueryCell& qc, int grp, FeatureLoads& fl) {
+                                     const QueryCell& qc, int grp, const FeatureLane& fn,
+                                     FeatureLoads& fl) {
+  if constexpr (!MORTON) {
+    // When every query of the wave has its 27 cells inside the grid (the
+    // common case) a feature's cell is the query's cell plus a fixed offset.
+    const bool inner = qc.cx >= 1 && qc.cy >= 1 && qc.cz >= 1 && qc.cx + 1 < g.dim[0] &&
+                       qc.cy + 1 < g.dim[1] && qc.cz + 1 < g.dim[2];
+    if (__ballot(!inner) == 0) {
+      const int lin = (qc.cz * g.dim[1] + qc.cy) * g.dim[0] + qc.cx;
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        fl.c0[t] = counts[(fn.cnt >> t) & 1u ? (uint32_t)(lin + fn.off[t]) : 0u];
+        fl.c1[t] = 0u;
+      }
+      fl.valid = fn.cnt;
+      return;
+    }
+  }
   uint32_t id[8];
   fl.valid = 0;
 #pragma unroll
@@ -160,14 +205,13 @@ __device__ inline void feature_loads(const GridDesc& g, const uint32_t* __restri
 }
 
 template <bool MORTON>
-__device__ inline void feature_frag(const QueryCell& qc, int grp, float count_scale,
+__device__ inline void feature_frag(const QueryCell& qc, float scale_ln2, const FeatureLane& fn,
                                     const FeatureLoads& fl, bf16x4 (&x)[2]) {
 #pragma unroll
   for (int t = 0; t < 8; ++t) {
-    const int f = 16 * (t / 4) + 4 * grp + (t % 4);
     const uint32_t c = MORTON ? fl.c1[t] - fl.c0[t] : fl.c0[t];
-    float v = (fl.valid >> t) & 1u ? __logf(1.0f + (float)c) * count_scale : 0.0f;
-    v = f == 27 ? qc.ox : (f == 28 ? qc.oy : (f == 29 ? qc.oz : v));
+    float v = (fl.valid >> t) & 1u ? count_feature(c, scale_ln2) : 0.0f;
+    if (t >= 4) v = fmaf(fn.wx[t - 4], qc.ox, fmaf(fn.wy[t - 4], qc.oy, fmaf(fn.wz[t - 4], qc.oz, v)));
     x[t / 4][t % 4] = (short)f2bf(v);
   }
 }
@@ -263,7 +307,7 @@ __device__ inline float mlp_tile(const MfmaLds& w, const bf16x4 (&x)[2]) {
   for (int s = 0; s < 4; ++s)
 #pragma unroll
     for (int j = 0; j < 4; ++j)
-      part = fmaf(w.w3[16 * s + 4 * grp + j], round_bf(gelu(acc[s][j])), part);
+      part = fmaf(w.w3[16 * s + 4 * grp + j], gelu(acc[s][j]), part);
   part += __shfl_xor(part, 16, 64);
   part += __shfl_xor(part, 32, 64);
   return part + w.b3;
@@ -316,21 +360,24 @@ __global__ __launch_bounds__(256) void k_mlp_propose(const float* __restrict__ P
   };
   uint32_t t = wave;
   if (t >= ntiles) return;
+  FeatureLane fn;
+  feature_lane(g, grp, fn);
+  const float scale_ln2 = count_scale * kLn2;
   QueryCell qa = query_cell(g, 0.f, 0.f, 0.f);
   {
     const float4 q = load_q(t);
     qa = query_cell(g, q.x, q.y, q.z);
   }
   FeatureLoads la;
-  feature_loads<MORTON>(g, counts, qa, grp, la);
+  feature_loads<MORTON>(g, counts, qa, grp, fn, la);
   float4 qb = load_q(t + nwaves);
   for (; t < ntiles; t += nwaves) {
     const QueryCell qn = query_cell(g, qb.x, qb.y, qb.z);
     FeatureLoads ln;
-    feature_loads<MORTON>(g, counts, qn, grp, ln);
+    feature_loads<MORTON>(g, counts, qn, grp, fn, ln);
     const float4 qc = load_q(t + 2 * nwaves);
     bf16x4 x[2];
-    feature_frag<MORTON>(qa, grp, count_scale, la, x);
+    feature_frag<MORTON>(qa, scale_ln2, fn, la, x);
     const float o = mlp_tile(w, x);
     const uint32_t qi = t * 16 + row;
     if (grp == 0 && qi < nq) {
@@ -408,7 +455,7 @@ __global__ __launch_bounds__(kGradBlock) void k_mlp_grad(const float* __restrict
     sZ2[t][o] = z + sB2[o];
   }
   float out = 0.0f;
-  for (int o = 0; o < kMlpH; ++o) out += sW3[o] * round_bf(gelu(sZ2[t][o]));
+  for (int o = 0; o < kMlpH; ++o) out += sW3[o] * gelu(sZ2[t][o]);
   out += b3;
   const float tgt = valid ? targets[row] : 0.0f;
   const float diff = tgt - out;
@@ -432,7 +479,7 @@ __global__ __launch_bounds__(kGradBlock) void k_mlp_grad(const float* __restrict
     for (int s = 0; s < kGradBlock; ++s) {
       gb2 += sD2[s][t];
       gb1 += sD1[s][t];
-      gw3 += sG[s] * round_bf(gelu(sZ2[s][t]));
+      gw3 += sG[s] * gelu(sZ2[s][t]);
     }
     dst[kOB2 + t] = gb2;
     dst[kOB1 + t] = gb1;

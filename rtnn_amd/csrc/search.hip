@@ -894,6 +894,283 @@ __global__ __launch_bounds__(kTileQ) void k_topk_group(const SearchArgs a) {
 }
 
 // --------------------------------------------------------------------------
+// The candidates of a wave's queries as one pool
+//
+// k_topk_group gives each query G lanes and walks its runs batch by batch, so
+// a wave takes as long as its slowest query and most lanes test nothing at a
+// time: queries differ in row count and run lengths.  Here a wave lists the
+// runs of its QW queries in LDS first -- one row of cells per run, each row
+// clipped to the cells its query's sphere reaches -- and numbers their points
+// as one pool.  Lane l tests pool slots l, l + 64, ...: consecutive lanes
+// read consecutive points of a run (coalesced loads), each slot against the
+// query its run belongs to, and a hit goes to that query's LDS buffer
+// (CAPQ keys, filled by an LDS atomic).  The G = 64 / QW lanes of a query
+// then rank and write its row as k_topk_group does.  A query with more hits
+// than its buffer holds appends its position to ovf_rows; the group kernel
+// searches those rows again.  Raster grid, one grid per launch.
+
+constexpr int kPoolRows = 256;    // runs listed per pass (an index fits a byte)
+constexpr int kPoolSlots = 1024;  // pool slots mapped to runs per window
+constexpr int kPoolUnroll = 4;    // slots per lane per step, loads in flight
+
+template <int QW, int CAPQ>
+struct PoolLds {
+  float4 q[QW];                    // query xyz and d2 limit
+  uint32_t cnt[QW];                // hits found (above CAPQ: overflow)
+  uint32_t rcell[kPoolRows];       // row: first cell; then run: first point - pool offset
+  uint32_t rlen[kPoolRows];        // row: cells
+  uint8_t rq[kPoolRows];           // query of each row
+  __align__(16) uint8_t map[kPoolSlots];  // run of each slot of the current window
+  // CAPQ | 1 keys per query: an odd stride puts the queries' buffers on
+  // different LDS banks (the finish's rank loop reads entry j of every
+  // query's buffer at once)
+  uint64_t buf[QW][CAPQ | 1];
+};
+
+// Cells [clo, chi] of row (cy, cz) that can hold a point within R of q: the
+// row's y and z slabs, padded for the FP32 cell mapping (a point's cell index
+// can round across a boundary), bound dy and dz from below, so a hit has
+// |dx| <= sqrt(R'^2 - dy^2 - dz^2) with R' = R (1 + 2^-16) covering the FP32
+// d2 and these bounds' rounding; cell_range then maps that half-width.  The
+// border rows reach to infinity (cell indices are clamped into the grid).
+__device__ inline void clip_row(const GridDesc& g, const float4& q, float R, int cy, int cz,
+                                int lo0, int hi0, int& clo, int& chi) {
+  const float c = g.cell;
+  const float mag = fabsf(g.mn[1]) + fabsf(g.mn[2]) + fabsf(q.y) + fabsf(q.z) +
+                    (float)(max(cy, cz) + 1) * c;
+  const float pad = c * 0x1.0p-12f + mag * 0x1.0p-19f;
+  const float inf = __builtin_huge_valf();
+  const float ylo = cy == 0 ? -inf : g.mn[1] + (float)cy * c - pad;
+  const float yhi = cy == g.dim[1] - 1 ? inf : g.mn[1] + (float)(cy + 1) * c + pad;
+  const float zlo = cz == 0 ? -inf : g.mn[2] + (float)cz * c - pad;
+  const float zhi = cz == g.dim[2] - 1 ? inf : g.mn[2] + (float)(cz + 1) * c + pad;
+  const float dy = fmaxf(fmaxf(ylo - q.y, q.y - yhi), 0.0f);
+  const float dz = fmaxf(fmaxf(zlo - q.z, q.z - zhi), 0.0f);
+  const float Rp = R * (1.0f + 0x1.0p-16f);
+  const float h2 = Rp * Rp - (dy * dy + dz * dz);
+  if (!(h2 >= 0.0f)) {
+    clo = 1;
+    chi = 0;
+    return;
+  }
+  int a0, b0;
+  cell_range(g, q.x, sqrtf(h2), 0, a0, b0);
+  clo = max(a0, lo0);
+  chi = min(b0, hi0);
+}
+
+template <int QW, int KB, int CAPQ, bool KNN>
+__global__ __launch_bounds__(kTileQ) void k_topk_pool(const SearchArgs a) {
+  constexpr int G = kWave / QW;
+  constexpr int kWaves = kTileQ / kWave;
+  static_assert(4 * kWave == kPoolRows, "four rows per lane");
+  using L = PoolLds<QW, CAPQ>;
+  __shared__ L s_pool[kWaves];
+  const int w = threadIdx.x / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int gq = lane / G, l = lane % G;
+  L& W = s_pool[w];
+  const uint32_t k = a.k;
+  const uint32_t nq = launch_nq(a);
+  for (TileIter it(a, nq); it.t < it.end; it.t += it.step) {
+    uint32_t q0, q1, gi;
+    tile_of(a, nq, it.t, q0, q1, gi);
+    const GridDesc g = a.grids[gi];
+    for (uint32_t c0 = q0 + (uint32_t)(w * QW); c0 < q1; c0 += (uint32_t)(kWaves * QW)) {
+      // The chunk's queries, G lanes each.
+      const uint32_t qi = c0 + (uint32_t)gq;
+      const bool active = qi < q1;
+      uint32_t qs = 0, row = 0;
+      float4 qv = make_float4(0.f, 0.f, 0.f, 0.f);
+      float R = a.r, lim = -1.0f;
+      int lo0 = 0, hi0 = -1, lo1 = 0, hi1 = -1, lo2 = 0, hi2 = -1;
+      if (active) {
+        qs = a.qsel ? a.qsel[qi] : qi;
+        qv = a.queries[qs];
+        row = __float_as_uint(qv.w);
+        R = query_radius(a, qi, qs);
+        lim = r2_limit(R * R);
+        cell_range(g, qv.x, R, 0, lo0, hi0);
+        cell_range(g, qv.y, R, 1, lo1, hi1);
+        cell_range(g, qv.z, R, 2, lo2, hi2);
+      }
+      const int ny = hi1 - lo1 + 1, nz = hi2 - lo2 + 1;
+      const uint32_t nrows =
+          (active && hi0 >= lo0 && ny > 0 && nz > 0) ? (uint32_t)(ny * nz) : 0u;
+      wave_sync();  // the previous chunk's finish has read the buffers
+      if (l == 0) {
+        W.q[gq] = make_float4(qv.x, qv.y, qv.z, lim);
+        W.cnt[gq] = 0;
+      }
+      // rows of the chunk numbered query by query
+      const uint32_t rv = l == 0 ? nrows : 0u;
+      const uint32_t rincl = group_incl_scan<kWave>(rv, lane);
+      const uint32_t total_rows = __shfl(rincl, kWave - 1, kWave);
+      const uint32_t rbase = __shfl(rincl - rv, lane - l, kWave);
+      for (uint32_t rw = 0; rw < total_rows; rw += kPoolRows) {
+        const uint32_t nr = min((uint32_t)kPoolRows, total_rows - rw);
+        // each query lists its rows of this pass (row t: cy = lo1 + t % ny,
+        // cz = lo2 + t / ny), clipped to its sphere
+        if (nrows) {
+          int cy = lo1 + l, cz = lo2;
+          while (cy > hi1) {
+            cy -= ny;
+            ++cz;
+          }
+          for (uint32_t t = (uint32_t)l; t < nrows; t += G) {
+            const uint32_t j = rbase + t - rw;
+            if (rbase + t >= rw && j < nr) {
+              int clo, chi;
+              clip_row(g, qv, R, cy, cz, lo0, hi0, clo, chi);
+              const uint32_t rowc = g.cell_base + ((uint32_t)cz * (uint32_t)g.dim[1] +
+                                                   (uint32_t)cy) * (uint32_t)g.dim[0];
+              W.rcell[j] = rowc + (uint32_t)max(clo, 0);
+              W.rlen[j] = chi >= clo ? (uint32_t)(chi - clo + 1) : 0u;
+              W.rq[j] = (uint8_t)gq;
+            }
+            cy += G;
+            while (cy > hi1) {
+              cy -= ny;
+              ++cz;
+            }
+          }
+        }
+        wave_sync();
+        // runs: lane takes rows 4 lane .. 4 lane + 3; their points are
+        // numbered in row order (pool offsets by a wave scan)
+        uint32_t st[4], ln[4], own = 0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const uint32_t j = (uint32_t)(4 * lane + u);
+          uint32_t s0 = 0, e0 = 0;
+          if (j < nr) {
+            const uint32_t c = W.rcell[j], n = W.rlen[j];
+            if (n) {
+              s0 = a.cell_start[c];
+              e0 = a.cell_start[c + n];
+            }
+          }
+          st[u] = s0;
+          ln[u] = e0 - s0;
+          own += e0 - s0;
+        }
+        const uint32_t pincl = group_incl_scan<kWave>(own, lane);
+        const uint32_t P = __shfl(pincl, kWave - 1, kWave);
+        uint32_t of[4];
+        {
+          uint32_t o = pincl - own;
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            of[u] = o;
+            const uint32_t j = (uint32_t)(4 * lane + u);
+            if (j < nr) W.rcell[j] = st[u] - o;  // point of slot v = this + v
+            o += ln[u];
+          }
+        }
+        uint32_t wcarry = 0;  // run of the last slot of the previous window
+        for (uint32_t pw = 0; pw < P; pw += kPoolSlots) {
+          const uint32_t wend = min(P, pw + kPoolSlots);
+          // slot -> run map of the window [pw, wend): run indices rise with
+          // the slot, so the run of a slot is the largest index of a run that
+          // starts at or before it: zero the window, mark each run's first
+          // slot with its index, then a prefix maximum (16 slots per lane and
+          // a wave scan, carried over from the previous window).
+          reinterpret_cast<uint4*>(W.map)[lane] = make_uint4(0u, 0u, 0u, 0u);
+          wave_sync();
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            if (ln[u] && of[u] >= pw && of[u] < wend) W.map[of[u] - pw] = (uint8_t)(4 * lane + u);
+          wave_sync();
+          {
+            uint4 m = reinterpret_cast<const uint4*>(W.map)[lane];
+            uint32_t wv[4] = {m.x, m.y, m.z, m.w};
+            uint32_t run_max = 0;
+#pragma unroll
+            for (int b = 0; b < 16; ++b) run_max = max(run_max, (wv[b >> 2] >> (8 * (b & 3))) & 0xffu);
+            uint32_t incl = run_max;
+#pragma unroll
+            for (int o = 1; o < kWave; o <<= 1) {
+              const uint32_t t = __shfl_up(incl, o, kWave);
+              if (lane >= o) incl = max(incl, t);
+            }
+            uint32_t cur = max(wcarry, __shfl_up(incl, 1, kWave));
+            if (lane == 0) cur = wcarry;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              uint32_t o = 0;
+#pragma unroll
+              for (int b = 0; b < 4; ++b) {
+                cur = max(cur, (wv[i] >> (8 * b)) & 0xffu);
+                o |= cur << (8 * b);
+              }
+              wv[i] = o;
+            }
+            reinterpret_cast<uint4*>(W.map)[lane] = make_uint4(wv[0], wv[1], wv[2], wv[3]);
+            wcarry = max(wcarry, __shfl(incl, kWave - 1, kWave));
+          }
+          wave_sync();
+          for (uint32_t v0 = pw; v0 < wend; v0 += (uint32_t)(kWave * kPoolUnroll)) {
+            // In phases, each over every slot of the step, with no branch
+            // between reads (indices masked into range), so the LDS reads,
+            // then the point loads, of all slots are in flight together.
+            uint32_t rid[kPoolUnroll], qq[kPoolUnroll], pi[kPoolUnroll];
+            bool ok[kPoolUnroll];
+#pragma unroll
+            for (int u = 0; u < kPoolUnroll; ++u) {
+              const uint32_t v = v0 + (uint32_t)(u * kWave + lane);
+              ok[u] = v < wend;
+              rid[u] = W.map[(v - pw) & (uint32_t)(kPoolSlots - 1)];
+            }
+#pragma unroll
+            for (int u = 0; u < kPoolUnroll; ++u) {
+              const uint32_t v = v0 + (uint32_t)(u * kWave + lane);
+              qq[u] = (uint32_t)W.rq[rid[u]] & (uint32_t)(QW - 1);
+              pi[u] = ok[u] ? W.rcell[rid[u]] + v : 0u;
+            }
+            float4 pv[kPoolUnroll], qd[kPoolUnroll];
+#pragma unroll
+            for (int u = 0; u < kPoolUnroll; ++u) {
+              pv[u] = a.spts[pi[u]];
+              qd[u] = W.q[qq[u]];
+            }
+            uint32_t at[kPoolUnroll];
+            uint64_t key[kPoolUnroll];
+#pragma unroll
+            for (int u = 0; u < kPoolUnroll; ++u) {
+              const float d2 = d2_of(qd[u].x, qd[u].y, qd[u].z, pv[u].x, pv[u].y, pv[u].z);
+              const bool hit = ok[u] && (KNN ? (d2 > 0.0f && d2 <= qd[u].w) : (d2 <= qd[u].w));
+              key[u] = pack_key(d2, __float_as_uint(pv[u].w));
+              at[u] = hit ? atomicAdd(&W.cnt[qq[u]], 1u) : (uint32_t)CAPQ;
+            }
+#pragma unroll
+            for (int u = 0; u < kPoolUnroll; ++u)
+              if (at[u] < (uint32_t)CAPQ) W.buf[qq[u]][at[u]] = key[u];
+          }
+          wave_sync();
+        }
+      }
+      wave_sync();
+      const uint32_t cnt = active ? W.cnt[gq] : 0u;
+      // Over-full buffers: the row is searched again by the group kernel.
+      {
+        const bool ov = active && l == 0 && cnt > (uint32_t)CAPQ;
+        const uint64_t m = __ballot(ov);
+        if (m) {
+          const int leader = __ffsll((unsigned long long)m) - 1;
+          uint32_t b = 0;
+          if (lane == leader) b = atomicAdd(a.ovf_count, (uint32_t)__popcll(m));
+          b = __shfl(b, leader, kWave);
+          if (ov) a.ovf_rows[b + (uint32_t)__popcll(m & lanemask_lt64())] = qs;
+        }
+      }
+      if (active && cnt <= (uint32_t)CAPQ)
+        group_finish<G, KB>(a, W.buf[gq], cnt, lim, k, l, row,
+                            a.local_index ? g.point_base : 0u, qs, R);
+    }
+  }
+}
+
+// --------------------------------------------------------------------------
 // G lanes per query, candidates staged per wave in LDS
 //
 // The 64/G queries of a wave are consecutive in Morton order, so their cell
@@ -1102,6 +1379,23 @@ static void launch_group(const SearchArgs& a, bool knn, bool wide, int tile, dim
     hipLaunchKernelGGL((k_topk_group<G, KB, false, false, false>), grid, dim3(kTileQ), 0, st, a);
 }
 
+template <int QW, int KB>
+static void launch_pool_kb(const SearchArgs& a, bool knn, dim3 grid, hipStream_t st) {
+  if (knn)
+    hipLaunchKernelGGL((k_topk_pool<QW, KB, 2 * KB, true>), grid, dim3(kTileQ), 0, st, a);
+  else
+    hipLaunchKernelGGL((k_topk_pool<QW, KB, 2 * KB, false>), grid, dim3(kTileQ), 0, st, a);
+}
+
+template <int QW>
+static void launch_pool(const SearchArgs& a, bool knn, dim3 grid, hipStream_t st) {
+  if (a.k <= 16) launch_pool_kb<QW, 16>(a, knn, grid, st);
+  else if constexpr (QW <= 32) {
+    if (a.k <= 32) launch_pool_kb<QW, 32>(a, knn, grid, st);
+    else if constexpr (QW <= 16) launch_pool_kb<QW, 64>(a, knn, grid, st);
+  }
+}
+
 uint32_t max_k_for_group(int group_lanes) {
   switch (group_lanes) {
     case 1: return 64;
@@ -1115,10 +1409,41 @@ uint32_t max_k_for_group(int group_lanes) {
 }
 
 hipError_t launch_search(const SearchArgs& a0, int knn_mode, int group_lanes, hipStream_t st,
-                         int num_cus, bool wide, int tiled, bool deep) {
+                         int num_cus, bool wide, int tiled, bool deep, int pool) {
   SearchArgs a = a0;
   if (a.nq_dev) a.persistent = 1;  // tile count known only on the device
   const uint32_t k = a.k;
+  if (pool) {
+    // The pooled kernel, then the group kernel over the rows whose buffers
+    // overflowed (their count is read on the device).
+    if ((pool != 16 && pool != 32 && pool != 64) || k == 0 || k > 64 || a.nlevels > 0 ||
+        a.tiles || a.qsel_hits || !a.ovf_rows || !a.ovf_count)
+      return hipErrorInvalidValue;
+    // Hit buffers of 2 KB keys for QW queries: at most 16 KB of LDS per wave
+    // (QW * KB <= 1024), so wider rows take fewer queries per wave.
+    const int kb = k <= 16 ? 16 : k <= 32 ? 32 : 64;
+    pool = pool * kb > 1024 ? 1024 / kb : pool;
+    a.tile_q = (uint32_t)(4 * pool);
+    a.ntiles = (uint32_t)(((uint64_t)a.nq + a.tile_q - 1) / a.tile_q);
+    if (a.nq_dev) a.ntiles = 0;
+    if (a.ntiles || a.nq_dev) {
+      uint32_t blocks = a.ntiles;
+      if (a.persistent || a.nq_dev) blocks = (uint32_t)(num_cus > 0 ? num_cus : 256) * 8u;
+      const dim3 grid(blocks);
+      const bool knn = knn_mode != 0;
+      if (pool == 16) launch_pool<16>(a, knn, grid, st);
+      else if (pool == 32) launch_pool<32>(a, knn, grid, st);
+      else launch_pool<64>(a, knn, grid, st);
+      const hipError_t e = hipGetLastError();
+      if (e != hipSuccess) return e;
+    }
+    SearchArgs o = a0;
+    o.qsel = a0.ovf_rows;
+    o.nq_dev = a0.ovf_count;
+    o.ovf_rows = nullptr;
+    o.ovf_count = nullptr;
+    return launch_search(o, knn_mode, 4, st, num_cus, true, 0, true, 0);
+  }
   if (k == 0 || k > max_k_for_group(group_lanes)) return hipErrorInvalidValue;
   if (!a.tiles) {
     // Implicit tiles: a block of the thread kernel takes kTileQ queries, a

@@ -586,3 +586,53 @@ def test_mem_stats_track_the_library_allocations():
     assert after["device_bytes"] == base["device_bytes"]
     assert after["pinned_bytes"] == base["pinned_bytes"]
     assert after["device_peak"] >= during["device_bytes"]
+
+
+# ---- pooled kernel (group_buffer 3): the candidates of 16/32/64 queries per
+# wave tested as one pool; rows whose hit buffers overflow (more than 2k hits
+# within the radius) are searched again by the group kernel.
+
+@pytest.mark.parametrize("qpw", [16, 32, 64])
+@pytest.mark.parametrize("mode,r,k", [("radius", 2.0, 50), ("radius", 1.2, 8), ("knn", 2.5, 16),
+                                      ("knn", 1.5, 32), ("knn", 3.0, 4), ("radius", 2.0, 64)])
+def test_pooled_kernel_matches_oracle(ctx, qpw, mode, r, k):
+    rng = np.random.default_rng(100 + qpw + k)
+    # uniform background + dense clusters (rows far beyond 2k hits: overflow
+    # path) + far and duplicate queries
+    pts = np.concatenate([rng.uniform(0, 40, (60000, 3)), rng.normal(20, 0.4, (8000, 3)),
+                          np.repeat(rng.uniform(0, 40, (50, 3)), 20, axis=0)]).astype(np.float32)
+    qs = np.concatenate([rng.uniform(-3, 43, (12000, 3)), pts[60000:62000],
+                         [[500, 500, 500], [-100, 0, 0]]]).astype(np.float32)
+    ctx.set_points(pts)
+    for cf in (1.0, 2.0):
+        ctx.set_options(cell_factor=cf, queries_per_wave=qpw, group_buffer=3)
+        st = check(ctx, pts, qs, mode, r, k)
+        assert st.search_launches == 2
+        st = check(ctx, pts, None, mode, r, k, qsel=np.arange(0, len(pts), 7))
+
+
+def test_pooled_kernel_many_rows_and_windows(ctx):
+    # a radius spanning many cells (rows beyond one 256-row pass, pools beyond
+    # one 1024-slot window) and queries with no candidate at all
+    rng = np.random.default_rng(12)
+    pts = rng.uniform(0, 30, (40000, 3)).astype(np.float32)
+    qs = np.concatenate([rng.uniform(0, 30, (3000, 3)),
+                         rng.uniform(100, 130, (300, 3))]).astype(np.float32)
+    ctx.set_points(pts)
+    for qpw in (16, 32):
+        ctx.set_options(cell_factor=4.0, queries_per_wave=qpw, group_buffer=3)
+        check(ctx, pts, qs, "knn", 6.0, 16)
+        check(ctx, pts, qs, "radius", 3.0, 64)
+
+
+def test_pooled_kernel_in_proposal_searches(ctx):
+    pts = rtnn_amd.gen_spheres(3, 30, 400000, nspheres=80, box=800.0)
+    qs = rtnn_amd.gen_spheres(3, 31, 60000, nspheres=80, box=800.0)
+    ctx.set_points(pts)
+    pi, pd, _ = ctx.search(qs, mode="knn", radius=10.0, k=16)
+    ctx.mlp_train(10.0, 16, n_samples=1 << 16, steps=300, batch=4096, seed=3)
+    for qpw in (16, 32, 64):
+        ctx.set_options(cell_factor=2.0, queries_per_wave=qpw, group_buffer=3, grid_levels=1)
+        for fn in (ctx.search_learned, ctx.search_partitioned):
+            li, ld, st = fn(qs, radius=10.0, k=16)
+            assert np.array_equal(pi, li) and np.array_equal(pd, ld), (qpw, fn.__name__)

@@ -26,11 +26,25 @@ def test_bf16_known_answers():
 
 def test_gelu_closed_forms():
     assert M.gelu(np.array(0.0)) == 0.0
-    x = np.array([-6.0, 6.0])
-    assert np.allclose(M.gelu(x), [0.0, 6.0], atol=1e-6)
-    xs = np.linspace(-4, 4, 101)
+    # x sigma(a x) - (-x) sigma(-a x) = x, and the limits x and 0
+    xs = np.linspace(-8, 8, 161)
+    assert np.allclose(M.gelu(xs) - M.gelu(-xs), xs, atol=1e-12)
+    assert np.allclose(M.gelu(np.array([-30.0, 30.0])), [0.0, 30.0], atol=1e-9)
+    assert abs(M.gelu(np.array(1.0)) - 1.0 / (1.0 + np.exp(-1.702))) < 1e-12
     fd = (M.gelu(xs + 1e-6) - M.gelu(xs - 1e-6)) / 2e-6
     assert np.allclose(M.gelu_grad(xs), fd, atol=1e-6)
+
+
+def test_gelu_sigmoid_form_error_bound():
+    # Hendrycks & Gimpel's x sigma(1.702 x) stays within 0.021 of the exact
+    # x Phi(x) everywhere (the largest gap is near |x| = 2.1); a slope other
+    # than 1.702 or a dropped factor x breaks the bound.
+    xs = np.linspace(-10, 10, 20001)
+    err = np.abs(M.gelu(xs) - M.gelu_erf(xs))
+    assert err.max() < 0.021 and err.max() > 0.015
+    assert 1.8 < abs(xs[err.argmax()]) < 2.4
+    worse = np.abs(xs * M._sigma(1.6 * xs) - M.gelu_erf(xs)).max()
+    assert worse > 0.021
 
 
 def test_analytic_gradient_matches_finite_differences():
