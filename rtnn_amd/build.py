@@ -31,6 +31,9 @@ HIP_FLAGS = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
              "-packed-fp32-ops", "-fno-gpu-rdc"]
 
 SOURCES = ["sort.hip", "grid.hip", "search.hip", "partition.hip", "mlp.hip", "api.hip", "data.cpp"]
+# Per-file flags.  mlp.hip: MFMA results straight into VGPRs (the GELU reads
+# every accumulator; in AGPRs each would first need a v_accvgpr_read).
+EXTRA = {"mlp.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 
 
 def _run(cmd, verbose):
@@ -48,7 +51,7 @@ def _compile(src, verbose):
     path = os.path.join(CSRC, src)
     obj = os.path.join(BUILD, src + ".o")
     if src.endswith(".hip"):
-        cmd = [HIPCC, *HIP_FLAGS, *COMMON, "-x", "hip", "-c", path, "-o", obj]
+        cmd = [HIPCC, *HIP_FLAGS, *COMMON, *EXTRA.get(src, []), "-x", "hip", "-c", path, "-o", obj]
     else:
         cmd = ["g++", *COMMON, "-c", path, "-o", obj]
     out = _run(cmd, verbose)

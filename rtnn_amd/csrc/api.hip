@@ -2237,7 +2237,7 @@ struct rtnn_graph_s {
   hipGraphExec_t exec = nullptr;
   // Buffers owned by the graph, so calls on the context between replays
   // cannot move memory the graph's nodes point at.
-  DevBuf offs, bboxes, status, grids, keys, vals, cell_start, spts, tmp, tiles, ccells;
+  DevBuf offs, bboxes, status, grids, keys, vals, cell_start, spts, tmp, tiles, ccells, bbox_part;
   uint64_t n = 0;
   uint32_t nclouds = 0, ntiles = 0;
   uint64_t cap_cells = 0;
@@ -2250,7 +2250,7 @@ struct rtnn_graph_s {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     DevBuf* all[] = {&offs, &bboxes, &status, &grids, &keys, &vals, &cell_start, &spts, &tmp,
-                     &tiles, &ccells};
+                     &tiles, &ccells, &bbox_part};
     for (DevBuf* b : all) b->release();
   }
 };
@@ -2291,8 +2291,13 @@ hipError_t enqueue_batched(rtnn_graph g, const float* xyz, float cell, int mode,
   uint32_t* vals = g->vals.as<uint32_t>();
   hipError_t e;
   if ((e = fill_u32(status, 4, 0, st)) != hipSuccess) return e;
-  if ((e = bbox_segmented(xyz, off, g->nclouds, g->bboxes.as<float>(), status, st)) != hipSuccess)
-    return e;
+  // One cloud: the multi-block reduction (the per-cloud kernel gives each
+  // cloud one block, which for a single large cloud is the slowest step).
+  if (g->nclouds == 1)
+    e = bbox(xyz, g->n, g->bbox_part.as<float>(), g->bboxes.as<float>(), status, st);
+  else
+    e = bbox_segmented(xyz, off, g->nclouds, g->bboxes.as<float>(), status, st);
+  if (e != hipSuccess) return e;
   const float* cells = g->class_opt.empty() ? nullptr : g->ccells.as<float>();
   if ((e = plan_batched(g->bboxes.as<float>(), off, g->nclouds, cell, cells, g->cap_cells,
                         g->grids.as<GridDesc>(), status, st)) != hipSuccess)
@@ -2373,6 +2378,7 @@ int rtnn_graph_batched(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets,
   CK(g->tmp.ensure(4 * std::max(radix_tmp_words(n), cell_bounds_tmp_words(g->cap_cells))));
   CK(g->tiles.ensure(sizeof(Tile) * g->ntiles));
   CK(g->ccells.ensure(4 * (uint64_t)nclouds));
+  if (nclouds == 1) CK(g->bbox_part.ensure(4 * 6 * kBboxBlocks));
   CK(hipMemcpyAsync(g->ccells.p, cells.data(), 4 * (uint64_t)nclouds, hipMemcpyHostToDevice,
                     ctx->st));
   CK(hipMemcpyAsync(g->offs.p, offsets, 8 * (nclouds + 1), hipMemcpyHostToDevice, ctx->st));
