@@ -28,8 +28,8 @@ int rtnn_read_points(const char* path, float* xyz, uint64_t* n);
  * of 3); triple b of line i is point i of set b (util.cpp:18-99, tokenize and
  * read_pc_data(..., int* d): each number parsed with stof, decimal -> float
  * directly).  Two calls: with xyz == NULL it counts lines into *n and sets
- * into *nsets; then, with *n as returned and xyz of 3 * nsets * n floats, it
- * fills xyz set-major ([set][line][3]).  Differences, on purpose: a line with
+ * into *nsets; then, with *n and *nsets as returned and xyz of 3 * nsets * n
+ * floats, it fills xyz set-major ([set][line][3]).  Differences, on purpose: a line with
  * another count of numbers than the first, or a token that is not a number,
  * is an error (-1, *n = its 1-based line number) where the reference reads
  * past its vector or throws; a first line whose count is not a multiple of 3

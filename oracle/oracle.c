@@ -214,10 +214,12 @@ static int cgrid_build(cgrid* g, const float* pts, uint64_t n, float cell) {
   uint64_t ncells = 1;
   for (int a = 0; a < 3; ++a) {
     g->mn[a] = mn[a];
-    g->dim[a] = (int64_t)floorf((mx[a] - mn[a]) * g->inv) + 1;
+    float span = floorf((mx[a] - mn[a]) * g->inv);
+    if (!(span < 2147483648.0f)) return -1;  /* too many cells (or no finite span) */
+    g->dim[a] = (int64_t)span + 1;
     ncells *= (uint64_t)g->dim[a];
+    if (ncells > (1ull << 31)) return -1;
   }
-  if (ncells > (1ull << 31)) return -1;
   uint32_t* key = (uint32_t*)malloc(sizeof(uint32_t) * (n ? n : 1));
   g->start = (uint64_t*)calloc(ncells + 1, sizeof(uint64_t));
   for (uint64_t i = 0; i < n; ++i) {
@@ -276,7 +278,7 @@ static void* gs_worker(void* arg) {
       double u = floor(((double)q[a] + pad - g->mn[a]) * g->inv);
       if (l < 0) l = 0;
       if (u > (double)(g->dim[a] - 1)) u = (double)(g->dim[a] - 1);
-      if (!(l <= u)) empty = 1;
+      if (!(l <= u)) { empty = 1; break; }  /* beyond the box, or NaN */
       lo[a] = (int64_t)l; hi[a] = (int64_t)u;
     }
     if (!empty) {
@@ -309,6 +311,7 @@ int orc_grid_search(const float* pts, uint64_t n, const float* qs, const uint64_
     int rc = cgrid_build(&g, pts, n, cell);
     if (rc == 0) break;
     cell *= 2.0f;  /* too many cells: coarsen (any edge gives the same result) */
+    if (!(cell < INFINITY)) return -1;  /* non-finite points: no grid */
   }
   gs_job* jobs = (gs_job*)calloc(threads, sizeof(gs_job));
   for (int t = 0; t < threads; ++t) {

@@ -32,6 +32,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <queue>
 #include <string>
 #include <vector>
@@ -111,38 +112,58 @@ Args parse(int argc, char** argv) {
     if (i >= argc - 1) usage(argv[0], 1);
     return argv[++i];
   };
+  // Numeric values: a value that is not a number in full is a usage error
+  // (the reference's stof/atoi would throw or read 0).
+  auto num_f = [&](int& i) -> float {
+    const char* s = need(i);
+    char* end = nullptr;
+    float v = std::strtof(s, &end);
+    if (end == s || *end != '\0') usage(argv[0], 1);
+    return v;
+  };
+  auto num_i = [&](int& i) -> int {
+    const char* s = need(i);
+    char* end = nullptr;
+    long v = std::strtol(s, &end, 10);
+    if (end == s || *end != '\0' || v < -2147483647L || v > 2147483647L) usage(argv[0], 1);
+    return (int)v;
+  };
   for (int i = 1; i < argc; ++i) {
     std::string s = argv[i];
     if (s == "--help" || s == "-h") usage(argv[0], 0);
     else if (s == "--pfile" || s == "-f") a.pfile = need(i);
     else if (s == "--qfile" || s == "-q") a.qfile = need(i);
-    else if (s == "--knn" || s == "-k") a.knn = (unsigned)std::atoi(need(i));
+    else if (s == "--knn" || s == "-k") {
+      const int v = num_i(i);
+      if (v < 1) usage(argv[0], 1);
+      a.knn = (unsigned)v;
+    }
     else if (s == "--searchmode" || s == "-sm") {
       a.mode = need(i);
       if (a.mode != "knn" && a.mode != "radius") usage(argv[0], 1);
-    } else if (s == "--radius" || s == "-r") a.radius = std::stof(need(i));
-    else if (s == "--device" || s == "-d") a.device = std::atoi(need(i));
-    else if (s == "--check" || s == "-c") a.check = std::atoi(need(i)) != 0;
-    else if (s == "--partition" || s == "-p") a.partition = std::atoi(need(i)) != 0;
-    else if (s == "--approx" || s == "-a") a.approx = std::atoi(need(i));
+    } else if (s == "--radius" || s == "-r") a.radius = num_f(i);
+    else if (s == "--device" || s == "-d") a.device = num_i(i);
+    else if (s == "--check" || s == "-c") a.check = num_i(i) != 0;
+    else if (s == "--partition" || s == "-p") a.partition = num_i(i) != 0;
+    else if (s == "--approx" || s == "-a") a.approx = num_i(i);
     else if (s == "--gassort" || s == "-s") {
-      int v = std::atoi(need(i));
+      int v = num_i(i);
       if (v < 0 || v > 2) usage(argv[0], 1);
     } else if (s == "--gsrRatio" || s == "-sg") {
-      if (!(std::stof(need(i)) > 0)) usage(argv[0], 1);
+      if (!(num_f(i) > 0)) usage(argv[0], 1);
     } else if (s == "--estgassize" || s == "-egs") {
-      if (std::stof(need(i)) < 0) usage(argv[0], 1);
+      if (num_f(i) < 0) usage(argv[0], 1);
     } else if (s == "--crStep" || s == "--crStep " || s == "-crs") {
-      if (!(std::stof(need(i)) > 1)) {
+      if (!(num_f(i) > 1)) {
         std::fprintf(stderr, "crStep has to be greater than 1.\n");
         usage(argv[0], 1);
       }
     } else if (s == "--filterQueries" || s == "-fq") {
-      a.filter_queries = std::atoi(need(i)) != 0;
+      a.filter_queries = num_i(i) != 0;
     } else if (s == "--autobatch" || s == "-ab") {
-      a.autobatch = std::atoi(need(i)) != 0;
+      a.autobatch = num_i(i) != 0;
     } else if (s == "--numbatch" || s == "-nb") {
-      a.numbatch = std::atoi(need(i));
+      a.numbatch = num_i(i);
     } else if (s == "--msr" || s == "-m" || s == "--deferFree" || s == "-df" ||
                s == "--interleave" || s == "-i" || s == "--pointsort" || s == "-ps" ||
                s == "--querysort" || s == "-qs" ||
@@ -150,16 +171,16 @@ Args parse(int argc, char** argv) {
                s == "--metacellScale" || s == "-mc") {
       need(i);  // scheduling knobs of the BVH pipeline: no effect on results here
     } else if (s == "--crratio" || s == "-cr") {
-      a.cr_ratio = std::stof(need(i));
+      a.cr_ratio = num_f(i);
       if (!(a.cr_ratio > 0)) usage(argv[0], 1);
-    } else if (s == "--autocrratio" || s == "-ac") a.autotune = std::atoi(need(i)) != 0;
+    } else if (s == "--autocrratio" || s == "-ac") a.autotune = num_i(i) != 0;
     else if (s == "--boundary") a.boundary = need(i);
     else if (s == "-o" || s == "--out") a.out = need(i);
-    else if (s == "--learned") a.learned = std::atoi(need(i)) != 0;
+    else if (s == "--learned") a.learned = num_i(i) != 0;
     else if (s == "--mlp-in") a.mlp_in = need(i);
     else if (s == "--mlp-out") a.mlp_out = need(i);
-    else if (s == "--cell-factor") a.cell_factor = std::stof(need(i));
-    else if (s == "--qpw") a.qpw = std::atoi(need(i));
+    else if (s == "--cell-factor") a.cell_factor = num_f(i);
+    else if (s == "--qpw") a.qpw = num_i(i);
     else {
       std::fprintf(stderr, "Unknown option '%s'\n", argv[i]);
       usage(argv[0], 1);
@@ -317,6 +338,8 @@ int main(int argc, char** argv) {
   }
   rtnn_ctx ctx = nullptr;
   check_rc(nullptr, rtnn_create(a.device, &ctx), "rtnn_create");
+  // every return from here on releases the context
+  std::unique_ptr<rtnn_ctx_s, int (*)(rtnn_ctx)> ctx_guard(ctx, &rtnn_destroy);
   rtnn_options opt{};
   opt.cell_factor = a.cell_factor > 0 ? a.cell_factor : a.cr_ratio;
   opt.queries_per_wave = a.qpw;
@@ -337,11 +360,12 @@ int main(int argc, char** argv) {
     if (!a.mlp_in.empty()) {
       std::vector<float> blob(RTNN_MLP_NBLOB);
       FILE* f = std::fopen(a.mlp_in.c_str(), "rb");
-      if (!f || std::fread(blob.data(), 4, blob.size(), f) != blob.size()) {
+      const bool ok = f && std::fread(blob.data(), 4, blob.size(), f) == blob.size();
+      if (f) std::fclose(f);
+      if (!ok) {
         std::fprintf(stderr, "cannot read MLP weights %s\n", a.mlp_in.c_str());
         return 1;
       }
-      std::fclose(f);
       check_rc(ctx, rtnn_mlp_set_weights(ctx, blob.data()), "rtnn_mlp_set_weights");
     } else {
       Timer t;
@@ -355,7 +379,10 @@ int main(int argc, char** argv) {
       std::vector<float> blob(RTNN_MLP_NBLOB);
       check_rc(ctx, rtnn_mlp_get_weights(ctx, blob.data()), "rtnn_mlp_get_weights");
       FILE* f = std::fopen(a.mlp_out.c_str(), "wb");
-      if (!f) return 1;
+      if (!f) {
+        std::fprintf(stderr, "cannot write %s\n", a.mlp_out.c_str());
+        return 1;
+      }
       std::fwrite(blob.data(), 4, blob.size(), f);
       std::fclose(f);
     }
@@ -452,6 +479,5 @@ int main(int argc, char** argv) {
     std::printf("[MEM_STATS] peak device %.3f MB, peak pinned host %.3f MB\n",
                 peak / 1048576.0, pin_peak / 1048576.0);
   }
-  rtnn_destroy(ctx);
   return code;
 }

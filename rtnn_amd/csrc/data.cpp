@@ -100,7 +100,8 @@ int rtnn_read_points(const char* path, float* xyz, uint64_t* n) {
 
 int rtnn_read_point_sets(const char* path, float* xyz, uint64_t* n, uint32_t* nsets) {
   if (!path || !n || !nsets) return -3;
-  const uint64_t n_in = *n;  // second call: the line count of the first
+  const uint64_t n_in = *n;       // second call: the line count of the first
+  const uint32_t sets_in = *nsets;  // and its set count (the buffer's size)
   FILE* f = std::fopen(path, "rb");
   if (!f) return -2;
   std::vector<char> line(1 << 16);
@@ -137,7 +138,8 @@ int rtnn_read_point_sets(const char* path, float* xyz, uint64_t* n, uint32_t* ns
     }
     if (rc) { *n = count + 1; break; }
     if (xyz) {
-      if (count >= n_in) { rc = -4; break; }  // the file grew since the first call
+      // the file grew, or its lines widened, since the first call
+      if (count >= n_in || sets > sets_in) { rc = -4; break; }
       vals.resize(3 * (size_t)sets, 0.0f);  // a short first line is padded with zeros
       for (uint32_t b = 0; b < sets; ++b)
         for (int a = 0; a < 3; ++a) xyz[(b * n_in + count) * 3 + a] = vals[3 * b + a];
@@ -146,7 +148,7 @@ int rtnn_read_point_sets(const char* path, float* xyz, uint64_t* n, uint32_t* ns
   }
   std::fclose(f);
   if (rc == 0) {
-    if (xyz && count != n_in) return -4;  // the file changed between the two calls
+    if (xyz && (count != n_in || sets != sets_in)) return -4;  // the file changed
     *n = count;
     *nsets = sets;
   }
