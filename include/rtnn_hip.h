@@ -80,6 +80,13 @@ typedef struct {
                              (chunk by chunk, 32 points per bounding box) instead
                              of descended into; 0 = 256.  Large values suit very
                              clustered points (Levy flights: 65536)            */
+  float bucket_ratio;     /* learned/partitioned KNN, density buckets: a query whose
+                             proposed radius r' >= bucket_ratio * r is searched at
+                             r at once (its row is final after the first launch:
+                             no search at r' and no fallback), and the first
+                             launch takes the queries bucket by bucket (r' ones,
+                             then r ones) so a wave holds one bucket; 0 = off
+                             (every query at its r', processing order)          */
 } rtnn_options;
 
 /* Filled by search entry points when a non-NULL pointer is passed. */

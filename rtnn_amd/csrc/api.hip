@@ -227,6 +227,7 @@ struct rtnn_ctx_s {
 
   // radius proposals (learned / partitioned)
   DevBuf pgrid, pcounts, qradius;
+  DevBuf border;  // density buckets: processing positions bucket by bucket + 2 counts
   DevBuf ovf;  // pooled kernel: counter + rows whose hit buffers overflowed
   GridDesc hpgrid{};
   DevBuf tr_q, tr_idx, tr_d2, tr_t, tr_f, tr_out;
@@ -780,6 +781,8 @@ int check_options(rtnn_ctx ctx, const rtnn_options* opt) {
   if (!(opt->level_alpha >= 0.0f) || !std::isfinite(opt->level_alpha))
     return fail(ctx, RTNN_ERR_INVALID, "level_alpha must be >= 0");
   if (opt->tree_leaf < 0) return fail(ctx, RTNN_ERR_INVALID, "tree_leaf must be >= 0");
+  if (!(opt->bucket_ratio >= 0.0f && opt->bucket_ratio <= 1.0f))
+    return fail(ctx, RTNN_ERR_INVALID, "bucket_ratio must be in [0, 1]");
   return RTNN_OK;
 }
 
@@ -1069,7 +1072,8 @@ enum class Proposal { kLearned, kPartition };
 // queries in processing order (*qproc) and their launch radius in
 // ctx->qradius; ctx->ev[1] is recorded after the grid and query sort.
 int compute_proposals(rtnn_ctx ctx, Proposal kind, const float* qxyz, uint64_t nq, int on_device,
-                      float r, uint32_t k, const float4** qproc_out) {
+                      float r, uint32_t k, const float4** qproc_out,
+                      const uint32_t** order_out = nullptr) {
   const float f = ctx->opt.cell_factor > 0 ? ctx->opt.cell_factor
                                            : (kind == Proposal::kLearned ? 2.0f : 4.0f);
   const int levels = ctx->opt.grid_levels > 0 ? ctx->opt.grid_levels
@@ -1116,6 +1120,17 @@ int compute_proposals(rtnn_ctx ctx, Proposal kind, const float* qxyz, uint64_t n
                         ctx->pcounts.as<uint32_t>(), k, r, ctx->qradius.as<float>(), nullptr, 0,
                         ctx->st));
   }
+  if (order_out) *order_out = nullptr;
+  if (ctx->opt.bucket_ratio > 0.0f) {
+    // Density buckets: radii >= bucket_ratio * r become r, and the first
+    // launch takes the queries bucket by bucket.
+    CK(ctx->border.ensure(4 * (nq + 2)));
+    uint32_t* order = ctx->border.as<uint32_t>();
+    CK(fill_u32(order + nq, 2, 0, ctx->st));
+    CK(bucket_split(ctx->qradius.as<float>(), (uint32_t)nq, ctx->opt.bucket_ratio * r, r, order,
+                    order + nq, ctx->st));
+    if (order_out) *order_out = order;
+  }
   *qproc_out = qproc;
   return RTNN_OK;
 }
@@ -1142,7 +1157,8 @@ int proposal_search(rtnn_ctx ctx, Proposal kind, const float* qxyz, uint64_t nq,
   CK(hipSetDevice(ctx->device));
   CK(hipEventRecord(ctx->ev[0], ctx->st));
   const float4* qproc = nullptr;
-  rc = compute_proposals(ctx, kind, qxyz, nq, on_device, r, k, &qproc);
+  const uint32_t* order = nullptr;
+  rc = compute_proposals(ctx, kind, qxyz, nq, on_device, r, k, &qproc, &order);
   if (rc) return rc;
   CK(hipEventRecord(ctx->ev[2], ctx->st));
   OutBufs o;
@@ -1156,12 +1172,13 @@ int proposal_search(rtnn_ctx ctx, Proposal kind, const float* qxyz, uint64_t nq,
   uint32_t* counters = lists[0] + 4 * nq;
   CK(fill_u32(counters, kGrowStages + 1, 0, ctx->st));
   const int gl = group_lanes_from_opt(ctx, RTNN_MODE_KNN, k);
-  const bool pooled = pool_requested(ctx, k) && ctx->hlevels == 0;
+  const bool pooled = pool_requested(ctx, k) && ctx->hlevels == 0 && !order;
   if (!pooled && (gl < 0 || k > max_k_for_group(gl)))
     return fail(ctx, RTNN_ERR_UNSUPPORTED, "k=%u not supported by the selected kernel", k);
   // Stage 1: every query at its proposed radius r'.
   SearchArgs a = base_args(ctx, qproc, nq, r, k, o);
   a.qradius = ctx->qradius.as<float>();
+  a.qsel = order;  // bucket by bucket (density buckets), else processing order
   a.short_rows = lists[0];
   a.short_count = &counters[0];
   a.short_hits = hits[0];
@@ -1612,11 +1629,14 @@ int build_batched(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets, uint3
   // Per-cloud bbox on the device, plan on the host.
   CK(ctx->offs.ensure(8 * (nclouds + 1)));
   CK(hipMemcpyAsync(ctx->offs.p, offsets, 8 * (nclouds + 1), hipMemcpyHostToDevice, ctx->st));
-  CK(ctx->bboxes.ensure(24 * nclouds + 64));
+  CK(ctx->bboxes.ensure(24 * nclouds + 64 + (nclouds == 1 ? 4 * 6 * kBboxBlocks : 0)));
   uint32_t* bad = reinterpret_cast<uint32_t*>(ctx->bboxes.as<float>() + 6 * nclouds);
   CK(fill_u32(bad, 1, 0, ctx->st));
-  CK(bbox_segmented(xyz, ctx->offs.as<uint64_t>(), nclouds, ctx->bboxes.as<float>(), bad,
-                    ctx->st));
+  if (nclouds == 1)  // one cloud: the multi-block reduction (partials after the box)
+    CK(bbox(xyz, n, ctx->bboxes.as<float>() + 16, ctx->bboxes.as<float>(), bad, ctx->st));
+  else
+    CK(bbox_segmented(xyz, ctx->offs.as<uint64_t>(), nclouds, ctx->bboxes.as<float>(), bad,
+                      ctx->st));
   std::vector<float> hb(6 * nclouds + 1);
   CK(hipMemcpyAsync(hb.data(), ctx->bboxes.p, 4 * (6 * nclouds + 1), hipMemcpyDeviceToHost,
                     ctx->st));
@@ -1699,8 +1719,20 @@ int opt_lanes(const rtnn_options& o, int mode, uint32_t k) {
 // with size classes one per class over its tile range with its options.
 hipError_t launch_batched(const SearchArgs& a, int mode, uint32_t k, int num_cus,
                           const rtnn_options& o, const std::vector<uint32_t>& ctiles,
-                          const std::vector<rtnn_options>& copt, hipStream_t st) {
+                          const std::vector<rtnn_options>& copt, hipStream_t st,
+                          bool one_cloud = false) {
   const bool knn = mode == RTNN_MODE_KNN;
+  if (copt.empty() && one_cloud) {
+    // One cloud: the plain launch (implicit tiles over all its points, as
+    // many blocks as the queries fill) instead of the persistent walk of the
+    // batch's 256-query tile list, which leaves CUs idle on small batches.
+    SearchArgs b = a;
+    b.tiles = nullptr;
+    b.ntiles = 0;
+    b.persistent = 0;
+    return launch_search(b, knn, opt_lanes(o, mode, k), st, num_cus, o.group_buffer != 1,
+                         o.lds_tile == 1 ? 512 : o.lds_tile);
+  }
   if (copt.empty())
     return launch_search(a, knn, opt_lanes(o, mode, k), st, num_cus, o.group_buffer != 1,
                          o.lds_tile == 1 ? 512 : o.lds_tile);
@@ -1851,7 +1883,7 @@ int rtnn_search_batched(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets,
   OutBufs o{out_idx, out_d2};
   SearchArgs a = batched_args(ctx, n, r, k, o);
   CK(launch_batched(a, mode, k, ctx->num_cus, ctx->opt, ctx->class_tiles, ctx->class_opt,
-                    ctx->st));
+                    ctx->st, nclouds == 1));
   if (timing) CK(hipEventRecord(ctx->ev[2], ctx->st));
   CK(hipStreamSynchronize(ctx->st));
   if (stats) {
@@ -2072,6 +2104,33 @@ int rtnn_autotune(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_device, i
       }
     }
     ctx->mlp.margin = best_m;
+  }
+  if (!plain) {
+    // Density buckets: which proposed radii are close enough to r that the
+    // query is searched at r at once (measured with the chosen configuration).
+    const float b0 = best_opt.bucket_ratio;
+    for (float b : {1.0f, 0.9f, 0.8f, 0.7f, 0.6f, 0.5f}) {
+      if (b == b0) continue;
+      ctx->opt.bucket_ratio = b;
+      double ms = 0.0;
+      const int rc = time_candidate(
+          [&](rtnn_stats* st) {
+            return proposal_search(ctx, method == RTNN_METHOD_LEARNED ? Proposal::kLearned
+                                                                       : Proposal::kPartition,
+                                   qxyz, nq, on_device, r, k, out_idx, out_d2, 1, st);
+          },
+          ms);
+      if (rc != RTNN_OK) {
+        ctx->opt = base;
+        return rc;
+      }
+      tune_log("autotune %s: bucket ratio %g: %.4f ms", method_name(method), b, ms);
+      if (ms < best_t) {
+        best_t = ms;
+        best_opt.bucket_ratio = b;
+      }
+    }
+    ctx->opt = best_opt;
   }
   if (best) *best = best_opt;
   if (best_ms) *best_ms = best_t;
@@ -2328,7 +2387,8 @@ hipError_t enqueue_batched(rtnn_graph g, const float* xyz, float cell, int mode,
   a.out_d2 = out_d2;
   a.local_index = 1;
   a.persistent = 1;
-  return launch_batched(a, mode, k, g->ctx->num_cus, g->opt, g->class_tiles, g->class_opt, st);
+  return launch_batched(a, mode, k, g->ctx->num_cus, g->opt, g->class_tiles, g->class_opt, st,
+                        g->nclouds == 1);
 }
 
 }  // namespace

@@ -295,13 +295,22 @@ hipError_t cell_keys(const float* xyz, uint64_t n, const GridDesc* grids,
 // cell_start[c] = first sorted position whose key (>> shift) is >= c, for c
 // in [0, ncells] (entry ncells holds n), so a cell's points are
 // [cell_start[c], cell_start[c+1]) and no counts, atomics or scan are needed.
-// The cells are cut into tiles of kCellTile: k_tile_bounds finds each tile's
-// first sorted position by binary search (one thread per tile), then one
-// block per tile writes its cells, each a lower_bound over the tile's keys
-// staged in LDS.  Every block writes exactly one tile, however the points are
-// spread (empty space costs the same as occupied space).
-constexpr uint32_t kCellTile = 4096;
+// The cells are cut into tiles: k_tile_bounds finds each tile's first sorted
+// position by binary search (one thread per tile), then one block per tile
+// writes its cells, each a lower_bound over the tile's keys staged in LDS.
+// Every block writes exactly one tile, however the points are spread (empty
+// space costs the same as occupied space).  Tiles hold 4096 cells, fewer for
+// small tables (down to 256) so that at least ~1024 blocks share the work:
+// a 125k-cell table in 31 blocks of 4096 left most CUs idle.
+constexpr uint32_t kCellTileMax = 4096;
+constexpr uint32_t kCellTileMin = 256;
 constexpr uint32_t kCellTileKeys = 4096;  // keys staged in LDS per tile
+
+uint32_t cell_tile(uint64_t ncells1) {
+  uint32_t t = kCellTileMax;
+  while (t > kCellTileMin && ncells1 / t < 1024) t >>= 1;
+  return t;
+}
 
 __device__ inline uint64_t lower_bound_shifted(const uint32_t* __restrict__ keys, uint64_t lo,
                                                uint64_t hi, uint64_t c, int shift) {
@@ -313,28 +322,29 @@ __device__ inline uint64_t lower_bound_shifted(const uint32_t* __restrict__ keys
 }
 
 __global__ __launch_bounds__(256) void k_tile_bounds(const uint32_t* __restrict__ keys, uint64_t n,
-                                                     uint64_t ncells1, uint32_t ntiles, int shift,
+                                                     uint64_t ncells1, uint32_t ntiles,
+                                                     uint32_t tile, int shift,
                                                      uint64_t* __restrict__ tb) {
   const uint32_t t = blockIdx.x * 256 + threadIdx.x;
   if (t > ntiles) return;
-  const uint64_t c = min((uint64_t)t * kCellTile, ncells1);
+  const uint64_t c = min((uint64_t)t * tile, ncells1);
   tb[t] = lower_bound_shifted(keys, 0, n, c, shift);
 }
 
 __global__ __launch_bounds__(256) void k_cell_fill(const uint32_t* __restrict__ keys,
-                                                   uint64_t ncells1, int shift,
+                                                   uint64_t ncells1, uint32_t tile, int shift,
                                                    const uint64_t* __restrict__ tb,
                                                    uint32_t* __restrict__ cs) {
   __shared__ uint32_t s_keys[kCellTileKeys];
   const uint32_t t = blockIdx.x;
-  const uint64_t c0 = (uint64_t)t * kCellTile;
+  const uint64_t c0 = (uint64_t)t * tile;
   const uint64_t j0 = tb[t], j1 = tb[t + 1];
   const uint64_t m = j1 - j0;
   const bool staged = m <= kCellTileKeys;
   if (staged)
     for (uint32_t i = threadIdx.x; i < m; i += 256) s_keys[i] = keys[j0 + i] >> shift;
   __syncthreads();
-  for (uint32_t i = threadIdx.x; i < kCellTile; i += 256) {
+  for (uint32_t i = threadIdx.x; i < tile; i += 256) {
     const uint64_t c = c0 + i;
     if (c >= ncells1) break;
     uint64_t pos;
@@ -353,18 +363,20 @@ __global__ __launch_bounds__(256) void k_cell_fill(const uint32_t* __restrict__ 
 }
 
 uint64_t cell_bounds_tmp_words(uint64_t ncells) {
-  return 2 * ((ncells + 1 + kCellTile - 1) / kCellTile + 1) + 2;
+  const uint32_t tile = cell_tile(ncells + 1);
+  return 2 * ((ncells + 1 + tile - 1) / tile + 1) + 2;
 }
 
 hipError_t cell_bounds(const uint32_t* sorted_keys, uint64_t n, uint32_t ncells,
                        uint32_t* cell_start, uint32_t* tmp, hipStream_t st, int shift) {
   const uint64_t ncells1 = (uint64_t)ncells + 1;
-  const uint32_t ntiles = (uint32_t)((ncells1 + kCellTile - 1) / kCellTile);
+  const uint32_t tile = cell_tile(ncells1);
+  const uint32_t ntiles = (uint32_t)((ncells1 + tile - 1) / tile);
   uint64_t* tb = reinterpret_cast<uint64_t*>(tmp);
   hipLaunchKernelGGL(k_tile_bounds, dim3((ntiles + 1 + 255) / 256), dim3(256), 0, st,
-                     sorted_keys, n, ncells1, ntiles, shift, tb);
-  hipLaunchKernelGGL(k_cell_fill, dim3(ntiles), dim3(256), 0, st, sorted_keys, ncells1, shift,
-                     tb, cell_start);
+                     sorted_keys, n, ncells1, ntiles, tile, shift, tb);
+  hipLaunchKernelGGL(k_cell_fill, dim3(ntiles), dim3(256), 0, st, sorted_keys, ncells1, tile,
+                     shift, tb, cell_start);
   return hipGetLastError();
 }
 

@@ -264,3 +264,46 @@ def test_levy_uniform_queries_k32_large_radius(ctx):
     wi, wd = oracle.bruteforce(pts, qs, oracle.MODE_KNN, 100.0, 32, qsel=sel, threads=8)
     s = sel.astype(np.int64)
     assert np.array_equal(pi[s], wi) and np.array_equal(pd[s], wd)
+
+
+@pytest.mark.parametrize("ratio", [1.0, 0.7, 0.3])
+def test_density_buckets_exact(ctx, ratio):
+    # Density buckets (bucket_ratio): proposals r' >= ratio * r are searched at
+    # r at once and the first launch takes the queries bucket by bucket.  Rows
+    # equal the plain KNN for both proposal methods, on the raster grid, the
+    # hierarchical grid's cube walks and the octree walk; the radii
+    # rtnn_propose_radius reports are the proposals with that rule applied.
+    pts = rtnn_amd.gen_levy(5, 300_000, alpha=1.5, min_step=0.01, box=300.0)
+    qs = rtnn_amd.gen_uniform(52, 40_000, 0.0, 300.0)
+    ctx.set_points(pts)
+    r, k = 20.0, 16
+    ctx.mlp_train(r, k, steps=200, batch=2048, seed=4, queries=qs[:10000])
+    pi, pd, _ = ctx.search(qs, mode="knn", radius=r, k=k)
+    try:
+        for opts in ({"grid_levels": 1}, {"grid_levels": 5, "queries_per_wave": 16},
+                     {"grid_levels": 5, "queries_per_wave": 0}):
+            for method, fn in (("learned", ctx.search_learned),
+                               ("partitioned", ctx.search_partitioned)):
+                ctx.set_options(**opts)
+                want = ctx.propose_radius(qs, method, r, k)
+                ctx.set_options(bucket_ratio=ratio, **opts)
+                got = ctx.propose_radius(qs, method, r, k)
+                snap = want >= np.float32(ratio) * np.float32(r)
+                assert np.array_equal(got[snap], np.full(int(snap.sum()), r, np.float32))
+                assert np.array_equal(got[~snap], want[~snap])
+                li, ld, st = fn(qs, radius=r, k=k)
+                assert np.array_equal(pi, li) and np.array_equal(pd, ld), (opts, method)
+                if ratio <= 0.3:
+                    # most proposals land in the sparse bucket: few rows left
+                    # short after the first launch
+                    assert st.fallback_queries <= int((~snap).sum()), (opts, method)
+    finally:
+        ctx.set_options()
+
+
+def test_bucket_ratio_validated(ctx):
+    with pytest.raises(rtnn_amd.RtnnError):
+        ctx.set_options(bucket_ratio=1.5)
+    with pytest.raises(rtnn_amd.RtnnError):
+        ctx.set_options(bucket_ratio=-0.1)
+    ctx.set_options()
