@@ -89,6 +89,10 @@ def workload(name):
                     qs=rtnn_amd.gen_uniform(51, 30_000_000, 0.0, 1000.0), mode="knn", r=100.0,
                     k=32, methods=["partitioned", "learned"], train_on_queries=True,
                     autotune=False,  # each candidate search at r = 100 takes seconds
+                    # instead: the learned method's radius margin and density
+                    # bucket, timed on 3M of the queries (about 0.1 s each)
+                    proposal_tune={"sample": 3_000_000, "margins": [1.0, 1.5, 2.0, 3.0],
+                                   "buckets": [0.0, 0.8]},
                     # the octree walk of a 7-level hierarchical grid (fixed
                     # configuration, not tuned)
                     # leaves of up to 65536 points, scanned chunk by chunk
@@ -206,6 +210,40 @@ def pmc_row(workload_name, kernel):
     return None
 
 
+def proposal_tune(ctx, dq, r, k, spec, base_opts):
+    """Times the learned search on a sample of the queries for each radius
+    margin (times the trained one) and density-bucket ratio, one log line per
+    candidate, and keeps the fastest (options and weights in the context)."""
+    import torch
+    t0 = time.perf_counter()
+    sub = dq[:spec["sample"]]
+    w0 = ctx.mlp_weights().copy()
+    best = None
+    for f in spec["margins"]:
+        w = w0.copy()
+        w[-1] = w0[-1] * f
+        ctx.set_mlp_weights(w)
+        for b in spec["buckets"]:
+            ctx.set_options(**base_opts, bucket_ratio=b)
+            ctx.search_learned(sub, radius=r, k=k, out_device=True)
+            ms = min(ctx.search_learned(sub, radius=r, k=k, out_device=True)[2].ms_total
+                     for _ in range(2))
+            log(f"[bench] proposal tune: margin x{f} bucket {b}: {ms:.2f} ms on "
+                f"{len(sub)} queries")
+            if best is None or ms < best[0]:
+                best = (ms, f, b)
+    torch.cuda.empty_cache()
+    ms, f, b = best
+    w = w0.copy()
+    w[-1] = w0[-1] * f
+    ctx.set_mlp_weights(w)
+    ctx.set_options(**base_opts, bucket_ratio=b)
+    return {**base_opts, "radius_margin": round(float(w[-1]), 4), "margin_factor": f,
+            "bucket_ratio": b,
+            "sample_queries": len(sub), "sample_ms": round(ms, 3),
+            "seconds": round(time.perf_counter() - t0, 3)}
+
+
 def kernel_name(res, mode, opts):
     """Name of the dominant search kernel of a method's configuration."""
     o = opts or {}
@@ -314,6 +352,8 @@ def measure(args, wname, torch, rtnn_amd, extra=False):
             log(f"[bench] graph: captured with {graph_cfg}")
             graph = ctx.graph_batched(dp, goffs, mode=mode, radius=r, k=k,
                                       with_d2=bool(w.get("graph_d2")))
+        if m == "learned" and w.get("proposal_tune") and fixed_opts is None:
+            tuned = proposal_tune(ctx, dq, r, k, w["proposal_tune"], w.get("options", {}))
         if autotune and w.get("autotune", True) and m in ("plain", "partitioned", "learned"):
             t0 = time.perf_counter()
             opt, tms = ctx.autotune(dq, method=m, mode=mode, radius=r, k=k)

@@ -83,10 +83,12 @@ typedef struct {
   float bucket_ratio;     /* learned/partitioned KNN, density buckets: a query whose
                              proposed radius r' >= bucket_ratio * r is searched at
                              r at once (its row is final after the first launch:
-                             no search at r' and no fallback), and the first
-                             launch takes the queries bucket by bucket (r' ones,
-                             then r ones) so a wave holds one bucket; 0 = off
-                             (every query at its r', processing order)          */
+                             no search at r' and no fallback); 0 = off          */
+  int   learned_cap;      /* learned KNN: 1 = the network's r' is capped by the
+                             density partition's radius (rtnn_search_partitioned's
+                             first radius, which holds k neighbours), so queries
+                             the partition bounds more tightly keep its radius
+                             and do not fall back; 0 = the network alone        */
 } rtnn_options;
 
 /* Filled by search entry points when a non-NULL pointer is passed. */
@@ -236,6 +238,20 @@ int rtnn_mem_stats(uint64_t* device_bytes, uint64_t* device_peak, uint64_t* pinn
 int rtnn_search_stream(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int mode, int method,
                        float r, uint32_t k, uint64_t chunk, int32_t* out_idx, float* out_d2,
                        const char* out_path, rtnn_stats* stats);
+/* Streamed batched search: host clouds xyz (offsets[nclouds + 1] as in
+ * rtnn_search_batched) are searched in chunks of whole clouds of about
+ * `chunk` points (0: 4M; a larger cloud is a chunk of its own) through the
+ * two slots of rtnn_search_stream, so one chunk's rows travel over PCIe while
+ * the next chunk is searched.  Rows (global rows, cloud-local point indices)
+ * go to host arrays out_idx[n][k] (+ out_d2, nullable) or to a result file at
+ * out_path in the same format as rtnn_search_stream's.  The rows equal one
+ * rtnn_search_batched over the batch; size classes apply per cloud.
+ * Replaces the reference's per-batch copy of rows to pinned host memory
+ * (search.cpp:43-56) for batches of clouds. */
+int rtnn_search_batched_stream(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets,
+                               uint32_t nclouds, int mode, float r, uint32_t k, uint64_t chunk,
+                               int32_t* out_idx, float* out_d2, const char* out_path,
+                               rtnn_stats* stats);
 
 /* The first-stage launch radius of each query (input order) under method
  * RTNN_METHOD_PARTITIONED (the megacell rule, sort.cpp:366-371 with approx

@@ -53,14 +53,16 @@ class Options(ctypes.Structure):
                 ("group_buffer", ctypes.c_int),
                 ("lds_tile", ctypes.c_int),
                 ("tree_leaf", ctypes.c_int),
-                ("bucket_ratio", ctypes.c_float)]
+                ("bucket_ratio", ctypes.c_float),
+                ("learned_cap", ctypes.c_int)]
 
 
 def _options_struct(cell_factor=0.0, queries_per_wave=0, query_order=0, time_stages=0,
                     max_cells=0, grid_levels=0, level_alpha=0.0, group_buffer=0, lds_tile=0,
-                    tree_leaf=0, bucket_ratio=0.0):
+                    tree_leaf=0, bucket_ratio=0.0, learned_cap=0):
     return Options(cell_factor, queries_per_wave, query_order, time_stages, max_cells,
-                   grid_levels, level_alpha, group_buffer, lds_tile, tree_leaf, bucket_ratio)
+                   grid_levels, level_alpha, group_buffer, lds_tile, tree_leaf, bucket_ratio,
+                   learned_cap)
 
 
 def _options_dict(o):
@@ -120,6 +122,8 @@ SIGNATURES = {
     "rtnn_propose_radius": (_I, [_VP, _VP, _U64, _I, _I, _F, _U32, _VP, _I]),
     "rtnn_search_stream": (_I, [_VP, _VP, _U64, _I, _I, _F, _U32, _U64, _VP, _VP,
                                 ctypes.c_char_p, ctypes.POINTER(Stats)]),
+    "rtnn_search_batched_stream": (_I, [_VP, _VP, _VP, _U32, _I, _F, _U32, _U64, _VP, _VP,
+                                        ctypes.c_char_p, ctypes.POINTER(Stats)]),
     "rtnn_mem_stats": (_I, [ctypes.POINTER(_U64)] * 4),
     "rtnn_search_batched": (_I, [_VP, _VP, _VP, _U32, _I, _F, _U32, _VP, _VP,
                                  ctypes.POINTER(Stats)]),
@@ -260,9 +264,10 @@ class Context:
 
     def set_options(self, cell_factor=0.0, queries_per_wave=0, query_order=0,
                     time_stages=0, max_cells=0, grid_levels=0, level_alpha=0.0,
-                    group_buffer=0, lds_tile=0, tree_leaf=0, bucket_ratio=0.0):
+                    group_buffer=0, lds_tile=0, tree_leaf=0, bucket_ratio=0.0, learned_cap=0):
         o = Options(cell_factor, queries_per_wave, query_order, time_stages, max_cells,
-                    grid_levels, level_alpha, group_buffer, lds_tile, tree_leaf, bucket_ratio)
+                    grid_levels, level_alpha, group_buffer, lds_tile, tree_leaf, bucket_ratio,
+                    learned_cap)
         self._check(self._L.rtnn_set_options(self._h, ctypes.byref(o)))
 
     def synchronize(self):
@@ -468,6 +473,29 @@ class Context:
     def search_batched(self, xyz, offsets, mode="radius", radius=0.2, k=64, with_d2=True):
         return self._batched(self._L.rtnn_search_batched, xyz, offsets, mode, radius, k,
                              with_d2)
+
+    def search_batched_stream(self, xyz, offsets, mode="radius", radius=0.2, k=64, chunk=0,
+                              with_d2=True, path=None):
+        """Host clouds in, host rows out (or a result file at `path`), searched in
+        chunks of whole clouds with the copies of one chunk overlapping the
+        search of the next -> (idx, d2 or None, Stats); (None, None, Stats)
+        with a path."""
+        x = np.ascontiguousarray(np.asarray(xyz, dtype=np.float32).reshape(-1, 3))
+        offs = np.ascontiguousarray(offsets, dtype=np.uint64)
+        n = int(offs[-1])
+        stats = Stats()
+        idx = d2 = None
+        if path is None:
+            idx = np.empty((n, k), dtype=np.int32)
+            d2 = np.empty((n, k), dtype=np.float32) if with_d2 else None
+        rc = self._L.rtnn_search_batched_stream(
+            self._h, x.ctypes.data_as(ctypes.c_void_p), offs.ctypes.data_as(ctypes.c_void_p),
+            len(offs) - 1, _mode(mode), radius, k, chunk,
+            None if idx is None else idx.ctypes.data_as(ctypes.c_void_p),
+            None if d2 is None else d2.ctypes.data_as(ctypes.c_void_p),
+            None if path is None else os.fsencode(path), ctypes.byref(stats))
+        self._check(rc)
+        return idx, d2, stats
 
     def search_percloud(self, xyz, offsets, mode="radius", radius=0.2, k=64, with_d2=True):
         return self._batched(self._L.rtnn_search_percloud, xyz, offsets, mode, radius, k,

@@ -227,7 +227,6 @@ struct rtnn_ctx_s {
 
   // radius proposals (learned / partitioned)
   DevBuf pgrid, pcounts, qradius;
-  DevBuf border;  // density buckets: processing positions bucket by bucket + 2 counts
   DevBuf ovf;  // pooled kernel: counter + rows whose hit buffers overflowed
   GridDesc hpgrid{};
   DevBuf tr_q, tr_idx, tr_d2, tr_t, tr_f, tr_out;
@@ -783,6 +782,8 @@ int check_options(rtnn_ctx ctx, const rtnn_options* opt) {
   if (opt->tree_leaf < 0) return fail(ctx, RTNN_ERR_INVALID, "tree_leaf must be >= 0");
   if (!(opt->bucket_ratio >= 0.0f && opt->bucket_ratio <= 1.0f))
     return fail(ctx, RTNN_ERR_INVALID, "bucket_ratio must be in [0, 1]");
+  if (opt->learned_cap != 0 && opt->learned_cap != 1)
+    return fail(ctx, RTNN_ERR_INVALID, "learned_cap must be 0 or 1");
   return RTNN_OK;
 }
 
@@ -1071,9 +1072,26 @@ enum class Proposal { kLearned, kPartition };
 // Grid for the proposal method (hierarchical Morton grid or one raster grid),
 // queries in processing order (*qproc) and their launch radius in
 // ctx->qradius; ctx->ev[1] is recorded after the grid and query sort.
+// The density partition's launch radius per processing position into
+// ctx->qradius (its grid already built: hierarchical or raster).
+int partition_proposals(rtnn_ctx ctx, bool hier, const float4* qproc, uint64_t nq, float r,
+                        uint32_t k) {
+  if (hier) {
+    CK(partition_hier(qproc, (uint32_t)nq, ctx->grids.as<GridDesc>(), ctx->hlevels,
+                      ctx->hlevel_min, ctx->cell_start.as<uint32_t>(), k, r,
+                      ctx->qradius.as<float>(), ctx->st));
+    return RTNN_OK;
+  }
+  const int rc = build_count_grid(ctx, r / 8.0f, true);
+  if (rc) return rc;
+  CK(partition_radius(qproc, (uint32_t)nq, ctx->pgrid.as<GridDesc>(),
+                      ctx->pcounts.as<uint32_t>(), k, r, ctx->qradius.as<float>(), nullptr, 0,
+                      ctx->st));
+  return RTNN_OK;
+}
+
 int compute_proposals(rtnn_ctx ctx, Proposal kind, const float* qxyz, uint64_t nq, int on_device,
-                      float r, uint32_t k, const float4** qproc_out,
-                      const uint32_t** order_out = nullptr) {
+                      float r, uint32_t k, const float4** qproc_out) {
   const float f = ctx->opt.cell_factor > 0 ? ctx->opt.cell_factor
                                            : (kind == Proposal::kLearned ? 2.0f : 4.0f);
   const int levels = ctx->opt.grid_levels > 0 ? ctx->opt.grid_levels
@@ -1094,6 +1112,14 @@ int compute_proposals(rtnn_ctx ctx, Proposal kind, const float* qxyz, uint64_t n
   CK(hipEventRecord(ctx->ev[1], ctx->st));
   CK(ctx->qradius.ensure(4 * nq));
   if (kind == Proposal::kLearned) {
+    // learned_cap: the density partition's radius first; the MLP's proposal
+    // is then capped by it (that radius holds k neighbours, so a query it
+    // bounds more tightly than the network never falls back).
+    const bool cap = ctx->opt.learned_cap != 0;
+    if (cap) {
+      rc = partition_proposals(ctx, hier, qproc, nq, r, k);
+      if (rc) return rc;
+    }
     // Features: point counts of the edge-r cells.  On the hierarchical grid
     // they are the top level's table (end - start); otherwise a count grid.
     const GridDesc* coarse;
@@ -1107,30 +1133,16 @@ int compute_proposals(rtnn_ctx ctx, Proposal kind, const float* qxyz, uint64_t n
       coarse = ctx->pgrid.as<GridDesc>();
       counts = ctx->pcounts.as<uint32_t>();
     }
-    CK(mlp_propose(ctx->mlp, qproc, (uint32_t)nq, coarse, hier, counts, r, ctx->qradius.as<float>(),
-                   ctx->st, ctx->num_cus));
-  } else if (hier) {
-    CK(partition_hier(qproc, (uint32_t)nq, ctx->grids.as<GridDesc>(), ctx->hlevels,
-                      ctx->hlevel_min, ctx->cell_start.as<uint32_t>(), k, r,
-                      ctx->qradius.as<float>(), ctx->st));
+    CK(mlp_propose(ctx->mlp, qproc, (uint32_t)nq, coarse, hier, counts, r, cap,
+                   ctx->qradius.as<float>(), ctx->st, ctx->num_cus));
   } else {
-    rc = build_count_grid(ctx, r / 8.0f, true);
+    rc = partition_proposals(ctx, hier, qproc, nq, r, k);
     if (rc) return rc;
-    CK(partition_radius(qproc, (uint32_t)nq, ctx->pgrid.as<GridDesc>(),
-                        ctx->pcounts.as<uint32_t>(), k, r, ctx->qradius.as<float>(), nullptr, 0,
-                        ctx->st));
   }
-  if (order_out) *order_out = nullptr;
-  if (ctx->opt.bucket_ratio > 0.0f) {
-    // Density buckets: radii >= bucket_ratio * r become r, and the first
-    // launch takes the queries bucket by bucket.
-    CK(ctx->border.ensure(4 * (nq + 2)));
-    uint32_t* order = ctx->border.as<uint32_t>();
-    CK(fill_u32(order + nq, 2, 0, ctx->st));
-    CK(bucket_split(ctx->qradius.as<float>(), (uint32_t)nq, ctx->opt.bucket_ratio * r, r, order,
-                    order + nq, ctx->st));
-    if (order_out) *order_out = order;
-  }
+  // Density buckets: proposals >= bucket_ratio * r are searched at r at once.
+  if (ctx->opt.bucket_ratio > 0.0f)
+    CK(bucket_snap(ctx->qradius.as<float>(), (uint32_t)nq, ctx->opt.bucket_ratio * r, r,
+                   ctx->st));
   *qproc_out = qproc;
   return RTNN_OK;
 }
@@ -1157,8 +1169,7 @@ int proposal_search(rtnn_ctx ctx, Proposal kind, const float* qxyz, uint64_t nq,
   CK(hipSetDevice(ctx->device));
   CK(hipEventRecord(ctx->ev[0], ctx->st));
   const float4* qproc = nullptr;
-  const uint32_t* order = nullptr;
-  rc = compute_proposals(ctx, kind, qxyz, nq, on_device, r, k, &qproc, &order);
+  rc = compute_proposals(ctx, kind, qxyz, nq, on_device, r, k, &qproc);
   if (rc) return rc;
   CK(hipEventRecord(ctx->ev[2], ctx->st));
   OutBufs o;
@@ -1172,13 +1183,12 @@ int proposal_search(rtnn_ctx ctx, Proposal kind, const float* qxyz, uint64_t nq,
   uint32_t* counters = lists[0] + 4 * nq;
   CK(fill_u32(counters, kGrowStages + 1, 0, ctx->st));
   const int gl = group_lanes_from_opt(ctx, RTNN_MODE_KNN, k);
-  const bool pooled = pool_requested(ctx, k) && ctx->hlevels == 0 && !order;
+  const bool pooled = pool_requested(ctx, k) && ctx->hlevels == 0;
   if (!pooled && (gl < 0 || k > max_k_for_group(gl)))
     return fail(ctx, RTNN_ERR_UNSUPPORTED, "k=%u not supported by the selected kernel", k);
   // Stage 1: every query at its proposed radius r'.
   SearchArgs a = base_args(ctx, qproc, nq, r, k, o);
   a.qradius = ctx->qradius.as<float>();
-  a.qsel = order;  // bucket by bucket (density buckets), else processing order
   a.short_rows = lists[0];
   a.short_count = &counters[0];
   a.short_hits = hits[0];
@@ -1197,10 +1207,16 @@ int proposal_search(rtnn_ctx ctx, Proposal kind, const float* qxyz, uint64_t nq,
   // query with wide, deep buffers (k <= 64 here; room for 2 max(k, 32) keys)
   // measured faster than one lane (C3: 1.25 -> 0.84 ms), two lanes, or the
   // k <= 16 buffer.
-  for (int j = 0; j <= kGrowStages; ++j) {
+  // The octree walk (hierarchical grid, one lane per query) costs about as
+  // much at a grown radius as at r -- its limit, not its radius, bounds the
+  // walk -- so its short rows go straight to the last stage: on the
+  // Levy-flight KNN 151 short rows took 27 ms through five stages, since
+  // each stage waits for its slowest walk.
+  const bool tree_fallback = ctx->hlevels > 0 && k <= 64;
+  for (int j = tree_fallback ? kGrowStages : 0; j <= kGrowStages; ++j) {
     SearchArgs s = base_args(ctx, qproc, nq, r, k, o);
-    s.qsel = lists[j & 1];
-    s.nq_dev = &counters[j];
+    s.qsel = tree_fallback ? lists[0] : lists[j & 1];
+    s.nq_dev = tree_fallback ? &counters[0] : &counters[j];
     if (j < kGrowStages) {
       s.qsel_hits = hits[j & 1];
       s.qradius = ctx->qradius.as<float>();
@@ -1232,7 +1248,7 @@ int proposal_search(rtnn_ctx ctx, Proposal kind, const float* qxyz, uint64_t nq,
     stats->ms_copy = out_on_device ? 0.0 : ev_ms(ctx->ev[4], ctx->ev[5]);
     stats->ms_total = ev_ms(ctx->ev[0], ctx->ev[5]);
     stats->fallback_queries = nshort;
-    stats->search_launches = 2 + kGrowStages + (pooled ? 1 : 0);
+    stats->search_launches = 2 + (tree_fallback ? 0 : kGrowStages) + (pooled ? 1 : 0);
     stats->lanes_per_query = pooled ? (uint32_t)(64 / ctx->opt.queries_per_wave) : (uint32_t)gl;
     fill_stats(ctx, stats);
   }
@@ -1288,6 +1304,19 @@ int rtnn_propose_radius(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_dev
 }
 
 // ----------------------------------------------------- streamed search --
+
+}  // extern "C"
+
+namespace {
+// Defined with the batched search below.
+int check_offsets(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets, uint32_t nclouds);
+int check_batched_lanes(rtnn_ctx ctx, int mode, uint32_t k);
+int batched_impl(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets, uint32_t nclouds,
+                 int mode, float r, uint32_t k, int32_t* out_idx, float* out_d2,
+                 rtnn_stats* stats);
+}  // namespace
+
+extern "C" {
 
 // Host queries in, host results (or a result file) out, in chunks of
 // `chunk` queries through two slots: while chunk i is searched, chunk i-1's
@@ -1401,6 +1430,139 @@ int rtnn_search_stream(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int mode, i
     sum.ms_mlp += st.ms_mlp;
     sum.ms_fallback += st.ms_fallback;
     sum.fallback_queries += st.fallback_queries;
+    sum.search_launches += st.search_launches;
+    sum.lanes_per_query = st.lanes_per_query;
+  }
+  for (uint64_t c = nchunks >= 2 ? nchunks - 2 : 0; c < nchunks && rc == RTNN_OK; ++c)
+    CK(flush(c));
+  CK(hipStreamSynchronize(ctx->st3));
+  if (f && std::fclose(fguard.release()) != 0) io_err = true;
+  if (rc) return rc;
+  if (io_err) return fail(ctx, RTNN_ERR_IO, "writing %s failed", out_path);
+  if (stats) {
+    *stats = sum;
+    fill_stats(ctx, stats);
+  }
+  return RTNN_OK;
+}
+
+// Host clouds in, host rows (or a result file) out, in chunks of whole
+// clouds of about `chunk` points through the two slots of the streamed
+// search: while chunk i is searched, chunk i-1's rows travel to pinned
+// memory on the copy stream and chunk i-2's are copied from there to the
+// caller's arrays or written to the file.  Each chunk is an ordinary
+// batched search of its clouds (rows = its points, cloud-local indices), so
+// the concatenated rows equal one batched search over all clouds.
+int rtnn_search_batched_stream(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets,
+                               uint32_t nclouds, int mode, float r, uint32_t k, uint64_t chunk,
+                               int32_t* out_idx, float* out_d2, const char* out_path,
+                               rtnn_stats* stats) {
+  if (!ctx) return RTNN_ERR_INVALID;
+  if (!out_path && !out_idx) return fail(ctx, RTNN_ERR_INVALID, "no output (out_idx or out_path)");
+  int rc = check_search_args(ctx, mode, r, k, out_path ? reinterpret_cast<const void*>(out_path)
+                                                      : out_idx);
+  if (rc) return rc;
+  rc = check_offsets(ctx, xyz, offsets, nclouds);
+  if (rc) return rc;
+  rc = check_batched_lanes(ctx, mode, k);
+  if (rc) return rc;
+  const uint64_t n = offsets[nclouds];
+  if (chunk == 0) chunk = 1ull << 22;
+  // Chunks of whole clouds: cloud c0 .. c1-1 of about `chunk` points (a
+  // cloud larger than that is a chunk of its own).
+  std::vector<uint32_t> cuts{0};
+  uint64_t maxpts = 0;
+  for (uint32_t c = 0; c < nclouds; ++c) {
+    const uint64_t start = offsets[cuts.back()];
+    if (c > cuts.back() && offsets[c + 1] - start > chunk) {
+      maxpts = std::max(maxpts, offsets[c] - start);
+      cuts.push_back(c);
+    }
+  }
+  maxpts = std::max<uint64_t>({maxpts, offsets[nclouds] - offsets[cuts.back()], 1});
+  cuts.push_back(nclouds);
+  const uint64_t nchunks = cuts.size() - 1;
+  CK(hipSetDevice(ctx->device));
+  if (!ctx->st3) {
+    CK(hipStreamCreateWithFlags(&ctx->st3, hipStreamNonBlocking));
+    for (int i = 0; i < 2; ++i) {
+      CK(hipEventCreateWithFlags(&ctx->ev_up[i], hipEventDisableTiming));
+      CK(hipEventCreateWithFlags(&ctx->ev_done[i], hipEventDisableTiming));
+      CK(hipEventCreateWithFlags(&ctx->ev_down[i], hipEventDisableTiming));
+    }
+  }
+  std::unique_ptr<FILE, int (*)(FILE*)> fguard(nullptr, &std::fclose);
+  FILE* f = nullptr;
+  const uint64_t head = 8 + 8 + 4 + 4;
+  if (out_path) {
+    f = std::fopen(out_path, "wb");
+    if (!f) return fail(ctx, RTNN_ERR_IO, "cannot open %s for writing", out_path);
+    fguard.reset(f);
+    const uint32_t flags = 1;  // d2 present
+    std::fwrite("RTNNRES1", 1, 8, f);
+    std::fwrite(&n, 8, 1, f);
+    std::fwrite(&k, 4, 1, f);
+    std::fwrite(&flags, 4, 1, f);
+  }
+  const bool want_d2 = out_path || out_d2;
+  const uint64_t rowb = 4ull * k;
+  for (int i = 0; i < 2; ++i) {
+    if (ctx->sq[i].ensure(12 * maxpts) != hipSuccess || ctx->sidx[i].ensure(rowb * maxpts) ||
+        (want_d2 && ctx->sd2[i].ensure(rowb * maxpts)) || ctx->pq[i].ensure(12 * maxpts) ||
+        ctx->pidx[i].ensure(rowb * maxpts) || (want_d2 && ctx->pd2[i].ensure(rowb * maxpts)))
+      return fail(ctx, RTNN_ERR_OOM, "streaming buffers of %llu points",
+                  (unsigned long long)maxpts);
+  }
+  rtnn_stats sum{};
+  bool io_err = false;
+  auto flush = [&](uint64_t c) -> hipError_t {
+    const int s = (int)(c & 1);
+    hipError_t e = hipEventSynchronize(ctx->ev_down[s]);
+    if (e != hipSuccess) return e;
+    const uint64_t p0 = offsets[cuts[c]], np = offsets[cuts[c + 1]] - p0;
+    if (f) {
+      io_err |= std::fseek(f, (long)(head + p0 * rowb), SEEK_SET) != 0 ||
+                std::fwrite(ctx->pidx[s].p, 1, np * rowb, f) != np * rowb;
+      io_err |= std::fseek(f, (long)(head + n * rowb + p0 * rowb), SEEK_SET) != 0 ||
+                std::fwrite(ctx->pd2[s].p, 1, np * rowb, f) != np * rowb;
+    } else {
+      std::memcpy(out_idx + p0 * k, ctx->pidx[s].p, np * rowb);
+      if (out_d2) std::memcpy(out_d2 + p0 * k, ctx->pd2[s].p, np * rowb);
+    }
+    return hipSuccess;
+  };
+  std::vector<uint64_t> sub;
+  for (uint64_t c = 0; c < nchunks && rc == RTNN_OK; ++c) {
+    const int s = (int)(c & 1);
+    const uint32_t c0 = cuts[c], nc = cuts[c + 1] - c0;
+    const uint64_t p0 = offsets[c0], np = offsets[c0 + nc] - p0;
+    if (c >= 2) CK(flush(c - 2));  // frees slot s's pinned rows
+    if (np == 0) {  // clouds without points: no rows
+      CK(hipEventRecord(ctx->ev_down[s], ctx->st3));
+      continue;
+    }
+    sub.assign(offsets + c0, offsets + c0 + nc + 1);
+    for (uint64_t& o : sub) o -= p0;
+    CK(hipEventSynchronize(ctx->ev_up[s]));
+    std::memcpy(ctx->pq[s].p, xyz + 3 * p0, 12 * np);
+    CK(hipStreamWaitEvent(ctx->st, ctx->ev_down[s], 0));  // slot s's device rows are free
+    CK(hipMemcpyAsync(ctx->sq[s].p, ctx->pq[s].p, 12 * np, hipMemcpyHostToDevice, ctx->st));
+    CK(hipEventRecord(ctx->ev_up[s], ctx->st));
+    rtnn_stats st{};
+    int32_t* di = ctx->sidx[s].as<int32_t>();
+    float* dd = want_d2 ? ctx->sd2[s].as<float>() : nullptr;
+    rc = batched_impl(ctx, ctx->sq[s].as<float>(), sub.data(), nc, mode, r, k, di, dd,
+                      stats ? &st : nullptr);
+    if (rc) break;
+    CK(hipEventRecord(ctx->ev_done[s], ctx->st));
+    CK(hipStreamWaitEvent(ctx->st3, ctx->ev_done[s], 0));
+    CK(hipMemcpyAsync(ctx->pidx[s].p, di, np * rowb, hipMemcpyDeviceToHost, ctx->st3));
+    if (want_d2)
+      CK(hipMemcpyAsync(ctx->pd2[s].p, dd, np * rowb, hipMemcpyDeviceToHost, ctx->st3));
+    CK(hipEventRecord(ctx->ev_down[s], ctx->st3));
+    sum.ms_total += st.ms_total;
+    sum.ms_grid += st.ms_grid;
+    sum.ms_search += st.ms_search;
     sum.search_launches += st.search_launches;
     sum.lanes_per_query = st.lanes_per_query;
   }
@@ -1856,6 +2018,10 @@ int time_candidate(Run run, double& best_ms) {
   return RTNN_OK;
 }
 
+int batched_impl(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets, uint32_t nclouds,
+                 int mode, float r, uint32_t k, int32_t* out_idx, float* out_d2,
+                 rtnn_stats* stats);
+
 }  // namespace
 
 extern "C" {
@@ -1867,11 +2033,24 @@ int rtnn_search_batched(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets,
   if (rc) return rc;
   rc = check_offsets(ctx, xyz, offsets, nclouds);
   if (rc) return rc;
-  const uint64_t n = offsets[nclouds];
-  if (n == 0) return RTNN_OK;
+  if (offsets[nclouds] == 0) return RTNN_OK;
   rc = check_batched_lanes(ctx, mode, k);
   if (rc) return rc;
   CK(hipSetDevice(ctx->device));
+  return batched_impl(ctx, xyz, offsets, nclouds, mode, r, k, out_idx, out_d2, stats);
+}
+
+}  // extern "C"
+
+namespace {
+
+// The batched search of rtnn_search_batched after its argument checks
+// (device points and outputs, host offsets, at least one point).
+int batched_impl(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets, uint32_t nclouds,
+                 int mode, float r, uint32_t k, int32_t* out_idx, float* out_d2,
+                 rtnn_stats* stats) {
+  const uint64_t n = offsets[nclouds];
+  int rc;
   const bool timing = ctx->opt.time_stages != 0 || stats != nullptr;
   if (timing) CK(hipEventRecord(ctx->ev[0], ctx->st));
   const float f = ctx->opt.cell_factor > 0 ? ctx->opt.cell_factor : default_cell_factor(mode);
@@ -1897,6 +2076,10 @@ int rtnn_search_batched(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets,
   }
   return RTNN_OK;
 }
+
+}  // namespace
+
+extern "C" {
 
 int rtnn_search_percloud(rtnn_ctx ctx, const float* xyz, const uint64_t* offsets,
                          uint32_t nclouds, int mode, float r, uint32_t k, int32_t* out_idx,
@@ -2084,7 +2267,7 @@ int rtnn_autotune(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_device, i
     // measured here with the chosen configuration (results are exact for any).
     const float m0 = ctx->mlp.margin;
     float best_m = m0;
-    for (float f : {0.85f, 1.15f, 1.3f, 1.5f}) {
+    for (float f : {0.85f, 1.15f, 1.3f, 1.5f, 2.0f}) {
       ctx->mlp.margin = m0 * f;
       double ms = 0.0;
       const int rc = time_candidate(
@@ -2109,7 +2292,7 @@ int rtnn_autotune(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_device, i
     // Density buckets: which proposed radii are close enough to r that the
     // query is searched at r at once (measured with the chosen configuration).
     const float b0 = best_opt.bucket_ratio;
-    for (float b : {1.0f, 0.9f, 0.8f, 0.7f, 0.6f, 0.5f}) {
+    for (float b : {0.95f, 0.9f, 0.8f, 0.7f, 0.6f, 0.5f}) {
       if (b == b0) continue;
       ctx->opt.bucket_ratio = b;
       double ms = 0.0;
@@ -2131,6 +2314,27 @@ int rtnn_autotune(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_device, i
       }
     }
     ctx->opt = best_opt;
+    if (method == RTNN_METHOD_LEARNED) {
+      // The network's proposal capped by the density partition's radius.
+      ctx->opt.learned_cap = 1 - best_opt.learned_cap;
+      double ms = 0.0;
+      const int rc = time_candidate(
+          [&](rtnn_stats* st) {
+            return proposal_search(ctx, Proposal::kLearned, qxyz, nq, on_device, r, k, out_idx,
+                                   out_d2, 1, st);
+          },
+          ms);
+      if (rc != RTNN_OK) {
+        ctx->opt = base;
+        return rc;
+      }
+      tune_log("autotune learned: learned_cap %d: %.4f ms", ctx->opt.learned_cap, ms);
+      if (ms < best_t) {
+        best_t = ms;
+        best_opt.learned_cap = ctx->opt.learned_cap;
+      }
+      ctx->opt = best_opt;
+    }
   }
   if (best) *best = best_opt;
   if (best_ms) *best_ms = best_t;

@@ -82,11 +82,9 @@ hipError_t partition_hier(const float4* qproc, uint32_t nq, const GridDesc* leve
                           int level_min, const uint32_t* table, uint32_t k, float r,
                           float* qradius, hipStream_t st);
 
-// Density buckets: radii >= snap_r become r (sparse bucket); order[] gets the
-// dense bucket's positions from the front and the sparse bucket's from the
-// back; counts[0], counts[1] (zeroed by the caller) the bucket sizes.
-hipError_t bucket_split(float* qradius, uint32_t nq, float snap_r, float r, uint32_t* order,
-                        uint32_t* counts, hipStream_t st);
+// Density buckets: radii >= snap_r become r (the sparse bucket, searched at
+// r at once).
+hipError_t bucket_snap(float* qradius, uint32_t nq, float snap_r, float r, hipStream_t st);
 
 // ---- search.hip
 // Runs the top-k search.  group_lanes = 1 selects the thread-per-query kernel

@@ -52,10 +52,10 @@ hipError_t mlp_forward(const MlpState& m, const float* feats, uint64_t n, float*
 // Fused features + MFMA forward for queries in processing order; writes the
 // proposed radius per processing position.  morton: *coarse is a level of a
 // hierarchical Morton grid (counts = its table: end - start), else a raster
-// count grid.
+// count grid.  cap: each proposal is at most the radius qradius holds on entry.
 hipError_t mlp_propose(const MlpState& m, const float4* qproc, uint32_t nq,
                        const GridDesc* coarse, bool morton, const uint32_t* counts, float r,
-                       float* qradius, hipStream_t st, int num_cus);
+                       bool cap, float* qradius, hipStream_t st, int num_cus);
 // Pinball-loss gradient partials over `count` samples (rows sel[i] of
 // feats/targets, or rows 0..count-1 if sel == nullptr, or seeded random rows
 // of [0, n) if rand_seed != 0).  norm = 1 / batch size.

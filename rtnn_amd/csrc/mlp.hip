@@ -343,7 +343,7 @@ __global__ __launch_bounds__(256) void k_mlp_propose(const float* __restrict__ P
                                                      const GridDesc* __restrict__ coarse,
                                                      const uint32_t* __restrict__ counts,
                                                      float count_scale, float margin, float r,
-                                                     float* __restrict__ qradius) {
+                                                     int cap, float* __restrict__ qradius) {
   __shared__ MfmaLds w;
   stage_weights(P, w);
   const GridDesc g = *coarse;
@@ -353,7 +353,9 @@ __global__ __launch_bounds__(256) void k_mlp_propose(const float* __restrict__ P
   const uint32_t ntiles = (nq + 15) / 16;
   // Software pipeline over this wave's tiles: while tile t runs its MFMA
   // layers, the count loads of tile t + 1 and the query of tile t + 2 are in
-  // flight.  A position past nq reads query 0 (its result is not written).
+  // flight (two tiles of count loads ahead measured no faster at 10M
+  // queries, 0.623 vs 0.622 ms).  A position past nq reads query 0 (its
+  // result is not written).
   auto load_q = [&](uint32_t t) -> float4 {
     const uint64_t qi = (uint64_t)t * 16 + row;
     return qi < nq ? qproc[qi] : qproc[0];
@@ -376,13 +378,14 @@ __global__ __launch_bounds__(256) void k_mlp_propose(const float* __restrict__ P
     FeatureLoads ln;
     feature_loads<MORTON>(g, counts, qn, grp, fn, ln);
     const float4 qc = load_q(t + 2 * nwaves);
+    const uint32_t qi = t * 16 + row;
+    const float bound = cap && grp == 0 && qi < nq ? qradius[qi] : r;
     bf16x4 x[2];
     feature_frag<MORTON>(qa, scale_ln2, fn, la, x);
     const float o = mlp_tile(w, x);
-    const uint32_t qi = t * 16 + row;
     if (grp == 0 && qi < nq) {
       const float frac = fminf(fmaxf(o * margin, kMinFrac), 1.0f);
-      qradius[qi] = fminf(r * frac, r);
+      qradius[qi] = fminf(r * frac, bound);
     }
     qa = qn;
     la = ln;
@@ -621,6 +624,8 @@ hipError_t mlp_features(const float* qxyz, uint64_t n, const GridDesc* coarse,
 
 static dim3 persistent_grid(uint64_t tiles16, int num_cus) {
   uint64_t want = (tiles16 + 3) / 4;  // 4 waves per block
+  // two blocks per resident slot: one resident round (4 blocks per CU)
+  // measured 5 % slower at 10M queries
   uint64_t cap = (uint64_t)(num_cus > 0 ? num_cus : 256) * 8;
   return dim3((uint32_t)(want < cap ? (want ? want : 1) : cap));
 }
@@ -635,15 +640,15 @@ hipError_t mlp_forward(const MlpState& m, const float* feats, uint64_t n, float*
 
 hipError_t mlp_propose(const MlpState& m, const float4* qproc, uint32_t nq,
                        const GridDesc* coarse, bool morton, const uint32_t* counts, float r,
-                       float* qradius, hipStream_t st, int num_cus) {
+                       bool cap, float* qradius, hipStream_t st, int num_cus) {
   if (nq == 0) return hipSuccess;
   const dim3 grid = persistent_grid((nq + 15) / 16, num_cus);
   if (morton)
     hipLaunchKernelGGL(k_mlp_propose<true>, grid, dim3(256), 0, st, m.params, qproc, nq, coarse,
-                       counts, m.count_scale, m.margin, r, qradius);
+                       counts, m.count_scale, m.margin, r, cap ? 1 : 0, qradius);
   else
     hipLaunchKernelGGL(k_mlp_propose<false>, grid, dim3(256), 0, st, m.params, qproc, nq, coarse,
-                       counts, m.count_scale, m.margin, r, qradius);
+                       counts, m.count_scale, m.margin, r, cap ? 1 : 0, qradius);
   return hipGetLastError();
 }
 

@@ -143,44 +143,25 @@ __global__ __launch_bounds__(256) void k_partition_hier(const float4* __restrict
   qradius[i] = fminf(R, r);
 }
 
-// Density buckets of the proposal methods (the reference's query partition
-// into batches of one launch radius each, sort.cpp:346-391 genBatches, here
-// two buckets in one launch).  A query whose proposed radius is at least
-// snap_r belongs to the sparse bucket: it is searched at r at once, so its row
-// is final after the first launch (no wasted search at r' and no fallback
-// stages); the others keep r'.  order[] lists the processing positions bucket
-// by bucket for that launch -- the dense bucket from the front, the sparse one
-// from the back, each wave's queries together -- so the waves of one
-// persistent launch hold queries of one bucket.  counts[0..1] (zeroed by the
-// caller) receive the bucket sizes.
-__global__ __launch_bounds__(256) void k_bucket_split(float* __restrict__ qradius, uint32_t nq,
-                                                      float snap_r, float r,
-                                                      uint32_t* __restrict__ order,
-                                                      uint32_t* __restrict__ counts) {
+// Density buckets of the proposal methods: a query whose proposed radius is
+// at least snap_r belongs to the sparse bucket and is searched at r at once,
+// so its row is final after the first launch (no search at r' that is bound
+// to come up short, and no fallback stages); the others keep r'.  The
+// queries stay in one launch in their Morton order: laying the launch out
+// bucket by bucket (the reference's batches, sort.cpp:346-391) cost the
+// Levy-flight KNN 2.3x in L2 locality.
+__global__ __launch_bounds__(256) void k_bucket_snap(float* __restrict__ qradius, uint32_t nq,
+                                                     float snap_r, float r) {
   const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-  const bool act = i < nq;
-  const float R = act ? qradius[i] : 0.0f;
-  const bool sparse = act && R >= snap_r;
-  if (sparse && R != r) qradius[i] = r;
-  const uint64_t md = __ballot(act && !sparse), ms = __ballot(sparse);
-  const uint32_t lane = threadIdx.x & 63u;
-  uint32_t bd = 0, bs = 0;
-  if (lane == 0) {
-    if (md) bd = atomicAdd(&counts[0], (uint32_t)__popcll(md));
-    if (ms) bs = atomicAdd(&counts[1], (uint32_t)__popcll(ms));
-  }
-  bd = __shfl(bd, 0, 64);
-  bs = __shfl(bs, 0, 64);
-  const uint64_t below = (1ull << lane) - 1ull;
-  if (act && !sparse) order[bd + (uint32_t)__popcll(md & below)] = i;
-  if (sparse) order[nq - 1u - (bs + (uint32_t)__popcll(ms & below))] = i;
+  if (i >= nq) return;
+  const float R = qradius[i];
+  if (R >= snap_r && R != r) qradius[i] = r;
 }
 
-hipError_t bucket_split(float* qradius, uint32_t nq, float snap_r, float r, uint32_t* order,
-                        uint32_t* counts, hipStream_t st) {
+hipError_t bucket_snap(float* qradius, uint32_t nq, float snap_r, float r, hipStream_t st) {
   if (nq == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_bucket_split, dim3((nq + 255) / 256), dim3(256), 0, st, qradius, nq,
-                     snap_r, r, order, counts);
+  hipLaunchKernelGGL(k_bucket_snap, dim3((nq + 255) / 256), dim3(256), 0, st, qradius, nq,
+                     snap_r, r);
   return hipGetLastError();
 }
 

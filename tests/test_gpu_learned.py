@@ -130,7 +130,14 @@ def test_partitioned_equals_plain(ctx, k):
     qs = rtnn_amd.gen_gaussian_mixture(2, 21, 50000)
     ctx.set_points(pts)
     st = _plain_vs(ctx, qs, 20.0, k, ctx.search_partitioned)
-    assert st.search_launches == 6  # stage 1, four radius-growth stages, the last at r
+    # hierarchical grid (the default): stage 1, then the octree walk at r
+    assert st.search_launches == 2
+    ctx.set_options(grid_levels=1)
+    try:
+        st = _plain_vs(ctx, qs, 20.0, k, ctx.search_partitioned)
+        assert st.search_launches == 6  # stage 1, four radius-growth stages, the last at r
+    finally:
+        ctx.set_options()
     rng = np.random.default_rng(k)
     u = rng.uniform(0, 30, (40000, 3)).astype(np.float32)
     ctx.set_points(u)
@@ -269,10 +276,11 @@ def test_levy_uniform_queries_k32_large_radius(ctx):
 @pytest.mark.parametrize("ratio", [1.0, 0.7, 0.3])
 def test_density_buckets_exact(ctx, ratio):
     # Density buckets (bucket_ratio): proposals r' >= ratio * r are searched at
-    # r at once and the first launch takes the queries bucket by bucket.  Rows
-    # equal the plain KNN for both proposal methods, on the raster grid, the
-    # hierarchical grid's cube walks and the octree walk; the radii
-    # rtnn_propose_radius reports are the proposals with that rule applied.
+    # r at once.  Rows equal the plain KNN for both proposal methods, on the
+    # raster grid, the hierarchical grid's cube walks and the octree walk; the
+    # radii rtnn_propose_radius reports are the proposals with that rule
+    # applied.  learned_cap: the network's radius capped by the partition's,
+    # exactly their minimum.
     pts = rtnn_amd.gen_levy(5, 300_000, alpha=1.5, min_step=0.01, box=300.0)
     qs = rtnn_amd.gen_uniform(52, 40_000, 0.0, 300.0)
     ctx.set_points(pts)
@@ -297,6 +305,15 @@ def test_density_buckets_exact(ctx, ratio):
                     # most proposals land in the sparse bucket: few rows left
                     # short after the first launch
                     assert st.fallback_queries <= int((~snap).sum()), (opts, method)
+            ctx.set_options(**opts)
+            part = ctx.propose_radius(qs, "partitioned", r, k)
+            net = ctx.propose_radius(qs, "learned", r, k)
+            ctx.set_options(learned_cap=1, **opts)
+            capped = ctx.propose_radius(qs, "learned", r, k)
+            assert np.array_equal(capped, np.minimum(net, part)), opts
+            assert (capped < net).any()
+            li, ld, _ = ctx.search_learned(qs, radius=r, k=k)
+            assert np.array_equal(pi, li) and np.array_equal(pd, ld), opts
     finally:
         ctx.set_options()
 
@@ -306,4 +323,6 @@ def test_bucket_ratio_validated(ctx):
         ctx.set_options(bucket_ratio=1.5)
     with pytest.raises(rtnn_amd.RtnnError):
         ctx.set_options(bucket_ratio=-0.1)
+    with pytest.raises(rtnn_amd.RtnnError):
+        ctx.set_options(learned_cap=2)
     ctx.set_options()

@@ -566,6 +566,44 @@ def test_search_stream_chunks_equal_one_search(ctx, tmp_path):
         ctx.search_stream(qs, mode="radius", method="learned", radius=1.2, k=8)
 
 
+def test_batched_stream_chunks_equal_one_batched_search(fresh_ctx, tmp_path):
+    # Host clouds searched in chunks of whole clouds (a cloud larger than the
+    # chunk on its own, empty clouds, a last short chunk): rows equal one
+    # batched search, into host arrays and into a result file; size classes
+    # apply per cloud inside every chunk.
+    import torch
+    ctx = fresh_ctx
+    xyz, offs = rtnn_amd.gen_lattice_clouds(24, 60, 200, 12000, h=0.1, jit=0.02)
+    offs = np.insert(offs, 5, offs[5])  # an empty cloud
+    offs = np.insert(offs, len(offs), offs[-1])  # and one at the end
+    t = torch.from_numpy(xyz).cuda()
+    for mode, r, k in (("radius", 0.2, 64), ("knn", 0.25, 8)):
+        wi, wd, _ = ctx.search_batched(t, offs, mode=mode, radius=r, k=k)
+        wi, wd = wi.cpu().numpy(), wd.cpu().numpy()
+        for chunk in (0, 10000, 50000):
+            si, sd, st = ctx.search_batched_stream(xyz, offs, mode=mode, radius=r, k=k,
+                                                   chunk=chunk)
+            assert np.array_equal(si, wi) and np.array_equal(sd, wd), (mode, chunk)
+        si, sd, _ = ctx.search_batched_stream(xyz, offs, mode=mode, radius=r, k=k, chunk=7000,
+                                              with_d2=False)
+        assert np.array_equal(si, wi) and sd is None
+        path = tmp_path / f"batched_{mode}.bin"
+        ctx.search_batched_stream(xyz, offs, mode=mode, radius=r, k=k, chunk=20000,
+                                  path=str(path))
+        fi, fd = rtnn_amd.read_results(str(path))
+        assert np.array_equal(fi, wi) and np.array_equal(fd, wd)
+    ctx.set_size_classes([1500, 5000], CLASS_OPTS)
+    try:
+        wi, wd, _ = ctx.search_batched(t, offs, mode="radius", radius=0.2, k=64)
+        si, sd, st = ctx.search_batched_stream(xyz, offs, mode="radius", radius=0.2, k=64,
+                                               chunk=30000)
+        assert np.array_equal(si, wi.cpu().numpy()) and np.array_equal(sd, wd.cpu().numpy())
+    finally:
+        ctx.set_size_classes([], [])
+    with pytest.raises(rtnn_amd.RtnnError):
+        ctx.search_batched_stream(xyz, offs[::-1].copy(), mode="radius", radius=0.2, k=64)
+
+
 def test_mem_stats_track_the_library_allocations():
     # Device bytes rise with a context's buffers and fall back when it closes;
     # the peak covers the largest buffer set (memstatlib's running total).
