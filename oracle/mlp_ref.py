@@ -7,8 +7,9 @@ This module restates the exact network the GPU computes, in float64 numpy:
     nearest even); the last layer (64 -> 1, VALU on the GPU) takes the FP32
     activations;
   * GELU in the sigmoid form of Hendrycks & Gimpel (2016, section 2),
-    x sigma(1.702 x), which the GPU evaluates with one exp2 and one
-    reciprocal (`gelu_erf` is the exact x Phi(x), for its error bound);
+    x sigma(1.702 x), with the logistic replaced by the hard sigmoid of
+    Howard et al.'s hard-swish (2019, section 5.2): x clamp(A x + 1/2, 0, 1),
+    A = 1.702 / 6 (`gelu_erf` is the exact x Phi(x), for its error bound);
   * features: log1p(count) * count_scale of the 27 edge-r cells around the
     query's cell ((dx,dy,dz) = (f%3-1, f//3%3-1, f//9-1)), then the query's
     offset inside its cell;
@@ -20,7 +21,7 @@ import numpy as np
 
 H, F, FP = 64, 30, 32
 OW1, OB1, OW2, OB2, OW3, OB3, NP = 0, 1920, 1984, 6080, 6144, 6208, 6209
-S = 1.702
+A = 1.702 / 6.0  # slope of the ramp
 
 
 def bf16(x):
@@ -31,17 +32,18 @@ def bf16(x):
     return (u.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
 
 
-def _sigma(x):
-    return 0.5 * (1.0 + np.tanh(0.5 * x))  # 1 / (1 + exp(-x)) without overflow
+def _t(x, a=A):
+    return np.clip(a * np.asarray(x, np.float64) + 0.5, 0.0, 1.0)
 
 
-def gelu(x):
-    return x * _sigma(S * x)
+def gelu(x, a=A):
+    return np.asarray(x, np.float64) * _t(x, a)
 
 
 def gelu_grad(x):
-    s = _sigma(S * x)
-    return s + S * x * s * (1.0 - s)
+    x = np.asarray(x, np.float64)
+    h = A * x + 0.5
+    return np.where(h <= 0.0, 0.0, np.where(h >= 1.0, 1.0, 2.0 * A * x + 0.5))
 
 
 def gelu_erf(x):
@@ -51,8 +53,8 @@ def gelu_erf(x):
     return 0.5 * x * (1.0 + np.vectorize(lambda v: erf(v / sqrt(2.0)))(x))
 
 
-def unpack(p):
-    p = np.asarray(p, dtype=np.float32)
+def unpack(p, dtype=np.float32):
+    p = np.asarray(p, dtype=dtype)
     return (p[OW1:OB1].reshape(H, F), p[OB1:OW2], p[OW2:OB2].reshape(H, H), p[OB2:OW3],
             p[OW3:OB3], p[OB3])
 
@@ -93,12 +95,15 @@ def grad(p, feats, t, tau):
     return out
 
 
-def grad_fd(p, feats, t, tau, idx, eps=1e-3):
+def grad_fd(p, feats, t, tau, idx, eps=1e-5):
     """Central differences of `loss` w.r.t. parameters idx, with the BF16
     rounding of the weights bypassed (the analytic gradient treats it as the
-    identity), i.e. on the network whose weights are exactly p."""
+    identity), i.e. on the network whose weights are exactly p.  The
+    activation is piecewise quadratic in each weight, so a central difference
+    is exact unless the step crosses a kink: the step is small (1e-5) and the
+    perturbed weights stay in float64."""
     def loss_exact(q):
-        W1, b1, W2, b2, w3, b3 = [np.asarray(v, np.float64) for v in unpack(q)]
+        W1, b1, W2, b2, w3, b3 = unpack(q, np.float64)  # the perturbation kept exact
         x = bf16(np.asarray(feats, np.float32)[:, :F])
         z1 = x @ W1.T + b1
         h1 = gelu(z1)

@@ -26,25 +26,37 @@ def test_bf16_known_answers():
 
 def test_gelu_closed_forms():
     assert M.gelu(np.array(0.0)) == 0.0
-    # x sigma(a x) - (-x) sigma(-a x) = x, and the limits x and 0
+    # x t(x) - (-x) t(-x) = x since t(-x) = 1 - t(x); exactly x and 0 beyond
+    # |x| = 1 / (2A) = 3 / 1.702; t(1) = A + 1/2
     xs = np.linspace(-8, 8, 161)
     assert np.allclose(M.gelu(xs) - M.gelu(-xs), xs, atol=1e-12)
-    assert np.allclose(M.gelu(np.array([-30.0, 30.0])), [0.0, 30.0], atol=1e-9)
-    assert abs(M.gelu(np.array(1.0)) - 1.0 / (1.0 + np.exp(-1.702))) < 1e-12
-    fd = (M.gelu(xs + 1e-6) - M.gelu(xs - 1e-6)) / 2e-6
+    edge = 0.5 / M.A
+    assert abs(edge - 3.0 / 1.702) < 1e-12
+    assert np.array_equal(M.gelu(np.array([edge, 3.0, 30.0])), [edge, 3.0, 30.0])
+    assert np.array_equal(M.gelu(np.array([-edge, -3.0, -30.0])), [0.0, 0.0, 0.0])
+    assert abs(M.gelu(np.array(1.0)) - (1.702 / 6 + 0.5)) < 1e-15
+    # the derivative away from the two kinks at +-edge
+    xs = xs[np.abs(np.abs(xs) - edge) > 1e-3]
+    fd = (M.gelu(xs + 1e-7) - M.gelu(xs - 1e-7)) / 2e-7
     assert np.allclose(M.gelu_grad(xs), fd, atol=1e-6)
+    assert M.gelu_grad(np.array(edge - 1e-9)) > 1.49  # 2 A x + 1/2 -> 3/2 at the kink
+    assert M.gelu_grad(np.array(edge + 1e-9)) == 1.0
 
 
-def test_gelu_sigmoid_form_error_bound():
-    # Hendrycks & Gimpel's x sigma(1.702 x) stays within 0.021 of the exact
-    # x Phi(x) everywhere (the largest gap is near |x| = 2.1); a slope other
-    # than 1.702 or a dropped factor x breaks the bound.
+def test_gelu_hard_sigmoid_error_bound():
+    # x clamp(1.702 x / 6 + 1/2, 0, 1) stays within 0.069 of the exact
+    # x Phi(x) everywhere (the largest gap is at the kinks, |x| = 3 / 1.702);
+    # the sigmoid form it replaces is within 0.021; another slope (1/4 or
+    # 1/3) or a dropped factor x breaks the bound.
     xs = np.linspace(-10, 10, 20001)
     err = np.abs(M.gelu(xs) - M.gelu_erf(xs))
-    assert err.max() < 0.021 and err.max() > 0.015
-    assert 1.8 < abs(xs[err.argmax()]) < 2.4
-    worse = np.abs(xs * M._sigma(1.6 * xs) - M.gelu_erf(xs)).max()
-    assert worse > 0.021
+    assert 0.06 < err.max() < 0.069
+    assert 1.7 < abs(xs[err.argmax()]) < 1.82
+    sig = np.abs(xs * 0.5 * (1 + np.tanh(0.5 * 1.702 * xs)) - M.gelu_erf(xs)).max()
+    assert 0.015 < sig < 0.021
+    for a in (0.25, 1.0 / 3.0):
+        assert np.abs(M.gelu(xs, a) - M.gelu_erf(xs)).max() > 0.069
+    assert np.abs(M._t(xs) - M.gelu_erf(xs)).max() > 0.069
 
 
 def test_analytic_gradient_matches_finite_differences():
