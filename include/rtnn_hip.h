@@ -89,6 +89,10 @@ typedef struct {
                              first radius, which holds k neighbours), so queries
                              the partition bounds more tightly keep its radius
                              and do not fall back; 0 = the network alone        */
+  int   tree_lanes;       /* hierarchical-grid KNN, the octree walk: lanes per
+                             query (2, 4 or 8 split the chunks and points of
+                             every leaf; 0 or 1 = one lane).  2 is the fastest
+                             on clustered points (Levy flights)               */
 } rtnn_options;
 
 /* Filled by search entry points when a non-NULL pointer is passed. */

@@ -611,6 +611,7 @@ SearchArgs base_args(rtnn_ctx ctx, const float4* qproc, uint64_t nq, float r, ui
     a.level_alpha = ctx->opt.level_alpha > 0 ? ctx->opt.level_alpha : 1.0f;
     a.chunk_box = ctx->cboxes.as<float4>();
     a.tree_leaf = ctx->opt.tree_leaf > 0 ? (uint32_t)ctx->opt.tree_leaf : kTreeLeafDefault;
+    a.tree_lanes = ctx->opt.tree_lanes > 1 ? (uint32_t)ctx->opt.tree_lanes : 1u;
   }
   return a;
 }
@@ -784,6 +785,8 @@ int check_options(rtnn_ctx ctx, const rtnn_options* opt) {
     return fail(ctx, RTNN_ERR_INVALID, "bucket_ratio must be in [0, 1]");
   if (opt->learned_cap != 0 && opt->learned_cap != 1)
     return fail(ctx, RTNN_ERR_INVALID, "learned_cap must be 0 or 1");
+  if (opt->tree_lanes < 0 || opt->tree_lanes > 8 || (opt->tree_lanes & (opt->tree_lanes - 1)))
+    return fail(ctx, RTNN_ERR_INVALID, "tree_lanes must be 0, 1, 2, 4 or 8");
   return RTNN_OK;
 }
 
@@ -2189,6 +2192,7 @@ int rtnn_autotune(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_device, i
     int qpw, levels;
     float alpha;
     int buffer, tile, leaf;
+    int tlanes = 0;
   };
   std::vector<Cand> cands;
   for (int i = 0; i < ncf; ++i)
@@ -2212,7 +2216,8 @@ int rtnn_autotune(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_device, i
   if (!plain)
     for (int lv : {3, 5})
       for (float al : {1.0f, 0.5f}) {
-        for (int leaf : {0, 65536}) cands.push_back({0.0f, 0, lv, al, 0, 0, leaf});
+        for (int leaf : {0, 65536})
+          for (int tl : {1, 2}) cands.push_back({0.0f, 0, lv, al, 0, 0, leaf, tl});
         cands.push_back({0.0f, 16, lv, al, 0, 0, 0});
       }
   for (const Cand& cand : cands) {
@@ -2228,6 +2233,7 @@ int rtnn_autotune(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_device, i
       ctx->opt.group_buffer = cand.buffer;
       ctx->opt.lds_tile = cand.tile;
       ctx->opt.tree_leaf = cand.leaf;
+      ctx->opt.tree_lanes = cand.tlanes;
       double ms = 0.0;
       const int rc = time_candidate(
           [&](rtnn_stats* st) {
@@ -2248,9 +2254,9 @@ int rtnn_autotune(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_device, i
         return rc;
       }
       tune_log("autotune %s: candidate %zu/%zu cell_factor %g qpw %d levels %d alpha %g "
-               "buffer %d tile %d leaf %d: %.4f ms", method_name(method),
+               "buffer %d tile %d leaf %d tree lanes %d: %.4f ms", method_name(method),
                (size_t)(&cand - cands.data()) + 1, cands.size(), cand.cf, cand.qpw, cand.levels,
-               cand.alpha, cand.buffer, cand.tile, cand.leaf, ms);
+               cand.alpha, cand.buffer, cand.tile, cand.leaf, cand.tlanes, ms);
       if (ms < best_t) {
         best_t = ms;
         best_opt = ctx->opt;

@@ -81,6 +81,7 @@ struct SearchArgs {
   // a large leaf that lie beyond its limit.
   const float4* chunk_box;
   uint32_t tree_leaf;  // octree walk: cells of at most this many points are leaves
+  uint32_t tree_lanes;  // octree walk: lanes per query (1: k_topk_tree; 2, 4, 8: k_topk_treeg)
   // Pooled kernel (k_topk_pool): a query with more hits than its LDS buffer
   // holds appends its processing position here (count in ovf_count); the
   // group kernel then searches those rows again.

@@ -424,6 +424,180 @@ __global__ __launch_bounds__(kTileQ) void k_topk_tree(const SearchArgs a) {
 }
 
 // --------------------------------------------------------------------------
+// The octree walk with G lanes per query
+//
+// k_topk_tree walks one query per lane; a walk that crosses a dense cluster
+// scans its leaves chunk after chunk on that one lane while the L2 latency of
+// every load is exposed (the Levy-flight KNN: 0.30 of the VALU slots, 97 %
+// L2 hits).  Here the G lanes of a group walk the same octree path -- every
+// decision reads a limit the group shares, so their control flow is the
+// same -- and deal each leaf's chunks to them round-robin (lane l tests and
+// scans chunks l, l + G, ...; a leaf of one chunk has its points split), each
+// into a top-k list of its own.  After each round the group takes the
+// smallest of its lanes' limits: once a lane holds
+// k keys with d2 <= t, no point beyond t is among the group's k nearest, so
+// every lane may prune at the smallest lane limit.  At the end the G sorted
+// lists are merged by repeatedly taking the group's smallest head (the
+// points of one list are distinct from every other's, so each head is
+// unique); the row is the first k taken.  Same result as k_topk_tree.
+// On the Levy-flight KNN (30M x 30M, leaves of 65536 points) one lane took
+// 1080 ms, two lanes 900 ms, four 1062 ms, eight 1245 ms: beyond two, the
+// walk every lane repeats (cell bounds, boxes) outweighs the split chunks.
+
+template <int G>
+__device__ inline float group_min_f(float v) {
+#pragma unroll
+  for (int o = 1; o < G; o <<= 1) v = fminf(v, __shfl_xor(v, o, G));
+  return v;
+}
+
+template <int G>
+__device__ inline uint64_t group_min_u64(uint64_t v) {
+#pragma unroll
+  for (int o = 1; o < G; o <<= 1) {
+    const uint64_t w = (uint64_t)__shfl_xor((unsigned long long)v, o, G);
+    v = w < v ? w : v;
+  }
+  return v;
+}
+
+template <int KC, int G>
+__global__ __launch_bounds__(kTileQ) void k_topk_treeg(const SearchArgs a) {
+  __shared__ uint32_t s_base[8];
+  if (threadIdx.x < 8) s_base[threadIdx.x] = (int)threadIdx.x < a.nlevels ?
+                                               a.grids[threadIdx.x].cell_base : 0u;
+  __syncthreads();
+  const uint32_t nq = launch_nq(a);
+  const int T = a.nlevels - 1, L0 = a.level_min;
+  const GridDesc g0 = a.grids[0];
+  const int gq = threadIdx.x / G, l = threadIdx.x % G;
+  for (TileIter it(a, nq); it.t < it.end; it.t += it.step) {
+    uint32_t q0, q1, gi;
+    tile_of(a, nq, it.t, q0, q1, gi);
+    const uint32_t qi = q0 + gq;
+    if (qi >= q1) continue;  // the whole group
+    const uint32_t qs = a.qsel ? a.qsel[qi] : qi;
+    const float4 qv = a.queries[qs];
+    const uint32_t row = __float_as_uint(qv.w);
+    const float R = query_radius(a, qi, qs);
+    float lim = r2_limit(R * R);
+    uint64_t list[KC];
+#pragma unroll
+    for (int i = 0; i < KC; ++i) list[i] = ~0ull;
+    // points j0, j0 + step, ... < j1 into the lane's list
+    auto scan = [&](uint32_t j0, uint32_t j1, uint32_t step) {
+      for (uint32_t j = j0; j < j1; j += step) {
+        const float4 p = a.spts[j];
+        const float d2 = d2_of(qv.x, qv.y, qv.z, p.x, p.y, p.z);
+        if (d2 > 0.0f && d2 <= lim) {
+          const uint64_t key = pack_key(d2, __float_as_uint(p.w));
+          if (key < list[KC - 1]) {
+            topk_insert<KC>(list, key);
+            if (list[KC - 1] != ~0ull) lim = fminf(lim, key_d2(list[KC - 1]));
+          }
+        }
+      }
+    };
+    int lo0, hi0, lo1, hi1, lo2, hi2;
+    cell_range(g0, qv.x, R, 0, lo0, hi0);
+    cell_range(g0, qv.y, R, 1, lo1, hi1);
+    cell_range(g0, qv.z, R, 2, lo2, hi2);
+    const int qc0 = min(max((int)floorf((qv.x - g0.mn[0]) * g0.inv), 0), g0.dim[0] - 1);
+    const int qc1 = min(max((int)floorf((qv.y - g0.mn[1]) * g0.inv), 0), g0.dim[1] - 1);
+    const int qc2 = min(max((int)floorf((qv.z - g0.mn[2]) * g0.inv), 0), g0.dim[2] - 1);
+    const uint32_t qm = morton3((uint32_t)qc0, (uint32_t)qc1, (uint32_t)qc2);
+    const float mag = fabsf(g0.mn[0]) + fabsf(g0.mn[1]) + fabsf(g0.mn[2]) + fabsf(qv.x) +
+                      fabsf(qv.y) + fabsf(qv.z) + (float)g0.dim[0] * g0.cell;
+    if (hi0 >= lo0 && hi1 >= lo1 && hi2 >= lo2) {
+      const int tl0 = lo0 >> T, th0 = hi0 >> T, tl1 = lo1 >> T, th1 = hi1 >> T;
+      const int tl2 = lo2 >> T, th2 = hi2 >> T;
+      const int qt0 = qc0 >> T, qt1 = qc1 >> T, qt2 = qc2 >> T;
+      const int nring = max(max(max(qt0 - tl0, th0 - qt0), max(qt1 - tl1, th1 - qt1)),
+                            max(qt2 - tl2, th2 - qt2));
+      for (int ring = 0; ring <= nring; ++ring)
+      for (int tz = max(tl2, qt2 - ring); tz <= min(th2, qt2 + ring); ++tz)
+        for (int ty = max(tl1, qt1 - ring); ty <= min(th1, qt1 + ring); ++ty)
+          for (int tx = max(tl0, qt0 - ring); tx <= min(th0, qt0 + ring); ++tx) {
+            if (max(max(abs(tx - qt0), abs(ty - qt1)), abs(tz - qt2)) != ring) continue;
+            int lv = T;
+            uint32_t m = morton3((uint32_t)tx, (uint32_t)ty, (uint32_t)tz);
+            for (;;) {
+              const float c = ldexpf(g0.cell, lv);
+              const float pad = c * 0x1.0p-12f + mag * 0x1.0p-19f;
+              const float bd = box_d2(qv, g0, c, pad, compact3(m), compact3(m >> 1),
+                                      compact3(m >> 2));
+              bool down = false;
+              if (bd <= lim * 1.001f) {
+                const uint32_t cb = s_base[lv] + m;
+                const uint32_t s0 = a.cell_start[cb], e = a.cell_start[cb + 1];
+                if (e > s0) {
+                  if (lv > L0 && e - s0 > a.tree_leaf) {
+                    down = true;
+                  } else if (a.chunk_box && e - s0 > kChunkPts) {
+                    // the leaf's chunks dealt round-robin to the lanes, each
+                    // tested against its box and scanned by its lane; the
+                    // group's limit after every round
+                    const uint32_t first = s0 / kChunkPts, last = (e - 1) / kChunkPts;
+                    for (uint32_t cr = first; cr <= last; cr += G) {
+                      const uint32_t ch = cr + (uint32_t)l;
+                      if (ch <= last) {
+                        const float4 blo = a.chunk_box[2 * ch], bhi = a.chunk_box[2 * ch + 1];
+                        const float bx = fmaxf(fmaxf(blo.x - qv.x, qv.x - bhi.x), 0.0f);
+                        const float by = fmaxf(fmaxf(blo.y - qv.y, qv.y - bhi.y), 0.0f);
+                        const float bz = fmaxf(fmaxf(blo.z - qv.z, qv.z - bhi.z), 0.0f);
+                        if ((bx * bx + by * by) + bz * bz <= lim * 1.001f)
+                          scan(max(s0, ch * kChunkPts), min(e, (ch + 1) * kChunkPts), 1u);
+                      }
+                      lim = group_min_f<G>(lim);
+                    }
+                  } else {
+                    scan(s0 + (uint32_t)l, e, G);  // a small leaf: its points split
+                    lim = group_min_f<G>(lim);
+                  }
+                }
+              }
+              if (down) {
+                --lv;
+                m = (m << 3) | ((qm >> (3 * lv)) & 7u);
+                continue;
+              }
+              while (lv < T && (((m & 7u) ^ ((qm >> (3 * lv)) & 7u)) == 7u)) {
+                m >>= 3;
+                ++lv;
+              }
+              if (lv == T) break;
+              const uint32_t o = (qm >> (3 * lv)) & 7u;
+              m = (m & ~7u) | ((((m & 7u) ^ o) + 1u) ^ o);
+            }
+          }
+    }
+    // merge: the group's smallest head, k times
+    const uint32_t k = a.k;
+    const uint32_t base = a.local_index ? g0.point_base : 0u;
+    int32_t* oi = a.out_idx + (uint64_t)row * k;
+    float* od = a.out_d2 ? a.out_d2 + (uint64_t)row * k : nullptr;
+    uint32_t hits = 0;
+#pragma unroll
+    for (int i = 0; i < KC; ++i) {
+      if ((uint32_t)i < k) {
+        const uint64_t m = group_min_u64<G>(list[0]);
+        const bool have = m != ~0ull;
+        hits += have ? 1u : 0u;
+        if (l == i % G) {
+          oi[i] = have ? (int32_t)(key_idx(m) - base) : -1;
+          if (od) od[i] = have ? key_d2(m) : __builtin_huge_valf();
+        }
+        const bool win = have && list[0] == m;
+#pragma unroll
+        for (int j = 0; j < KC - 1; ++j) list[j] = win ? list[j + 1] : list[j];
+        list[KC - 1] = win ? ~0ull : list[KC - 1];
+      }
+    }
+    if (a.short_rows) append_short(a, l == 0 && hits < k && R < a.r, qs, hits, R);
+  }
+}
+
+// --------------------------------------------------------------------------
 // G lanes per query
 
 template <int G>
@@ -1335,7 +1509,13 @@ __global__ __launch_bounds__(kTileQ) void k_topk_tiled(const SearchArgs a) {
 template <int KC>
 static void launch_thread(const SearchArgs& a, bool knn, dim3 grid, hipStream_t st) {
   const bool hier = a.nlevels > 0;
-  if (knn && hier)
+  if (knn && hier && a.tree_lanes == 2)
+    hipLaunchKernelGGL((k_topk_treeg<KC, 2>), grid, dim3(kTileQ), 0, st, a);
+  else if (knn && hier && a.tree_lanes == 4)
+    hipLaunchKernelGGL((k_topk_treeg<KC, 4>), grid, dim3(kTileQ), 0, st, a);
+  else if (knn && hier && a.tree_lanes == 8)
+    hipLaunchKernelGGL((k_topk_treeg<KC, 8>), grid, dim3(kTileQ), 0, st, a);
+  else if (knn && hier)
     hipLaunchKernelGGL((k_topk_tree<KC>), grid, dim3(kTileQ), 0, st, a);
   else if (knn)
     hipLaunchKernelGGL((k_topk_thread<KC, true, false>), grid, dim3(kTileQ), 0, st, a);
@@ -1449,7 +1629,12 @@ hipError_t launch_search(const SearchArgs& a0, int knn_mode, int group_lanes, hi
     // Implicit tiles: a block of the thread kernel takes kTileQ queries, a
     // block of the group kernel one query per group, so small workloads still
     // spread over many blocks.
-    a.tile_q = group_lanes == 1 ? (uint32_t)kTileQ : (uint32_t)(kTileQ / group_lanes);
+    // the octree walk with several lanes per query takes kTileQ / lanes
+    // queries per block, as a group kernel does
+    const bool tree_g = group_lanes == 1 && knn_mode != 0 && a.nlevels > 0 &&
+                        (a.tree_lanes == 2 || a.tree_lanes == 4 || a.tree_lanes == 8);
+    a.tile_q = tree_g ? (uint32_t)(kTileQ / a.tree_lanes)
+                      : group_lanes == 1 ? (uint32_t)kTileQ : (uint32_t)(kTileQ / group_lanes);
     a.ntiles = (uint32_t)(((uint64_t)a.nq + a.tile_q - 1) / a.tile_q);
   }
   if (a.ntiles == 0) return hipSuccess;

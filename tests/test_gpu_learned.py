@@ -185,6 +185,36 @@ def test_octree_leaf_sizes_exact(ctx, leaf):
         ctx.set_options()
 
 
+@pytest.mark.parametrize("lanes", [2, 4, 8])
+def test_octree_walk_with_several_lanes_exact(ctx, lanes):
+    # G lanes per query split every leaf chunk's points, share the smallest
+    # lane limit and merge their lists: rows equal the plain KNN for k up to
+    # 64, small and huge leaves, the partitioned and learned methods (the
+    # fallback's walks at r use the same lanes), clustered and far queries.
+    pts = rtnn_amd.gen_gaussian_mixture(2, 20, 100000)
+    qs = np.concatenate([rtnn_amd.gen_gaussian_mixture(2, 21, 20000),
+                         [[1e4, 0, 0], [-50, -50, -50]]]).astype(np.float32)
+    ctx.set_points(pts)
+    w = np.random.default_rng(3).normal(0, 0.2, M.NP).astype(np.float32)
+    ctx.set_mlp_weights(np.concatenate([w, [0.25, 1.0]]).astype(np.float32))
+    try:
+        for leaf in (0, 65536):
+            ctx.set_options(grid_levels=5, level_alpha=1.0, queries_per_wave=0, tree_leaf=leaf,
+                            tree_lanes=lanes)
+            for k in (1, 8, 32, 64):
+                _plain_vs(ctx, qs, 20.0, k, ctx.search_partitioned)
+            _plain_vs(ctx, qs, 20.0, 16, ctx.search_learned)
+        lv = rtnn_amd.gen_levy(5, 300000, alpha=1.5, min_step=0.01, box=100.0)
+        lq = rtnn_amd.gen_uniform(51, 20000, 0.0, 100.0)
+        ctx.set_points(lv)
+        ctx.set_options(grid_levels=7, queries_per_wave=0, tree_leaf=65536, tree_lanes=lanes)
+        _plain_vs(ctx, lq, 10.0, 32, ctx.search_partitioned)
+        with pytest.raises(rtnn_amd.RtnnError):
+            ctx.set_options(tree_lanes=3)
+    finally:
+        ctx.set_options()
+
+
 def test_autotune_picks_a_valid_config(ctx):
     rng = np.random.default_rng(21)
     pts = rng.uniform(0, 40, (50000, 3)).astype(np.float32)

@@ -72,7 +72,7 @@ def test_levy_knn_full_size(ctx):
     dp = torch.from_numpy(pts).cuda()
     dq = torch.from_numpy(qs).cuda()
     ctx.set_points(dp)
-    ctx.set_options(grid_levels=7, queries_per_wave=0, tree_leaf=65536)
+    ctx.set_options(grid_levels=7, queries_per_wave=0, tree_leaf=65536, tree_lanes=2)
     try:
         t = time.perf_counter()
         pi, pd, st = ctx.search_partitioned(dq, radius=R, k=K, out_device=True)
