@@ -249,8 +249,9 @@ def kernel_name(res, mode, opts):
     """Name of the dominant search kernel of a method's configuration."""
     o = opts or {}
     if res.get("lanes_per_query") == 1 and not o.get("group_buffer") == 3:
-        return "k_topk_tree" if mode == "knn" and o.get("grid_levels", 0) >= 2 else \
-            "k_topk_thread"
+        if mode == "knn" and o.get("grid_levels", 0) >= 2:
+            return "k_topk_treeg" if o.get("tree_lanes", 0) in (2, 4, 8) else "k_topk_tree"
+        return "k_topk_thread"
     if o.get("group_buffer") == 3:
         return "k_topk_pool"
     if o.get("lds_tile"):
