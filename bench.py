@@ -294,11 +294,22 @@ def measure(args, wname, torch, rtnn_amd, extra=False):
     train = None
     batched_opts = None
     graph = None
+    # Output rows of the single-cloud methods: device tensors allocated once
+    # per method and passed to every step (the C ABI writes into caller
+    # buffers; the Python binding would otherwise allocate a pair per call).
+    rows = len(pts) if qs is None else nq
+
+    def outs():
+        return (torch.empty((rows, k), dtype=torch.int32, device="cuda"),
+                torch.empty((rows, k), dtype=torch.float32, device="cuda"))
+
     for m in w["methods"]:
+        if m in ("plain", "partitioned", "learned"):
+            ob = outs()
         if m == "plain":
-            fn = lambda: ctx.search(dq, mode=mode, radius=r, k=k, out_device=True)
+            fn = lambda: ctx.search(dq, mode=mode, radius=r, k=k, out=ob)
         elif m == "partitioned":
-            fn = lambda: ctx.search_partitioned(dq, radius=r, k=k, out_device=True)
+            fn = lambda: ctx.search_partitioned(dq, radius=r, k=k, out=ob)
         elif m == "percloud":
             fn = lambda: ctx.search_percloud(dp, offs_all, mode=mode, radius=r, k=k,
                                              with_d2=False)
@@ -317,7 +328,7 @@ def measure(args, wname, torch, rtnn_amd, extra=False):
                      "samples": args.train_samples, "steps": args.train_steps, "batch": 4096,
                      "lr": 1e-3, "tau": 0.9}
             log(f"[bench] MLP trained on the GPU in {train['seconds']} s, loss {loss:.5f}")
-            fn = lambda: ctx.search_learned(dq, radius=r, k=k, out_device=True)
+            fn = lambda: ctx.search_learned(dq, radius=r, k=k, out=ob)
         ctx.set_options(**(fixed_opts or w.get("options", {})))
         tuned = None
         if m == "classes":

@@ -317,7 +317,7 @@ class Context:
             d2 = np.empty((nq, k), dtype=np.float32) if with_d2 else None
         return idx, d2
 
-    def _run(self, fn, queries, r, k, with_d2, out_device, extra=()):
+    def _run(self, fn, queries, r, k, with_d2, out_device, extra=(), out=None):
         stats = Stats()
         if queries is None:
             if self._points is None:
@@ -328,9 +328,22 @@ class Context:
             keep = _as_points(queries)
             qptr, qdev = _ptr(keep)
             nq = len(keep)
-        if out_device is None:
-            out_device = bool(qdev) if queries is not None else _is_torch(self._points)
-        idx, d2 = self._outputs(out_device, nq, k, with_d2)
+        if out is not None:
+            # caller-owned outputs (reused across calls): (idx, d2 or None),
+            # [nq, k] int32 / float32, both on the device or both on the host
+            idx, d2 = out
+            dev = _is_torch(idx) and idx.is_cuda
+            ok = tuple(idx.shape) == (nq, k) and str(idx.dtype).endswith("int32") and (
+                d2 is None or (tuple(d2.shape) == (nq, k) and str(d2.dtype).endswith("float32")
+                               and (_is_torch(d2) and d2.is_cuda) == dev))
+            if not ok or not (idx.is_contiguous() if _is_torch(idx) else idx.flags.c_contiguous):
+                raise ValueError(f"out must be contiguous ([{nq}, {k}] int32, [{nq}, {k}] float32 "
+                                 "or None) on one device")
+            out_device = dev
+        else:
+            if out_device is None:
+                out_device = bool(qdev) if queries is not None else _is_torch(self._points)
+            idx, d2 = self._outputs(out_device, nq, k, with_d2)
         ip, _ = _ptr(idx)
         dp, _ = _ptr(d2)
         rc = fn(self._h, qptr, nq, qdev, *extra, r, k, ip, dp, int(out_device),
@@ -339,10 +352,11 @@ class Context:
         return idx, d2, stats
 
     def search(self, queries=None, mode="radius", radius=2.0, k=50, with_d2=True,
-               out_device=None):
-        """Plain grid search -> (idx [nq, k] int32, d2 [nq, k] float32 or None, Stats)."""
+               out_device=None, out=None):
+        """Plain grid search -> (idx [nq, k] int32, d2 [nq, k] float32 or None, Stats).
+        out: optional (idx, d2) arrays or tensors to write into (then returned)."""
         return self._run(self._L.rtnn_search, queries, radius, k, with_d2, out_device,
-                         extra=(_mode(mode),))
+                         extra=(_mode(mode),), out=out)
 
     def search_async(self, queries, idx, d2=None, mode="radius", radius=2.0, stream=None):
         """Enqueues a plain search of device tensor `queries` (None: the points)
@@ -386,12 +400,14 @@ class Context:
         return idx, d2
 
     def search_partitioned(self, queries=None, radius=2.0, k=8, with_d2=True,
-                           out_device=None):
+                           out_device=None, out=None):
         return self._run(self._L.rtnn_search_partitioned, queries, radius, k, with_d2,
-                         out_device)
+                         out_device, out=out)
 
-    def search_learned(self, queries=None, radius=10.0, k=16, with_d2=True, out_device=None):
-        return self._run(self._L.rtnn_search_learned, queries, radius, k, with_d2, out_device)
+    def search_learned(self, queries=None, radius=10.0, k=16, with_d2=True, out_device=None,
+                       out=None):
+        return self._run(self._L.rtnn_search_learned, queries, radius, k, with_d2, out_device,
+                         out=out)
 
     def search_stream(self, queries, mode="knn", method="plain", radius=2.0, k=50, chunk=0,
                       with_d2=True, path=None):

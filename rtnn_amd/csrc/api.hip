@@ -396,15 +396,16 @@ int build_grid(rtnn_ctx ctx, float cell) {
   ctx->ncells = grid_cells(g);
   const uint64_t n = ctx->n;
   CK(ctx->grids.ensure(sizeof(GridDesc)));
-  CK(hipMemcpyAsync(ctx->grids.p, &ctx->hgrid, sizeof(GridDesc), hipMemcpyHostToDevice, ctx->st));
-  CK(hipEventRecord(ctx->ev_desc, ctx->st));
   CK(ctx->keys.ensure(4 * n));
   CK(ctx->vals.ensure(4 * n));
   CK(ctx->cell_start.ensure(4 * (ctx->ncells + 1)));
   CK(ctx->spts.ensure(16 * std::max<uint64_t>(n, 1) + kSptsPadBytes));
   CK(ctx->tmp.ensure(4 * std::max(radix_tmp_words(n), cell_bounds_tmp_words(ctx->ncells))));
-  CK(cell_keys(ctx->pts.as<float>(), n, ctx->grids.as<GridDesc>(), nullptr, 1,
-               ctx->keys.as<uint32_t>(), ctx->vals.as<uint32_t>(), nullptr, ctx->st));
+  // The descriptor goes to the device as the key kernel's argument (no
+  // pageable copy, which costs the host more than a launch).
+  CK(cell_keys_single(ctx->pts.as<float>(), n, ctx->hgrid, ctx->grids.as<GridDesc>(),
+                      ctx->keys.as<uint32_t>(), ctx->vals.as<uint32_t>(), ctx->st));
+  CK(hipEventRecord(ctx->ev_desc, ctx->st));
   CK(radix_sort_pairs(ctx->keys.as<uint32_t>(), ctx->vals.as<uint32_t>(), n,
                       bits_for(ctx->ncells - 1), ctx->tmp.as<uint32_t>(), ctx->st,
                       &ctx->sort_passes));

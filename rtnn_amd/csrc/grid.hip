@@ -292,6 +292,30 @@ hipError_t cell_keys(const float* xyz, uint64_t n, const GridDesc* grids,
   return hipGetLastError();
 }
 
+// One cloud: the grid descriptor comes by value (no host-to-device copy before
+// the first kernel) and thread 0 stores it to gstore for the kernels after it.
+__global__ __launch_bounds__(256) void k_cell_keys1(const float* __restrict__ xyz, uint64_t n,
+                                                    const GridDesc g, GridDesc* __restrict__ gstore,
+                                                    uint32_t* __restrict__ keys,
+                                                    uint32_t* __restrict__ vals) {
+  const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i == 0) *gstore = g;
+  if (i >= n) return;
+  const int cx = cell_of(xyz[3 * i], g.mn[0], g.inv, g.dim[0]);
+  const int cy = cell_of(xyz[3 * i + 1], g.mn[1], g.inv, g.dim[1]);
+  const int cz = cell_of(xyz[3 * i + 2], g.mn[2], g.inv, g.dim[2]);
+  keys[i] = g.cell_base + ((uint32_t)cz * (uint32_t)g.dim[1] + (uint32_t)cy) * (uint32_t)g.dim[0] +
+            (uint32_t)cx;
+  vals[i] = (uint32_t)i;
+}
+
+hipError_t cell_keys_single(const float* xyz, uint64_t n, const GridDesc& g, GridDesc* gstore,
+                            uint32_t* keys, uint32_t* vals, hipStream_t st) {
+  const uint32_t blocks = (uint32_t)std::max<uint64_t>(1, (n + 255) / 256);
+  hipLaunchKernelGGL(k_cell_keys1, dim3(blocks), dim3(256), 0, st, xyz, n, g, gstore, keys, vals);
+  return hipGetLastError();
+}
+
 // cell_start[c] = first sorted position whose key (>> shift) is >= c, for c
 // in [0, ncells] (entry ncells holds n), so a cell's points are
 // [cell_start[c], cell_start[c+1]) and no counts, atomics or scan are needed.

@@ -37,6 +37,9 @@ hipError_t plan_batched(const float* bboxes, const uint64_t* offsets, uint32_t n
 hipError_t cell_keys(const float* xyz, uint64_t n, const GridDesc* grids,
                      const uint64_t* offsets, uint32_t nclouds, uint32_t* keys,
                      uint32_t* vals, uint32_t* counts, hipStream_t st);
+// One cloud with its descriptor by value; *gstore receives it (device).
+hipError_t cell_keys_single(const float* xyz, uint64_t n, const GridDesc& g, GridDesc* gstore,
+                            uint32_t* keys, uint32_t* vals, hipStream_t st);
 // cell_start[0..ncells] from the sorted keys (first position with
 // (key >> shift) >= c).  tmp: cell_bounds_tmp_words(ncells) words, 8-byte
 // aligned.

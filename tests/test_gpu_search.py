@@ -157,6 +157,27 @@ def test_spheres_knn16(ctx):
     check(ctx, pts, qs, "knn", 10.0, 16)
 
 
+def test_caller_owned_outputs(ctx):
+    import torch
+    rng = np.random.default_rng(31)
+    pts = rng.uniform(0, 10, (5000, 3)).astype(np.float32)
+    ctx.set_points(torch.from_numpy(pts).cuda())
+    ref_i, ref_d, _ = ctx.search(None, mode="knn", radius=1.0, k=8)
+    oi = torch.full((5000, 8), 7, dtype=torch.int32, device="cuda")
+    od = torch.zeros((5000, 8), dtype=torch.float32, device="cuda")
+    for _ in range(2):  # the same buffers, written again
+        i, d, _ = ctx.search(None, mode="knn", radius=1.0, k=8, out=(oi, od))
+        assert i.data_ptr() == oi.data_ptr() and d.data_ptr() == od.data_ptr()
+        assert torch.equal(i, ref_i) and torch.equal(d, ref_d)
+    hi = np.empty((5000, 8), np.int32)
+    i, d, _ = ctx.search(None, mode="knn", radius=1.0, k=8, out=(hi, None))
+    assert d is None and np.array_equal(hi, ref_i.cpu().numpy())
+    with pytest.raises(ValueError):
+        ctx.search(None, mode="knn", radius=1.0, k=8, out=(oi[:, :4], od))
+    with pytest.raises(ValueError):
+        ctx.search(None, mode="knn", radius=1.0, k=8, out=(oi, od.cpu()))
+
+
 @pytest.mark.parametrize("opts", [{}, {"queries_per_wave": 4, "lds_tile": 1}])
 def test_edge_cases(ctx, opts):
     ctx.set_options(**opts)
