@@ -93,6 +93,10 @@ typedef struct {
                              query (2, 4 or 8 split the chunks and points of
                              every leaf; 0 or 1 = one lane).  2 is the fastest
                              on clustered points (Levy flights)               */
+  int   tree_fallback;    /* hierarchical-grid KNN with the octree walk (k <= 64):
+                             0 = the rows short at their proposed radius are
+                             searched once more at r; 1 = through the radius-
+                             growth stages (as on raster grids)               */
 } rtnn_options;
 
 /* Filled by search entry points when a non-NULL pointer is passed. */

@@ -55,15 +55,17 @@ class Options(ctypes.Structure):
                 ("tree_leaf", ctypes.c_int),
                 ("bucket_ratio", ctypes.c_float),
                 ("learned_cap", ctypes.c_int),
-                ("tree_lanes", ctypes.c_int)]
+                ("tree_lanes", ctypes.c_int),
+                ("tree_fallback", ctypes.c_int)]
 
 
 def _options_struct(cell_factor=0.0, queries_per_wave=0, query_order=0, time_stages=0,
                     max_cells=0, grid_levels=0, level_alpha=0.0, group_buffer=0, lds_tile=0,
-                    tree_leaf=0, bucket_ratio=0.0, learned_cap=0, tree_lanes=0):
+                    tree_leaf=0, bucket_ratio=0.0, learned_cap=0, tree_lanes=0,
+                    tree_fallback=0):
     return Options(cell_factor, queries_per_wave, query_order, time_stages, max_cells,
                    grid_levels, level_alpha, group_buffer, lds_tile, tree_leaf, bucket_ratio,
-                   learned_cap, tree_lanes)
+                   learned_cap, tree_lanes, tree_fallback)
 
 
 def _options_dict(o):
@@ -266,10 +268,10 @@ class Context:
     def set_options(self, cell_factor=0.0, queries_per_wave=0, query_order=0,
                     time_stages=0, max_cells=0, grid_levels=0, level_alpha=0.0,
                     group_buffer=0, lds_tile=0, tree_leaf=0, bucket_ratio=0.0, learned_cap=0,
-                    tree_lanes=0):
+                    tree_lanes=0, tree_fallback=0):
         o = Options(cell_factor, queries_per_wave, query_order, time_stages, max_cells,
                     grid_levels, level_alpha, group_buffer, lds_tile, tree_leaf, bucket_ratio,
-                    learned_cap, tree_lanes)
+                    learned_cap, tree_lanes, tree_fallback)
         self._check(self._L.rtnn_set_options(self._h, ctypes.byref(o)))
 
     def synchronize(self):

@@ -786,6 +786,8 @@ int check_options(rtnn_ctx ctx, const rtnn_options* opt) {
     return fail(ctx, RTNN_ERR_INVALID, "bucket_ratio must be in [0, 1]");
   if (opt->learned_cap != 0 && opt->learned_cap != 1)
     return fail(ctx, RTNN_ERR_INVALID, "learned_cap must be 0 or 1");
+  if (opt->tree_fallback != 0 && opt->tree_fallback != 1)
+    return fail(ctx, RTNN_ERR_INVALID, "tree_fallback must be 0 or 1");
   if (opt->tree_lanes < 0 || opt->tree_lanes > 8 || (opt->tree_lanes & (opt->tree_lanes - 1)))
     return fail(ctx, RTNN_ERR_INVALID, "tree_lanes must be 0, 1, 2, 4 or 8");
   return RTNN_OK;
@@ -1215,8 +1217,10 @@ int proposal_search(rtnn_ctx ctx, Proposal kind, const float* qxyz, uint64_t nq,
   // much at a grown radius as at r -- its limit, not its radius, bounds the
   // walk -- so its short rows go straight to the last stage: on the
   // Levy-flight KNN 151 short rows took 27 ms through five stages, since
-  // each stage waits for its slowest walk.
-  const bool tree_fallback = ctx->hlevels > 0 && k <= 64;
+  // each stage waits for its slowest walk.  Where many rows fall back at
+  // radii well below r the stages can still pay (option tree_fallback = 1,
+  // which the autotuner tries).
+  const bool tree_fallback = ctx->hlevels > 0 && k <= 64 && ctx->opt.tree_fallback == 0;
   for (int j = tree_fallback ? kGrowStages : 0; j <= kGrowStages; ++j) {
     SearchArgs s = base_args(ctx, qproc, nq, r, k, o);
     s.qsel = tree_fallback ? lists[0] : lists[j & 1];
@@ -2339,6 +2343,29 @@ int rtnn_autotune(rtnn_ctx ctx, const float* qxyz, uint64_t nq, int on_device, i
       if (ms < best_t) {
         best_t = ms;
         best_opt.learned_cap = ctx->opt.learned_cap;
+      }
+      ctx->opt = best_opt;
+    }
+    if (best_opt.grid_levels >= 2 && best_opt.queries_per_wave == 0 && k <= 64) {
+      // The octree walk's short rows: at r at once, or through the growth stages.
+      ctx->opt.tree_fallback = 1 - best_opt.tree_fallback;
+      double ms = 0.0;
+      const int rc = time_candidate(
+          [&](rtnn_stats* st) {
+            return proposal_search(ctx, method == RTNN_METHOD_LEARNED ? Proposal::kLearned
+                                                                       : Proposal::kPartition,
+                                   qxyz, nq, on_device, r, k, out_idx, out_d2, 1, st);
+          },
+          ms);
+      if (rc != RTNN_OK) {
+        ctx->opt = base;
+        return rc;
+      }
+      tune_log("autotune %s: tree_fallback %d: %.4f ms", method_name(method),
+               ctx->opt.tree_fallback, ms);
+      if (ms < best_t) {
+        best_t = ms;
+        best_opt.tree_fallback = ctx->opt.tree_fallback;
       }
       ctx->opt = best_opt;
     }

@@ -215,6 +215,37 @@ def test_octree_walk_with_several_lanes_exact(ctx, lanes):
         ctx.set_options()
 
 
+@pytest.mark.parametrize("lanes", [1, 2])
+def test_octree_growth_stage_fallback_exact(ctx, lanes):
+    # tree_fallback = 1: the octree walk's short rows go through the radius-
+    # growth stages instead of straight to r; rows still equal the plain KNN,
+    # with the forced-fallback network (almost every row short) and a random
+    # one, partitioned too.
+    pts = rtnn_amd.gen_gaussian_mixture(2, 20, 100000)
+    qs = np.concatenate([rtnn_amd.gen_gaussian_mixture(2, 21, 20000),
+                         [[1e4, 0, 0], [-50, -50, -50]]]).astype(np.float32)
+    ctx.set_points(pts)
+    try:
+        for blob_w in (np.zeros(M.NP, np.float32),
+                       np.random.default_rng(4).normal(0, 0.2, M.NP).astype(np.float32)):
+            ctx.set_mlp_weights(np.concatenate([blob_w, [0.25, 1.0]]).astype(np.float32))
+            ctx.set_options(grid_levels=5, level_alpha=1.0, queries_per_wave=0,
+                            tree_lanes=lanes, tree_fallback=1)
+            for k in (8, 32):
+                st = _plain_vs(ctx, qs, 20.0, k, ctx.search_learned)
+                assert st.search_launches == 6
+                if not blob_w.any():  # o = 0: r' at its floor, (almost) every row short
+                    assert st.fallback_queries > 0.5 * len(qs)
+            _plain_vs(ctx, qs, 20.0, 8, ctx.search_partitioned)
+        ctx.set_options(grid_levels=5, level_alpha=1.0, queries_per_wave=0, tree_lanes=lanes)
+        st = _plain_vs(ctx, qs, 20.0, 8, ctx.search_learned)
+        assert st.search_launches == 2
+        with pytest.raises(rtnn_amd.RtnnError, match="tree_fallback"):
+            ctx.set_options(tree_fallback=2)
+    finally:
+        ctx.set_options()
+
+
 def test_autotune_picks_a_valid_config(ctx):
     rng = np.random.default_rng(21)
     pts = rng.uniform(0, 40, (50000, 3)).astype(np.float32)
