@@ -617,9 +617,8 @@ def main():
                          "a 1 GiB gather of 128-byte blocks in random order)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: every box in this setup has a single MI355X and the "
-              "search needs no collective (queries shard with replicated points); run --gpus 1.",
-              file=sys.stderr)
+        print("bench.py measures one GPU (the search needs no collective: queries shard over "
+              "GPUs with the points replicated on each); run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     # one stderr line per autotuner candidate (a long sweep shows progress)
