@@ -30,7 +30,7 @@ SAN = ["-fsanitize=address,undefined", "-fno-sanitize-recover=all", "-fno-omit-f
        "-g", "-O1", "-ffp-contract=off"]
 ENV = dict(os.environ, ASAN_OPTIONS="halt_on_error=1:detect_leaks=1:abort_on_error=0",
            UBSAN_OPTIONS="halt_on_error=1:print_stacktrace=1")
-KNN = 8  # compile-time K of the CLI's KNN mode in this build
+KNN = 8  # compile-time K of the sanitized CLI build (RTNN_KNN)
 
 pytestmark = pytest.mark.skipif(shutil.which("gcc") is None or shutil.which("g++") is None,
                                 reason="no host compiler")
@@ -168,7 +168,7 @@ def test_cli_host_paths_under_sanitizers(bins, tmp_path):
     (tmp_path / "short.bin").write_bytes(b"\0" * 100)
     assert _run([cli, "-f", str(pf), "-sm", "knn", "--learned", "1", "--mlp-in",
                  str(tmp_path / "short.bin")]).returncode == 1
-    # the advanced flags the driver accepts
+    # the advanced flags the CLI accepts (scheduling flags of the reference)
     r = _run([cli, "-f", str(pf), "-a", "1", "-s", "2", "-sg", "2", "-egs", "0", "-crs", "2",
               "-gmu", "1", "-mc", "2", "-i", "0", "-m", "0", "-df", "1", "-g", "1", "-ps", "1",
               "-qs", "1", "-cr", "2", "--cell-factor", "1.5", "--qpw", "16", "-p", "0"])
