@@ -124,6 +124,25 @@ def test_learned_forced_fallback_and_no_fallback(ctx):
     assert st.fallback_queries == 0
 
 
+@pytest.mark.parametrize("k", [65, 100, 300])
+def test_proposal_methods_beyond_k64(ctx, k):
+    # k above the register top-k: the default partitioned configuration (a
+    # 5-level hierarchical grid) and the learned one search with 16- or
+    # 64-lane groups, stage 1 and fallback; rows equal the plain KNN.
+    pts = rtnn_amd.gen_gaussian_mixture(2, 20, 60000)
+    qs = rtnn_amd.gen_gaussian_mixture(2, 21, 5000)
+    ctx.set_points(pts)
+    w = np.random.default_rng(6).normal(0, 0.2, M.NP).astype(np.float32)
+    ctx.set_mlp_weights(np.concatenate([w, [0.25, 1.0]]).astype(np.float32))
+    try:
+        for opts in ({}, {"grid_levels": 1, "cell_factor": 4.0}):
+            ctx.set_options(**opts)
+            _plain_vs(ctx, qs, 20.0, k, ctx.search_partitioned)
+            _plain_vs(ctx, qs, 20.0, k, ctx.search_learned)
+    finally:
+        ctx.set_options()
+
+
 @pytest.mark.parametrize("k", [1, 8, 16, 32])
 def test_partitioned_equals_plain(ctx, k):
     pts = rtnn_amd.gen_gaussian_mixture(2, 20, 100000)
