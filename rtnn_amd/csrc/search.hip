@@ -822,6 +822,22 @@ constexpr int finish_chunk() {
   return ch <= 12 ? ch : kRankChunk;
 }
 
+// One ranking round of group_finish: entries b0 + l + G*i (i < C) of the
+// buffer ranked and those of rank < k written to the row.
+template <int G, int C>
+__device__ inline void finish_round(const uint64_t* buf, uint32_t cnt, uint32_t b0, int l,
+                                    uint32_t k, uint32_t base, int32_t* oi, float* od) {
+  uint64_t kv[C];
+  uint32_t rk[C];
+  rank_chunk<G, C>(buf, cnt, b0, l, kv, rk);
+#pragma unroll
+  for (int i = 0; i < C; ++i)
+    if (rk[i] < k) {
+      oi[rk[i]] = (int32_t)(key_idx(kv[i]) - base);
+      if (od) od[rk[i]] = key_d2(kv[i]);
+    }
+}
+
 // Writes a group's row: the buffer's entries by rank (the k smallest), then
 // empty slots; appends the query to the short list if it has fewer than k
 // and was searched below r (a row searched at r is final: it holds every
@@ -843,16 +859,30 @@ __device__ inline void group_finish(const SearchArgs& a, uint64_t* buf, uint32_t
   }
   int32_t* oi = a.out_idx + (uint64_t)row * k;
   float* od = a.out_d2 ? a.out_d2 + (uint64_t)row * k : nullptr;
-  for (uint32_t b0 = 0; b0 < cnt; b0 += G * CH) {
-    uint64_t kv[CH];
-    uint32_t rk[CH];
-    rank_chunk<G, CH>(buf, cnt, b0, l, kv, rk);
-#pragma unroll
-    for (int i = 0; i < CH; ++i)
-      if (rk[i] < k) {
-        oi[rk[i]] = (int32_t)(key_idx(kv[i]) - base);
-        if (od) od[rk[i]] = key_d2(kv[i]);
-      }
+  // Ranking rounds: each ranks G * C entries (C per lane) against the whole
+  // buffer, so a round wider than the entries left ranks empty slots.  The
+  // round width is chosen per wave (one ballot each, so every group of the
+  // wave runs the same code): the narrowest of CH, CH/2, CH/4 and 1 entries
+  // per lane that covers the largest remainder in the wave.  (Range rows of
+  // 33 hits in 4-lane groups took a second round of 32 slots for one entry.)
+  constexpr int C2 = (CH + 1) / 2, C3 = (CH + 3) / 4;
+  for (uint32_t b0 = 0;;) {
+    const uint32_t rem = cnt > b0 ? cnt - b0 : 0u;
+    if (__ballot(rem > (uint32_t)(G * C2))) {
+      finish_round<G, CH>(buf, cnt, b0, l, k, base, oi, od);
+      b0 += G * CH;
+    } else if (__ballot(rem > (uint32_t)(G * C3))) {
+      finish_round<G, C2>(buf, cnt, b0, l, k, base, oi, od);
+      b0 += G * C2;
+    } else if (__ballot(rem > (uint32_t)G)) {
+      finish_round<G, C3>(buf, cnt, b0, l, k, base, oi, od);
+      b0 += G * C3;
+    } else if (__ballot(rem > 0u)) {
+      finish_round<G, 1>(buf, cnt, b0, l, k, base, oi, od);
+      b0 += G;
+    } else {
+      break;
+    }
   }
   for (uint32_t s = cnt + (uint32_t)l; s < k; s += G) {
     oi[s] = -1;
