@@ -355,6 +355,13 @@ def test_config3_full_size_sampled(ctx):
     valid = idx >= 0
     assert (valid[:, 1:] <= valid[:, :-1]).all()  # sentinels only at the tail
     assert (d2[valid] > 0).all() and (d2[valid] < np.float32(100)).all()
+    # the learned-radius search (the configuration the autotuner picks for
+    # this workload) equals the plain one on every row
+    ctx.mlp_train(10.0, 16, n_samples=1 << 21, steps=2000, batch=4096, lr=1e-3, tau=0.9, seed=1)
+    ctx.set_options(cell_factor=2.0, queries_per_wave=32, group_buffer=2, grid_levels=1)
+    li, ld, lst = ctx.search_learned(qs, radius=10.0, k=16)
+    assert np.array_equal(li, idx) and np.array_equal(ld, d2)
+    assert 0 < lst.fallback_queries < len(qs) // 5
 
 
 def test_config2_full_size(ctx):
