@@ -338,7 +338,9 @@ class Context:
             ok = tuple(idx.shape) == (nq, k) and str(idx.dtype).endswith("int32") and (
                 d2 is None or (tuple(d2.shape) == (nq, k) and str(d2.dtype).endswith("float32")
                                and (_is_torch(d2) and d2.is_cuda) == dev))
-            if not ok or not (idx.is_contiguous() if _is_torch(idx) else idx.flags.c_contiguous):
+            contig = lambda t: t is None or (t.is_contiguous() if _is_torch(t)
+                                             else t.flags.c_contiguous)
+            if not ok or not contig(idx) or not contig(d2):
                 raise ValueError(f"out must be contiguous ([{nq}, {k}] int32, [{nq}, {k}] float32 "
                                  "or None) on one device")
             out_device = dev

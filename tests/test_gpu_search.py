@@ -176,6 +176,9 @@ def test_caller_owned_outputs(ctx):
         ctx.search(None, mode="knn", radius=1.0, k=8, out=(oi[:, :4], od))
     with pytest.raises(ValueError):
         ctx.search(None, mode="knn", radius=1.0, k=8, out=(oi, od.cpu()))
+    with pytest.raises(ValueError):  # a transposed view: right shape, not contiguous
+        ctx.search(None, mode="knn", radius=1.0, k=8,
+                   out=(oi, torch.zeros((8, 5000), dtype=torch.float32, device="cuda").t()))
 
 
 @pytest.mark.parametrize("opts", [{}, {"queries_per_wave": 4, "lds_tile": 1}])
