@@ -291,8 +291,24 @@ __device__ inline float box_d2(const float4& q, const GridDesc& g0, float c, flo
   return dx * dx + dy * dy + dz * dz;
 }
 
+// Waves per SIMD asked of the compiler for the octree walks: the walk is
+// bound by the latency of its dependent loads, so residency pays more than
+// registers, up to a few spilled ones.  Levy-flight KNN (30M points, 30M
+// uniform queries, r = 100), K = 32, two lanes: 908 ms at 137 VGPRs (three
+// waves per SIMD), 756 ms at 128 (four; 48 bytes spilled), 726 ms at 96
+// (five; 176 bytes); one lane: 1094 ms at 193 VGPRs (two waves), 802 ms at
+// 168 (three), 993 ms at 128 (four; 264 bytes).  With these targets, one /
+// two lanes: K = 8 363 -> 324 / 328 -> 328 ms, K = 16 544 -> 442 / 473 ->
+// 411 ms, K = 32 1092 -> 810 / 911 -> 724 ms, rows unchanged.  Beyond K = 32
+// no target is set (1: the compiler's own choice).
 template <int KC>
-__global__ __launch_bounds__(kTileQ) void k_topk_tree(const SearchArgs a) {
+constexpr int treeg_waves() { return KC <= 32 ? 5 : 1; }
+template <int KC>
+constexpr int tree_waves() { return KC <= 32 ? 3 : 1; }
+
+template <int KC>
+__global__ __launch_bounds__(kTileQ) __attribute__((amdgpu_waves_per_eu(tree_waves<KC>())))
+void k_topk_tree(const SearchArgs a) {
   __shared__ uint32_t s_base[8];
   if (threadIdx.x < 8) s_base[threadIdx.x] = (int)threadIdx.x < a.nlevels ?
                                                a.grids[threadIdx.x].cell_base : 0u;
@@ -462,7 +478,8 @@ __device__ inline uint64_t group_min_u64(uint64_t v) {
 }
 
 template <int KC, int G>
-__global__ __launch_bounds__(kTileQ) void k_topk_treeg(const SearchArgs a) {
+__global__ __launch_bounds__(kTileQ) __attribute__((amdgpu_waves_per_eu(treeg_waves<KC>())))
+void k_topk_treeg(const SearchArgs a) {
   __shared__ uint32_t s_base[8];
   if (threadIdx.x < 8) s_base[threadIdx.x] = (int)threadIdx.x < a.nlevels ?
                                                a.grids[threadIdx.x].cell_base : 0u;
