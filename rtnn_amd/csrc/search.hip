@@ -140,8 +140,16 @@ __device__ inline void topk_insert(uint64_t (&list)[KC], uint64_t key) {
   list[0] = list[0] > key ? key : list[0];
 }
 
+// Waves per SIMD asked of the compiler (as for the octree walks: a latency-
+// bound walk, residency before registers).  Spheres KNN, K = 16 (10M x 10M,
+// r = 10): 10.07 ms at 117 VGPRs (four waves), 8.60 ms at 90 (five, no
+// spills); Gaussian-mixture KNN, K = 8: 10.01 / 10.00 ms.
+template <int KC>
+constexpr int thread_waves() { return KC <= 8 ? 8 : KC <= 16 ? 5 : KC <= 32 ? 3 : 1; }
+
 template <int KC, bool KNN, bool HIER>
-__global__ __launch_bounds__(kTileQ) void k_topk_thread(const SearchArgs a) {
+__global__ __launch_bounds__(kTileQ) __attribute__((amdgpu_waves_per_eu(thread_waves<KC>())))
+void k_topk_thread(const SearchArgs a) {
   const uint32_t nq = launch_nq(a);
   for (TileIter it(a, nq); it.t < it.end; it.t += it.step) {
     uint32_t q0, q1, gi;
