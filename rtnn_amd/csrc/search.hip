@@ -466,9 +466,9 @@ void k_topk_tree(const SearchArgs a) {
 // unique); the row is the first k taken.  Same result as k_topk_tree.
 // On the Levy-flight KNN (30M x 30M, leaves of 65536 points; builds without
 // occupancy targets: two waves per SIMD for one lane, three for several) one
-// lane took 1080 ms, two lanes 900 ms,
-// four 1062 ms, eight 1245 ms: beyond two, the walk every lane repeats (cell
-// bounds, boxes) outweighs the split chunks.
+// lane took 1080 ms, two lanes 900 ms, four 1062 ms, eight 1245 ms: beyond
+// two, the walk every lane repeats (cell bounds, boxes) outweighs the split
+// chunks.
 
 template <int G>
 __device__ inline float group_min_f(float v) {
