@@ -2016,7 +2016,11 @@ int time_candidate(Run run, double& best_ms) {
   rtnn_stats st{};
   int rc = run(&st);
   if (rc != RTNN_OK) return rc;
-  const int reps = std::max(1, std::min(8, (int)(4.0 / std::max(st.ms_total, 1e-3))));
+  // the fastest of about 12 ms of runs, at least 3 for candidates under
+  // 50 ms (one 4 ms sample left near-equal configurations to chance), one
+  // for longer ones
+  const double t = std::max(st.ms_total, 1e-3);
+  const int reps = t < 50.0 ? std::max(3, std::min(8, (int)(12.0 / t))) : 1;
   best_ms = 1e300;
   for (int i = 0; i < reps; ++i) {
     rc = run(&st);
