@@ -305,14 +305,17 @@ __device__ inline float box_d2(const float4& q, const GridDesc& g0, float c, flo
 // uniform queries, r = 100), K = 32, two lanes: 908 ms at 137 VGPRs (three
 // waves per SIMD), 756 ms at 128 (four; 48 bytes spilled), 726 ms at 96
 // (five; 176 bytes); one lane: 1094 ms at 193 VGPRs (two waves), 802 ms at
-// 168 (three), 993 ms at 128 (four; 264 bytes).  With these targets, one /
-// two lanes: K = 8 363 -> 324 / 328 -> 328 ms, K = 16 544 -> 442 / 473 ->
-// 411 ms, K = 32 1092 -> 810 / 911 -> 724 ms, rows unchanged.  Beyond K = 32
-// no target is set (1: the compiler's own choice).
+// 168 (three), 993 ms at 128 (four; 264 bytes).  K = 16, two lanes: 473 ms
+// at 105 VGPRs (four waves), 411 at 96 (five), 379 at 80 (six); one lane:
+// 544 ms at 154 (three), 381 at 95 (five), 351 at 80 (six).  K = 8, two
+// lanes: 328 ms at 88 (five), 302 at 80 (six), 283 at 64 (eight); one lane:
+// 363 ms at 106 (four), 307 at 80 (six), 295 at 64 (eight).  Six waves at
+// K = 32 spill 248 bytes and take 1402 ms.  Rows are unchanged throughout.
+// Beyond K = 32 no target is set (1: the compiler's own choice).
 template <int KC>
-constexpr int treeg_waves() { return KC <= 32 ? 5 : 1; }
+constexpr int treeg_waves() { return KC <= 8 ? 8 : KC <= 16 ? 6 : KC <= 32 ? 5 : 1; }
 template <int KC>
-constexpr int tree_waves() { return KC <= 32 ? 3 : 1; }
+constexpr int tree_waves() { return KC <= 8 ? 8 : KC <= 16 ? 6 : KC <= 32 ? 3 : 1; }
 
 template <int KC>
 __global__ __launch_bounds__(kTileQ) __attribute__((amdgpu_waves_per_eu(tree_waves<KC>())))
