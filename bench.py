@@ -93,10 +93,10 @@ def workload(name):
                     # bucket, timed on 3M of the queries (about 0.1 s each)
                     proposal_tune={"sample": 3_000_000, "margins": [1.0, 1.5, 2.0, 3.0],
                                    "buckets": [0.0, 0.8]},
-                    # the octree walk of a 7-level hierarchical grid (fixed
+                    # the octree walk of a 6-level hierarchical grid (fixed
                     # configuration, not tuned): leaves of up to 65536
                     # points, scanned chunk by chunk, two lanes per query
-                    options={"grid_levels": 7, "queries_per_wave": 0, "tree_leaf": 65536,
+                    options={"grid_levels": 6, "queries_per_wave": 0, "tree_leaf": 65536,
                              "tree_lanes": 2},
                     cfg={"workload": "KNN K=32 r=100: 30M Levy-flight points (Pareto alpha 1.5, "
                                      "min step 0.01, periodic [0,1000]^3), 30M uniform queries",

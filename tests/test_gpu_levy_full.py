@@ -3,7 +3,7 @@
 Workload: 30,000,000 FP32 points from a Levy flight (Pareto step lengths,
 alpha 1.5, min step 0.01, isotropic directions, periodic wrap into
 [0, 1000)^3), 30,000,000 queries uniform in the cube, K = 32, r = 100.  Both
-proposal methods run as octree walks on a 7-level hierarchical grid (leaves of
+proposal methods run as octree walks on a 6-level hierarchical grid (leaves of
 up to 65536 points); the learned one uses a network trained on 2^21 of the
 queries.
 
@@ -72,7 +72,7 @@ def test_levy_knn_full_size(ctx):
     dp = torch.from_numpy(pts).cuda()
     dq = torch.from_numpy(qs).cuda()
     ctx.set_points(dp)
-    ctx.set_options(grid_levels=7, queries_per_wave=0, tree_leaf=65536, tree_lanes=2)
+    ctx.set_options(grid_levels=6, queries_per_wave=0, tree_leaf=65536, tree_lanes=2)
     try:
         t = time.perf_counter()
         pi, pd, st = ctx.search_partitioned(dq, radius=R, k=K, out_device=True)
